@@ -1,0 +1,1401 @@
+// TEST INFRASTRUCTURE ONLY — the FP64 CPU oracle for ImpressionFEA's 3D rigid-viscoplastic solve step.
+//
+// A scalar restatement of src/ImpressionFEA.Core/ThreeD/Forging3DSolver.cs (and TetMesh.cs, FlowStressLaw.cs,
+// Solver/Linear/ISpdSolver.cs, Solver/Principal.cs) used as the checker by tests/, by __graft_entry__.smoke() and by
+// bench.py's cpu_baseline leg. It is never linked into the product library. Each function cites the reference lines
+// it follows. Element work runs on OpenMP threads into per-element buffers and is summed serially in element order,
+// as the reference does (Forging3DSolver.cs:1133-1171), so results do not depend on the thread count.
+//
+// Linear solver behind the ISpdSolver slot: an FP64 left-looking sparse Cholesky (symbolic structure from the
+// elimination tree, ordering = caller permutation or an exact minimum-degree ordering on the node graph), or Intel
+// MKL PARDISO through its documented C prototype (oracle/mkl_pardiso.cpp), as the reference's MklPardisoSolver does.
+//
+// Parity status (see the tests): every formula here is pinned by a closed form, a finite-difference check, an
+// invariant or brute force, except the minimum-degree ordering, which is NOT CSparse's AMD (CSparse.NET is absent),
+// so the reference's own ordering is "parity unpinned"; orderings are compared by feeding one permutation to both sides.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "pardiso_dl.h"
+
+namespace {
+
+struct V3 {
+  double x = 0, y = 0, z = 0;
+  V3() = default;
+  V3(double a, double b, double c) : x(a), y(b), z(c) {}
+  V3 operator+(const V3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+  V3 operator-(const V3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+  V3 operator-() const { return {-x, -y, -z}; }
+  double dot(const V3& o) const { return x * o.x + y * o.y + z * o.z; }
+  V3 cross(const V3& o) const { return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x}; }
+  double len() const { return std::sqrt(x * x + y * y + z * z); }
+  V3 normalized() const { double l = len(); return l > 0 ? V3(x / l, y / l, z / l) : *this; }
+};
+inline V3 operator*(double s, const V3& v) { return {s * v.x, s * v.y, s * v.z}; }
+
+// FlowStressLaw.Stress (Materials/FlowStressLaw.cs:8-9)
+struct Law {
+  double C, N, M, E0;
+  double stress(double strain, double rate) const {
+    return C * std::pow(E0 + std::max(strain, 0.0), N) * std::pow(std::max(rate, 1e-12), M);
+  }
+};
+
+// Principal.Max3 (Solver/Principal.cs:7-19)
+double max3(double xx, double yy, double zz, double xy, double yz, double zx) {
+  double p1 = xy * xy + yz * yz + zx * zx;
+  double q = (xx + yy + zz) / 3;
+  if (p1 < 1e-30 * (xx * xx + yy * yy + zz * zz + 1e-300)) return std::max(xx, std::max(yy, zz));
+  double p2 = (xx - q) * (xx - q) + (yy - q) * (yy - q) + (zz - q) * (zz - q) + 2 * p1;
+  double p = std::sqrt(p2 / 6);
+  double bxx = (xx - q) / p, byy = (yy - q) / p, bzz = (zz - q) / p, bxy = xy / p, byz = yz / p, bzx = zx / p;
+  double detB = bxx * (byy * bzz - byz * byz) - bxy * (bxy * bzz - byz * bzx) + bzx * (bxy * byz - byy * bzx);
+  double r = std::min(1.0, std::max(-1.0, detB / 2));
+  double phi = std::acos(r) / 3;
+  return q + 2 * p * std::cos(phi);
+}
+
+// ---------------------------------------------------------------------------------------------- rigid dies
+// Analytic height-field dies. side 0 = bottom (metal above), 1 = top (metal below). kind 0: flat plane at z = p[0].
+// kind 1: top die with a hexagonal cavity: land at z = p[0] (the flash thickness t at closure), cavity ceiling at
+// p[0] + p[1], across-flats p[2], wall transition width p[3], land width p[4], gutter depth p[5].
+// Clearance follows HeightFieldDie.Clearance (HeightFieldDie.cs:130-134); Project is the vertical projection onto
+// the surface with the normal of the analytic surface pointing out of the die (the reference searches the closest
+// point on a triangulated height field, HeightFieldDie.cs:157-191; on flat regions both agree).
+struct Die {
+  int side = 0, kind = 0;
+  double p[8] = {0};
+  static double smooth(double u) { u = std::min(1.0, std::max(0.0, u)); return u * u * (3 - 2 * u); }
+  double height(double x, double y) const {
+    if (kind == 0) return p[0];
+    const double s3 = std::sqrt(3.0) / 2;
+    double d = std::max(std::fabs(x), std::max(std::fabs(0.5 * x + s3 * y), std::fabs(-0.5 * x + s3 * y))) - 0.5 * p[2];
+    double h = p[0] + p[1] * smooth((-d + 0.5 * p[3]) / p[3]);
+    h += p[5] * smooth((d - p[4]) / p[3]);
+    return h;
+  }
+  double clearance(const V3& q, double off) const {
+    double h = height(q.x, q.y) + off;
+    return side == 0 ? q.z - h : h - q.z;
+  }
+  V3 project(const V3& q, double off, V3* nrm) const {
+    const double dd = 1e-5;
+    double hx = (height(q.x + dd, q.y) - height(q.x - dd, q.y)) / (2 * dd);
+    double hy = (height(q.x, q.y + dd) - height(q.x, q.y - dd)) / (2 * dd);
+    V3 up = V3(-hx, -hy, 1.0).normalized();
+    if (nrm) *nrm = side == 0 ? up : -up;
+    return V3(q.x, q.y, height(q.x, q.y) + off);
+  }
+  double surface_z(double x, double y) const { return height(x, y); }
+};
+
+enum Contact : int { FREE = 0, BOTTOM = 1, TOP = 2 };
+
+}  // namespace
+
+extern "C" {
+
+typedef struct OrcParams {
+  double C, N, M, Eps0;          // flow law
+  double friction;               // shear friction factor m (SimulationConfig3D.FrictionFactor)
+  double ram;                    // ram speed, mm/s
+  double penalty_factor;         // Forging3DSolver.PenaltyFactor (relative)
+  double newton_blend;           // Forging3DSolver.NewtonBlend
+  double tolerance;              // SimulationConfig3D.Tolerance on beta*|dv|/|v|
+  double size;                   // billet size for the reference rate (H for a vertical cylinder, Forging3DSolver.cs:145)
+  double stroke_increment;       // nominal stroke increment, mm
+  double h;                      // nominal element size, mm (contact tolerances)
+  int max_iterations;            // SimulationConfig3D.MaxIterations
+  int pcg_budget;                // Forging3DSolver.PcgBudget
+  int quadratic_ls;              // Forging3DSolver.QuadraticLineSearch
+  int adaptive_newton;           // Forging3DSolver.AdaptiveNewton
+  int extrapolate_start;         // Forging3DSolver.ExtrapolateStart
+  int sym_factor;                // DieSet3D.SymmetryFactor
+  int linear;                    // 0: oracle sparse Cholesky, 1: MKL PARDISO
+  int bottom_kind, top_kind;     // die kinds
+  double bottom_p[8], top_p[8];  // die parameters (closed position)
+} OrcParams;
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------- sparse Cholesky
+// Symbolic: elimination tree (Liu), row structures by row-subtree traversal, column counts; numeric: left-looking
+// column Cholesky in FP64. The matrix is given as a full symmetric CSC pattern (Forging3DSolver.cs:469-521), the
+// solver works on the lower triangle of P A P^T.
+struct SparseChol {
+  int n = 0;
+  std::vector<int> perm, iperm;       // perm[k] = original index of the k-th pivot
+  std::vector<int> Lp, Li;            // CSC of L (lower, diagonal first)
+  std::vector<double> Lx;
+  std::vector<int> parent;            // etree of P A P^T
+  // gather map: for each entry of L's pattern that comes from A: (A position) -> (L position)
+  std::vector<int> amap_src, amap_dst;
+  std::vector<double> work;
+
+  static void etree_of(int n, const std::vector<int>& cp, const std::vector<int>& ri, std::vector<int>& parent) {
+    // Liu's algorithm with path compression on the upper triangle (cs_etree): column j, rows i < j.
+    parent.assign(n, -1);
+    std::vector<int> anc(n, -1);
+    for (int j = 0; j < n; j++) {
+      for (int p = cp[j]; p < cp[j + 1]; p++) {
+        int i = ri[p];
+        while (i != -1 && i < j) {
+          int nx = anc[i];
+          anc[i] = j;
+          if (nx == -1) { parent[i] = j; break; }
+          i = nx;
+        }
+      }
+    }
+  }
+
+  // Analyze: permuted upper pattern, etree, structure of L.
+  int analyze(int n_, const int* colPtr, const int* rowIdx, const int* userPerm, std::vector<int>* colcount_out) {
+    n = n_;
+    perm.resize(n);
+    iperm.resize(n);
+    for (int k = 0; k < n; k++) perm[k] = userPerm ? userPerm[k] : k;
+    std::vector<int> seen(n, 0);
+    for (int k = 0; k < n; k++) {
+      if (perm[k] < 0 || perm[k] >= n || seen[perm[k]]) return -2;
+      seen[perm[k]] = 1;
+      iperm[perm[k]] = k;
+    }
+    // upper triangle of C = P A P^T by columns: C(iperm[i], iperm[j]) = A(i, j)
+    std::vector<int> cnt(n + 1, 0);
+    for (int j = 0; j < n; j++)
+      for (int p = colPtr[j]; p < colPtr[j + 1]; p++) {
+        int i = rowIdx[p];
+        int pi = iperm[i], pj = iperm[j];
+        if (pi <= pj) cnt[pj + 1]++;
+      }
+    std::vector<int> ucp(n + 1, 0);
+    for (int j = 0; j < n; j++) ucp[j + 1] = ucp[j] + cnt[j + 1];
+    std::vector<int> uri(ucp[n]), usrc(ucp[n]);
+    std::vector<int> fill(ucp.begin(), ucp.end() - 1);
+    for (int j = 0; j < n; j++)
+      for (int p = colPtr[j]; p < colPtr[j + 1]; p++) {
+        int i = rowIdx[p];
+        int pi = iperm[i], pj = iperm[j];
+        if (pi <= pj) { uri[fill[pj]] = pi; usrc[fill[pj]] = p; fill[pj]++; }
+      }
+    etree_of(n, ucp, uri, parent);
+    // row structure of L: row i of L = reach of the upper-column i entries in the etree (cs_ereach)
+    std::vector<std::vector<int>> cols(n);
+    std::vector<int> mark(n, -1);
+    for (int i = 0; i < n; i++) {
+      mark[i] = i;
+      for (int p = ucp[i]; p < ucp[i + 1]; p++) {
+        int k = uri[p];
+        while (k != -1 && k < i && mark[k] != i) {
+          cols[k].push_back(i);  // L(i, k) != 0
+          mark[k] = i;
+          k = parent[k];
+        }
+      }
+    }
+    Lp.assign(n + 1, 0);
+    for (int j = 0; j < n; j++) Lp[j + 1] = Lp[j] + 1 + (int)cols[j].size();
+    Li.resize(Lp[n]);
+    for (int j = 0; j < n; j++) {
+      Li[Lp[j]] = j;
+      std::copy(cols[j].begin(), cols[j].end(), Li.begin() + Lp[j] + 1);  // rows pushed in increasing i
+    }
+    if (colcount_out) {
+      colcount_out->resize(n);
+      for (int j = 0; j < n; j++) (*colcount_out)[j] = Lp[j + 1] - Lp[j];
+    }
+    // value map A -> L (lower of C is the transpose of the upper part)
+    amap_src.clear();
+    amap_dst.clear();
+    for (int j = 0; j < n; j++) {  // upper column j of C: entries (i, j), i <= j => L position of (j, i) in column i
+      for (int p = ucp[j]; p < ucp[j + 1]; p++) {
+        int i = uri[p];
+        const int* b = &Li[Lp[i]];
+        const int* e = &Li[Lp[i + 1]];
+        const int* f = std::lower_bound(b, e, j);
+        amap_src.push_back(usrc[p]);
+        amap_dst.push_back((int)(f - &Li[0]));
+      }
+    }
+    Lx.assign(Lp[n], 0.0);
+    work.assign(n, 0.0);
+    return 0;
+  }
+
+  // Left-looking numeric factorization: column j = A(:,j) - sum_k L(j,k) L(j:,k).
+  int factorize(const double* full) {
+    std::fill(Lx.begin(), Lx.end(), 0.0);
+    for (size_t k = 0; k < amap_src.size(); k++) Lx[amap_dst[k]] += full[amap_src[k]];
+    std::vector<int> next(n, 0);           // next[k] = position in column k of the first row >= current j
+    std::vector<int> head(n, -1), link(n, -1);  // linked lists of columns k waiting for row j
+    for (int k = 0; k < n; k++) next[k] = Lp[k] + 1;
+    std::vector<double>& w = work;
+    for (int j = 0; j < n; j++) {
+      for (int p = Lp[j]; p < Lp[j + 1]; p++) w[Li[p]] = Lx[p];
+      int k = head[j];
+      while (k != -1) {
+        int nk = link[k];
+        int pj = next[k];  // Li[pj] == j
+        double ljk = Lx[pj];
+        for (int p = pj; p < Lp[k + 1]; p++) w[Li[p]] -= Lx[p] * ljk;
+        next[k] = pj + 1;
+        if (next[k] < Lp[k + 1]) {
+          int r = Li[next[k]];
+          link[k] = head[r];
+          head[r] = k;
+        }
+        k = nk;
+      }
+      double d = w[j];
+      if (!(d > 0) || !std::isfinite(d)) return -4;
+      double ljj = std::sqrt(d);
+      Lx[Lp[j]] = ljj;
+      for (int p = Lp[j] + 1; p < Lp[j + 1]; p++) Lx[p] = w[Li[p]] / ljj;
+      for (int p = Lp[j]; p < Lp[j + 1]; p++) w[Li[p]] = 0.0;
+      if (Lp[j + 1] > Lp[j] + 1) {
+        int r = Li[Lp[j] + 1];
+        link[j] = head[r];
+        head[r] = j;
+      }
+    }
+    return 0;
+  }
+
+  void solve(const double* b, double* x) {
+    std::vector<double>& y = work;
+    for (int k = 0; k < n; k++) y[k] = b[perm[k]];
+    for (int j = 0; j < n; j++) {
+      y[j] /= Lx[Lp[j]];
+      double yj = y[j];
+      for (int p = Lp[j] + 1; p < Lp[j + 1]; p++) y[Li[p]] -= Lx[p] * yj;
+    }
+    for (int j = n - 1; j >= 0; j--) {
+      double s = y[j];
+      for (int p = Lp[j] + 1; p < Lp[j + 1]; p++) s -= Lx[p] * y[Li[p]];
+      y[j] = s / Lx[Lp[j]];
+    }
+    for (int k = 0; k < n; k++) x[perm[k]] = y[k];
+    std::fill(y.begin(), y.end(), 0.0);
+  }
+
+  long long nnz() const { return Lp.empty() ? 0 : Lp[n]; }
+};
+
+// Exact minimum-degree ordering on a (node) graph with explicit elimination cliques (bitsets); ties -> lowest index.
+// Degrees count weighted neighbours (weight = DOFs per node). Not CSparse's AMD: an independent ordering for tests.
+std::vector<int> min_degree(int n, const std::vector<int>& adjp, const std::vector<int>& adj, const std::vector<int>& w) {
+  const int words = (n + 63) / 64;
+  std::vector<uint64_t> bits((size_t)n * words, 0);
+  auto B = [&](int v) { return &bits[(size_t)v * words]; };
+  for (int v = 0; v < n; v++)
+    for (int p = adjp[v]; p < adjp[v + 1]; p++) {
+      int u = adj[p];
+      if (u != v) B(v)[u >> 6] |= 1ull << (u & 63);
+    }
+  std::vector<char> done(n, 0);
+  std::vector<long long> deg(n, 0);
+  auto degree = [&](int v) {
+    long long d = 0;
+    const uint64_t* b = B(v);
+    for (int k = 0; k < words; k++) {
+      uint64_t x = b[k];
+      while (x) { int t = __builtin_ctzll(x); d += w[k * 64 + t]; x &= x - 1; }
+    }
+    return d;
+  };
+  for (int v = 0; v < n; v++) deg[v] = degree(v);
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<int> nb;
+  for (int it = 0; it < n; it++) {
+    int best = -1;
+    for (int v = 0; v < n; v++)
+      if (!done[v] && (best < 0 || deg[v] < deg[best])) best = v;
+    order.push_back(best);
+    done[best] = 1;
+    nb.clear();
+    const uint64_t* bb = B(best);
+    for (int k = 0; k < words; k++) {
+      uint64_t x = bb[k];
+      while (x) { int t = __builtin_ctzll(x); nb.push_back(k * 64 + t); x &= x - 1; }
+    }
+    for (int u : nb) {
+      uint64_t* bu = B(u);
+      for (int k = 0; k < words; k++) bu[k] |= bb[k];
+      bu[u >> 6] &= ~(1ull << (u & 63));
+      bu[best >> 6] &= ~(1ull << (best & 63));
+    }
+    for (int u : nb) deg[u] = degree(u);
+  }
+  return order;
+}
+
+// ---------------------------------------------------------------------------------------------- the solver
+struct Oracle {
+  OrcParams P;
+  Law law;
+  Die bottom, top;
+  int nn = 0, ne = 0, ndof = 0;
+  std::vector<int> tets;
+  std::vector<V3> x, vg, vgPrev;
+  std::vector<char> onX0, onY0;
+  // TetMesh (TetMesh.cs:9-64)
+  std::vector<int> ntStart, ntList, bfaces, bowner;
+  std::vector<char> fsym, isSurf;
+  // state
+  std::vector<double> eps, dmg, elDev, elSig, elRate, elMean, grad, vol, hLocal;
+  std::vector<int> contact, nFix, dieAxis;
+  std::vector<char> fixedv, release;
+  std::vector<double> fixedVal, kFric, aFric;
+  std::vector<V3> ax0, ax1, ax2;
+  // system
+  std::vector<int> colPtr, rowIdx, nbrStart, nbr;
+  std::vector<double> val, kvol, v, r, dv, vTrial, rvol, vNode;
+  std::vector<int> gStart, gNode;
+  std::vector<V3> gCoef;
+  std::vector<double> eR, eK, eEnergy, ePower;
+  std::vector<int> eNbr;
+  std::vector<double> pr, pz, pp, pq;
+  // constants (Forging3DSolver.cs:144-150)
+  double rateRef, rateCut, sigRef, kPen, u0, gap, gap0, ds, time_ = 0;
+  bool warm = false, needAnalyze = true, haveFactor = false;
+  int pcgFailures = 0, pcgSkip = 0;
+  double lastRel = 0;
+  // linear backend
+  SparseChol chol;
+  std::unique_ptr<PardisoDL> pardiso;
+  std::vector<int> userPerm;
+  bool useUserPerm = false;
+  // stats of the last SolveVelocity
+  int statFactor = 0, statPcgIts = 0, statPcgCalls = 0, statSolves = 0, statAssemblies = 0;
+  std::string err;
+
+  int Dof(int n, int c) const { return 3 * n + c; }
+  V3 Axis(int n, int k) const { return k == 0 ? ax0[n] : k == 1 ? ax1[n] : ax2[n]; }
+  V3 ToLocal(int n, const V3& g) const { return V3(g.dot(ax0[n]), g.dot(ax1[n]), g.dot(ax2[n])); }
+  V3 ToGlobal(int n, double l0, double l1, double l2) const { return l0 * ax0[n] + l1 * ax1[n] + l2 * ax2[n]; }
+  V3 GlobalVelocity(const double* vv, int n) const { return ToGlobal(n, vv[3 * n], vv[3 * n + 1], vv[3 * n + 2]); }
+  const Die& Surface(int c) const { return c == TOP ? top : bottom; }
+  double Offset(int c, double g) const { return c == TOP ? g : 0.0; }
+  V3 DieVelocity(int c) const { return c == TOP ? V3(0, 0, -P.ram) : V3(0, 0, 0); }
+  V3 Sym(const V3& p, int n) const { return V3(onX0[n] ? 0 : p.x, onY0[n] ? 0 : p.y, p.z); }
+
+  // TetMesh constructor (TetMesh.cs:9-64)
+  void BuildTopology() {
+    ntStart.assign(nn + 1, 0);
+    for (int q : tets) ntStart[q + 1]++;
+    for (int i = 0; i < nn; i++) ntStart[i + 1] += ntStart[i];
+    std::vector<int> fill(ntStart.begin(), ntStart.end() - 1);
+    ntList.resize(tets.size());
+    for (int e = 0; e < ne; e++)
+      for (int a = 0; a < 4; a++) ntList[fill[tets[4 * e + a]]++] = e;
+    struct F { int A, B, C, owner, count; long long order; };
+    std::unordered_map<long long, F> faces;
+    faces.reserve(4 * ne);
+    long long ord = 0;
+    for (int e = 0; e < ne; e++) {
+      int a = tets[4 * e], b = tets[4 * e + 1], c = tets[4 * e + 2], d = tets[4 * e + 3];
+      int fs[4][3] = {{a, c, b}, {a, b, d}, {a, d, c}, {b, c, d}};
+      for (auto& f : fs) {
+        int s[3] = {f[0], f[1], f[2]};
+        std::sort(s, s + 3);
+        long long key = ((long long)s[0] * nn + s[1]) * nn + s[2];
+        auto it = faces.find(key);
+        if (it == faces.end()) faces.emplace(key, F{f[0], f[1], f[2], e, 1, ord++});
+        else it->second.count++;
+      }
+    }
+    std::vector<F> bl;
+    for (auto& kv : faces) {
+      if (kv.second.count > 2) { err = "non-manifold mesh"; }
+      if (kv.second.count == 1) bl.push_back(kv.second);
+    }
+    std::sort(bl.begin(), bl.end(), [](const F& p, const F& q) { return p.order < q.order; });
+    bfaces.clear();
+    bowner.clear();
+    fsym.clear();
+    for (auto& f : bl) {
+      bfaces.push_back(f.A); bfaces.push_back(f.B); bfaces.push_back(f.C);
+      bowner.push_back(f.owner);
+      fsym.push_back((onX0[f.A] && onX0[f.B] && onX0[f.C]) || (onY0[f.A] && onY0[f.B] && onY0[f.C]));
+    }
+    isSurf.assign(nn, 0);
+    for (size_t k = 0; k < fsym.size(); k++)
+      if (!fsym[k]) for (int a = 0; a < 3; a++) isSurf[bfaces[3 * k + a]] = 1;
+  }
+
+  // LocalLengths (Forging3DSolver.cs:442-466)
+  void LocalLengths() {
+    std::vector<double> sum(nn, 0.0);
+    std::vector<int> cnt(nn, 0);
+    for (int e = 0; e < ne; e++)
+      for (int a = 0; a < 4; a++)
+        for (int b = a + 1; b < 4; b++) {
+          int i = tets[4 * e + a], j = tets[4 * e + b];
+          double l = (x[i] - x[j]).len();
+          sum[i] += l; sum[j] += l; cnt[i]++; cnt[j]++;
+        }
+    hLocal.resize(nn);
+    for (int n = 0; n < nn; n++) hLocal[n] = cnt[n] > 0 ? std::min(sum[n] / cnt[n], P.h) : P.h;
+  }
+
+  // BuildPattern (Forging3DSolver.cs:469-521)
+  void BuildPattern() {
+    std::vector<std::vector<int>> sets(nn);
+    std::vector<int> patch;
+    for (int n = 0; n < nn; n++) {
+      patch.clear();
+      for (int q = ntStart[n]; q < ntStart[n + 1]; q++) {
+        int e = ntList[q];
+        for (int a = 0; a < 4; a++) patch.push_back(tets[4 * e + a]);
+      }
+      for (int i : patch) for (int j : patch) sets[i].push_back(j);
+      if (n % 64 == 63) for (int i = 0; i < nn; i++) if (sets[i].size() > 4096) {
+        std::sort(sets[i].begin(), sets[i].end());
+        sets[i].erase(std::unique(sets[i].begin(), sets[i].end()), sets[i].end());
+      }
+    }
+    nbrStart.assign(nn + 1, 0);
+    nbr.clear();
+    for (int n = 0; n < nn; n++) {
+      auto& s = sets[n];
+      std::sort(s.begin(), s.end());
+      s.erase(std::unique(s.begin(), s.end()), s.end());
+      nbr.insert(nbr.end(), s.begin(), s.end());
+      nbrStart[n + 1] = (int)nbr.size();
+      std::vector<int>().swap(s);
+    }
+    colPtr.assign(ndof + 1, 0);
+    for (int j = 0; j < nn; j++)
+      for (int c = 0; c < 3; c++) colPtr[3 * j + c + 1] = colPtr[3 * j + c] + 3 * (nbrStart[j + 1] - nbrStart[j]);
+    rowIdx.resize(colPtr[ndof]);
+    for (int j = 0; j < nn; j++)
+      for (int c = 0; c < 3; c++) {
+        int p = colPtr[3 * j + c];
+        for (int q = nbrStart[j]; q < nbrStart[j + 1]; q++) {
+          int i = nbr[q];
+          rowIdx[p++] = 3 * i; rowIdx[p++] = 3 * i + 1; rowIdx[p++] = 3 * i + 2;
+        }
+      }
+    val.assign(rowIdx.size(), 0.0);
+    kvol.assign(rowIdx.size(), 0.0);
+    needAnalyze = true;
+    haveFactor = false;
+    // _eNbr (Forging3DSolver.cs:1249-1258)
+    eNbr.resize(16 * (size_t)ne);
+    for (int e = 0; e < ne; e++)
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) eNbr[16 * e + 4 * a + b] = NbrIndex(tets[4 * e + b], tets[4 * e + a]);
+  }
+
+  int NbrIndex(int j, int i) const {
+    const int* b = &nbr[nbrStart[j]];
+    const int* e = &nbr[0] + nbrStart[j + 1];
+    const int* f = std::lower_bound(b, e, i);
+    return (f != e && *f == i) ? (int)(f - b) : -1;
+  }
+  // Pos (Forging3DSolver.cs:524)
+  int Pos(int i, int ci, int j, int cj) const { return colPtr[3 * j + cj] + 3 * NbrIndex(j, i) + ci; }
+
+  // PrepareStep (Forging3DSolver.cs:660-758); returns non-zero on an inverted tet (MeshDistortionException)
+  int PrepareStep() {
+    std::fill(fixedv.begin(), fixedv.end(), 0);
+    std::fill(fixedVal.begin(), fixedVal.end(), 0.0);
+    std::fill(release.begin(), release.end(), 0);
+    for (int n = 0; n < nn; n++) {
+      V3 dirs[3], e[3];
+      double vals[3], ev[3];
+      int k = 0;
+      dieAxis[n] = -1;
+      if (onX0[n]) { dirs[k] = V3(1, 0, 0); vals[k] = 0; k++; }
+      if (onY0[n]) { dirs[k] = V3(0, 1, 0); vals[k] = 0; k++; }
+      int dieIdx = -1;
+      if (contact[n] != FREE) {
+        V3 nrm;
+        Surface(contact[n]).project(x[n], Offset(contact[n], gap), &nrm);
+        dirs[k] = nrm;
+        vals[k] = DieVelocity(contact[n]).dot(nrm);
+        dieIdx = k;
+        k++;
+      }
+      int m = 0;
+      for (int q = 0; q < k; q++) {
+        V3 d = dirs[q];
+        double vv = vals[q];
+        for (int s = 0; s < m; s++) {
+          double proj = d.dot(e[s]);
+          d = d - proj * e[s];
+          vv -= proj * ev[s];
+        }
+        double len = d.len();
+        if (len < 0.2) continue;
+        e[m] = (1 / len) * d;
+        ev[m] = vv / len;
+        if (q == dieIdx) dieAxis[n] = m;
+        m++;
+      }
+      if (m == 0) { e[0] = V3(1, 0, 0); e[1] = V3(0, 1, 0); e[2] = V3(0, 0, 1); }
+      else if (m == 1) {
+        V3 helper = std::fabs(e[0].x) < 0.8 ? V3(1, 0, 0) : V3(0, 1, 0);
+        e[1] = e[0].cross(helper).normalized();
+        e[2] = e[0].cross(e[1]);
+      } else if (m == 2) {
+        e[2] = e[0].cross(e[1]).normalized();
+      }
+      ax0[n] = e[0]; ax1[n] = e[1]; ax2[n] = e[2];
+      nFix[n] = m;
+      for (int q = 0; q < m; q++) { fixedv[Dof(n, q)] = 1; fixedVal[Dof(n, q)] = ev[q]; }
+    }
+    for (int el = 0; el < ne; el++) {
+      V3 p0 = x[tets[4 * el]];
+      V3 a = x[tets[4 * el + 1]] - p0, b = x[tets[4 * el + 2]] - p0, c = x[tets[4 * el + 3]] - p0;
+      double det = a.dot(b.cross(c));
+      if (!(det > 0)) { err = "Tetrahedron " + std::to_string(el) + " is inverted"; return -3; }
+      V3 g1 = (1 / det) * b.cross(c), g2 = (1 / det) * c.cross(a), g3 = (1 / det) * a.cross(b);
+      V3 g0 = -(g1 + g2 + g3);
+      V3 gs[4] = {g0, g1, g2, g3};
+      for (int q = 0; q < 4; q++) {
+        grad[12 * el + 3 * q] = gs[q].x; grad[12 * el + 3 * q + 1] = gs[q].y; grad[12 * el + 3 * q + 2] = gs[q].z;
+      }
+      vol[el] = det / 6;
+    }
+    BuildAnp();
+    BuildFriction();
+    LoadVelocityGuess();
+    return 0;
+  }
+
+  // BuildAnp (Forging3DSolver.cs:761-823)
+  void BuildAnp() {
+    std::fill(vNode.begin(), vNode.end(), 0.0);
+    for (int e = 0; e < ne; e++)
+      for (int a = 0; a < 4; a++) vNode[tets[4 * e + a]] += vol[e] / 4;
+    gStart.assign(nn + 1, 0);
+    gNode.clear();
+    gCoef.clear();
+    std::unordered_map<int, int> map;
+    for (int n = 0; n < nn; n++) {
+      map.clear();
+      for (int q = ntStart[n]; q < ntStart[n + 1]; q++) {
+        int e = ntList[q];
+        double w = vol[e] / 4 / vNode[n];
+        for (int a = 0; a < 4; a++) {
+          int m = tets[4 * e + a];
+          V3 gc = w * V3(grad[12 * e + 3 * a], grad[12 * e + 3 * a + 1], grad[12 * e + 3 * a + 2]);
+          auto it = map.find(m);
+          if (it != map.end()) gCoef[it->second] = gCoef[it->second] + gc;
+          else { map[m] = (int)gNode.size(); gNode.push_back(m); gCoef.push_back(gc); }
+        }
+      }
+      gStart[n + 1] = (int)gNode.size();
+    }
+    std::fill(kvol.begin(), kvol.end(), 0.0);
+    for (int n = 0; n < nn; n++) {
+      double kv = kPen * vNode[n];
+      for (int p = gStart[n]; p < gStart[n + 1]; p++) {
+        int i = gNode[p];
+        V3 ci = ToLocal(i, gCoef[p]);
+        double cia[3] = {ci.x, ci.y, ci.z};
+        for (int q = gStart[n]; q < gStart[n + 1]; q++) {
+          int j = gNode[q];
+          V3 cj = ToLocal(j, gCoef[q]);
+          double cja[3] = {cj.x, cj.y, cj.z};
+          int k = NbrIndex(j, i);
+          for (int cc = 0; cc < 3; cc++) {
+            int pp0 = colPtr[3 * j + cc] + 3 * k;
+            for (int rr = 0; rr < 3; rr++) kvol[pp0 + rr] += kv * (cia[rr] * cja[cc]);
+          }
+        }
+      }
+    }
+  }
+
+  // BuildFriction (Forging3DSolver.cs:829-850)
+  void BuildFriction() {
+    std::fill(aFric.begin(), aFric.end(), 0.0);
+    std::vector<double> ksum(nn, 0.0);
+    std::vector<int> kcnt(nn, 0);
+    for (size_t k = 0; k < fsym.size(); k++) {
+      if (fsym[k]) continue;
+      int a = bfaces[3 * k], b = bfaces[3 * k + 1], c = bfaces[3 * k + 2];
+      double area = 0.5 * (x[b] - x[a]).cross(x[c] - x[a]).len();
+      double kk = elSig[bowner[k]] / std::sqrt(3.0);
+      int ns[3] = {a, b, c};
+      for (int n : ns) { aFric[n] += area / 3; ksum[n] += kk; kcnt[n]++; }
+    }
+    for (int n = 0; n < nn; n++) kFric[n] = kcnt[n] > 0 ? ksum[n] / kcnt[n] : 0;
+  }
+
+  // LoadVelocityGuess (Forging3DSolver.cs:854-869)
+  void LoadVelocityGuess() {
+    bool trend = P.extrapolate_start && warm && (int)vgPrev.size() == nn;
+    for (int n = 0; n < nn; n++) {
+      V3 g = trend ? (2 * vg[n] - vgPrev[n]) : vg[n];
+      V3 l = ToLocal(n, g);
+      v[Dof(n, 0)] = l.x; v[Dof(n, 1)] = l.y; v[Dof(n, 2)] = l.z;
+    }
+    for (int d = 0; d < ndof; d++) if (fixedv[d]) v[d] = fixedVal[d];
+  }
+
+  // Kernel (Forging3DSolver.cs:1312-1400): element residual/stiffness in global axes
+  void Kernel(int e, bool matrix, bool newton, const double* Ve, double* Re, double* Ke, double& energy, double& power) {
+    double B[72] = {0}, MB[72], G[12];
+    double volE = vol[e];
+    int o = 12 * e;
+    for (int a = 0; a < 4; a++) {
+      double bx = grad[o + 3 * a], by = grad[o + 3 * a + 1], bz = grad[o + 3 * a + 2];
+      int c = 3 * a;
+      B[0 * 12 + c] = bx; B[1 * 12 + c + 1] = by; B[2 * 12 + c + 2] = bz;
+      B[3 * 12 + c] = by; B[3 * 12 + c + 1] = bx;
+      B[4 * 12 + c + 1] = bz; B[4 * 12 + c + 2] = by;
+      B[5 * 12 + c] = bz; B[5 * 12 + c + 2] = bx;
+    }
+    double ep[6];
+    for (int i = 0; i < 6; i++) {
+      double s = 0;
+      for (int k = 0; k < 12; k++) s += B[i * 12 + k] * Ve[k];
+      ep[i] = s;
+    }
+    double ev3 = (ep[0] + ep[1] + ep[2]) / 3;
+    double d0 = ep[0] - ev3, d1 = ep[1] - ev3, d2 = ep[2] - ev3;
+    double rate = std::sqrt(std::max(2.0 / 3.0 * (d0 * d0 + d1 * d1 + d2 * d2) + (ep[3] * ep[3] + ep[4] * ep[4] + ep[5] * ep[5]) / 3.0, 0.0));
+    double me[6] = {2.0 / 3.0 * d0, 2.0 / 3.0 * d1, 2.0 / 3.0 * d2, ep[3] / 3, ep[4] / 3, ep[5] / 3};
+    for (int k = 0; k < 12; k++) {
+      double s = 0;
+      for (int i = 0; i < 6; i++) s += B[i * 12 + k] * me[i];
+      G[k] = s;
+    }
+    double strain = eps[e];
+    double sigCut = law.stress(strain, rateCut);
+    double mu0 = sigCut / rateCut, mExp = law.M;
+    double mu, sig, wDens, coefN;
+    if (rate > rateCut) {
+      sig = law.stress(strain, rate);
+      mu = sig / rate;
+      wDens = 0.5 * sigCut * rateCut + (sig * rate - sigCut * rateCut) / (mExp + 1);
+      coefN = (mExp - 1) * mu / (rate * rate);
+    } else {
+      mu = mu0;
+      sig = mu0 * rate;
+      wDens = 0.5 * mu0 * rate * rate;
+      coefN = 0;
+    }
+    energy = volE * wDens;
+    power = volE * sig * rate;
+    for (int k = 0; k < 12; k++) Re[k] = volE * mu * G[k];
+    elSig[e] = sig;
+    elRate[e] = rate;
+    for (int i = 0; i < 6; i++) elDev[6 * e + i] = mu * me[i];
+    if (!matrix) return;
+    for (int k = 0; k < 12; k++) {
+      double tr = (B[k] + B[12 + k] + B[24 + k]) / 3;
+      MB[k] = 2.0 / 3.0 * (B[k] - tr);
+      MB[12 + k] = 2.0 / 3.0 * (B[12 + k] - tr);
+      MB[24 + k] = 2.0 / 3.0 * (B[24 + k] - tr);
+      MB[36 + k] = B[36 + k] / 3;
+      MB[48 + k] = B[48 + k] / 3;
+      MB[60 + k] = B[60 + k] / 3;
+    }
+    double wm = volE * mu, wn = newton ? volE * coefN * P.newton_blend : 0;
+    for (int p = 0; p < 12; p++)
+      for (int q = p; q < 12; q++) {
+        double s = 0;
+        for (int i = 0; i < 6; i++) s += B[i * 12 + p] * MB[i * 12 + q];
+        double vv = wm * s + wn * G[p] * G[q];
+        Ke[p * 12 + q] = vv;
+        Ke[q * 12 + p] = vv;
+      }
+  }
+
+  // ElementWork (Forging3DSolver.cs:1265-1310)
+  void ElementWork(int e, bool matrix, bool newton, const double* vv) {
+    double Ve[12], Re[12], Ke[144];
+    for (int a = 0; a < 4; a++) {
+      V3 g = GlobalVelocity(vv, tets[4 * e + a]);
+      Ve[3 * a] = g.x; Ve[3 * a + 1] = g.y; Ve[3 * a + 2] = g.z;
+    }
+    Kernel(e, matrix, newton, Ve, Re, Ke, eEnergy[e], ePower[e]);
+    for (int a = 0; a < 4; a++) {
+      V3 rl = ToLocal(tets[4 * e + a], V3(Re[3 * a], Re[3 * a + 1], Re[3 * a + 2]));
+      eR[12 * e + 3 * a] = rl.x; eR[12 * e + 3 * a + 1] = rl.y; eR[12 * e + 3 * a + 2] = rl.z;
+    }
+    if (!matrix) return;
+    double blk[9];
+    for (int a = 0; a < 4; a++) {
+      int i = tets[4 * e + a];
+      for (int b = 0; b < 4; b++) {
+        int j = tets[4 * e + b];
+        size_t o = 144 * (size_t)e + 9 * (4 * a + b);
+        for (int rr = 0; rr < 3; rr++) {
+          V3 ar = Axis(i, rr);
+          for (int c = 0; c < 3; c++)
+            blk[3 * rr + c] = ar.x * Ke[(3 * a) * 12 + 3 * b + c] + ar.y * Ke[(3 * a + 1) * 12 + 3 * b + c] + ar.z * Ke[(3 * a + 2) * 12 + 3 * b + c];
+        }
+        for (int rr = 0; rr < 3; rr++)
+          for (int c = 0; c < 3; c++) {
+            V3 bc = Axis(j, c);
+            eK[o + 3 * rr + c] = blk[3 * rr] * bc.x + blk[3 * rr + 1] * bc.y + blk[3 * rr + 2] * bc.z;
+          }
+      }
+    }
+  }
+
+  // Assemble (Forging3DSolver.cs:1115-1217); returns Pi(v), writes r, val (if matrix), power
+  double Assemble(bool matrix, bool newton, const double* vv, double& power) {
+    statAssemblies++;
+    std::fill(rvol.begin(), rvol.end(), 0.0);
+    for (int j = 0; j < ndof; j++) {
+      double vj = vv[j];
+      if (vj == 0) continue;
+      for (int p = colPtr[j]; p < colPtr[j + 1]; p++) rvol[rowIdx[p]] += kvol[p] * vj;
+    }
+    double energy = 0;
+    for (int d = 0; d < ndof; d++) energy += 0.5 * vv[d] * rvol[d];
+    power = 2 * energy;
+    std::copy(rvol.begin(), rvol.end(), r.begin());
+    if (matrix) std::copy(kvol.begin(), kvol.end(), val.begin());
+    if (matrix && eK.size() != 144 * (size_t)ne) eK.resize(144 * (size_t)ne);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; e++) ElementWork(e, matrix, newton, vv);
+    for (int e = 0; e < ne; e++) {
+      energy += eEnergy[e];
+      power += ePower[e];
+      for (int a = 0; a < 4; a++) {
+        int n = tets[4 * e + a];
+        r[3 * n] += eR[12 * e + 3 * a];
+        r[3 * n + 1] += eR[12 * e + 3 * a + 1];
+        r[3 * n + 2] += eR[12 * e + 3 * a + 2];
+      }
+      if (!matrix) continue;
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+          int j = tets[4 * e + b], k = eNbr[16 * e + 4 * a + b];
+          size_t o = 144 * (size_t)e + 9 * (4 * a + b);
+          for (int cj = 0; cj < 3; cj++) {
+            int p = colPtr[3 * j + cj] + 3 * k;
+            for (int ci = 0; ci < 3; ci++) val[p + ci] += eK[o + 3 * ci + cj];
+          }
+        }
+    }
+    double m = P.friction;
+    if (m > 0) {
+      for (int n = 0; n < nn; n++) {
+        if (contact[n] == FREE || nFix[n] >= 3) continue;
+        double c = m * kFric[n] * aFric[n] * (2 / M_PI);
+        if (c <= 0) continue;
+        V3 vd = ToLocal(n, DieVelocity(contact[n]));
+        double vda[3] = {vd.x, vd.y, vd.z};
+        int k0 = nFix[n];
+        double s[3] = {0, 0, 0};
+        double s2 = 0;
+        for (int q = k0; q < 3; q++) { s[q] = vv[3 * n + q] - vda[q]; s2 += s[q] * s[q]; }
+        double sl = std::sqrt(s2), z = sl / u0, at = std::atan(z);
+        energy += c * (sl * at - 0.5 * u0 * std::log(1 + z * z));
+        power += c * at * sl;
+        double gs = sl > 1e-12 * u0 ? at / sl : 1 / u0;
+        for (int q = k0; q < 3; q++) r[3 * n + q] += c * gs * s[q];
+        if (!matrix) continue;
+        double tang = c / (u0 * (1 + z * z));
+        for (int q = k0; q < 3; q++)
+          for (int p = k0; p < 3; p++) {
+            double kqp;
+            if (newton && sl > 1e-12 * u0) kqp = c * gs * ((q == p ? 1 : 0) - s[q] * s[p] / s2) + tang * s[q] * s[p] / s2;
+            else kqp = q == p ? c * gs : 0;
+            val[Pos(n, q, n, p)] += kqp;
+          }
+      }
+    }
+    return energy;
+  }
+
+  static double Dot(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+    return s;
+  }
+  static double Norm(const std::vector<double>& a) {
+    double s = 0;
+    for (double q : a) s += q * q;
+    return std::sqrt(s);
+  }
+  static double Norm(const double* a, int n) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += a[i] * a[i];
+    return std::sqrt(s);
+  }
+
+  // MatVec (Forging3DSolver.cs:1089-1098)
+  void MatVec(const double* xv, double* y) {
+    std::fill(y, y + ndof, 0.0);
+    for (int j = 0; j < ndof; j++) {
+      double vj = xv[j];
+      if (vj == 0) continue;
+      for (int p = colPtr[j]; p < colPtr[j + 1]; p++) y[rowIdx[p]] += val[p] * vj;
+    }
+  }
+
+  // ISpdSolver slot: Analyze / Factorize / Solve
+  int LinAnalyze() {
+    if (P.linear == 1) {
+      if (!pardiso) pardiso.reset(new PardisoDL());
+      if (!pardiso->ok()) { err = "MKL PARDISO unavailable"; return -10; }
+      return pardiso->analyze(ndof, colPtr.data(), rowIdx.data(), useUserPerm ? userPerm.data() : nullptr);
+    }
+    std::vector<int> perm;
+    if (useUserPerm) perm = userPerm;
+    else perm = NodeMinDegree();
+    return chol.analyze(ndof, colPtr.data(), rowIdx.data(), perm.data(), nullptr);
+  }
+  int LinFactorize() {
+    statFactor++;
+    if (P.linear == 1) return pardiso->factorize(val.data());
+    return chol.factorize(val.data());
+  }
+  int LinSolve(const double* b, double* xo) {
+    statSolves++;
+    if (P.linear == 1) return pardiso->solve(b, xo);
+    chol.solve(b, xo);
+    for (int i = 0; i < ndof; i++) if (!std::isfinite(xo[i])) return -4;
+    return 0;
+  }
+  long long FactorNonZeros() const { return P.linear == 1 ? (pardiso ? pardiso->nnz_l : 0) : chol.nnz(); }
+
+  std::vector<int> NodeMinDegree() {
+    std::vector<int> adjp(nn + 1, 0), adj;
+    for (int n = 0; n < nn; n++) {
+      for (int q = nbrStart[n]; q < nbrStart[n + 1]; q++) if (nbr[q] != n) adj.push_back(nbr[q]);
+      adjp[n + 1] = (int)adj.size();
+    }
+    std::vector<int> w(nn, 3);
+    std::vector<int> ord = min_degree(nn, adjp, adj, w);
+    std::vector<int> perm(ndof);
+    for (int k = 0; k < nn; k++) for (int c = 0; c < 3; c++) perm[3 * k + c] = 3 * ord[k] + c;
+    return perm;
+  }
+
+  // Pcg (Forging3DSolver.cs:1051-1087); returns iterations, or -1 if not converged, or < -1 on a solver error
+  int Pcg(const double* b, double* xo, double tol, int maxIt) {
+    double bnorm = Norm(b, ndof);
+    if (bnorm == 0) { std::fill(xo, xo + ndof, 0.0); return 0; }
+    int rc = LinSolve(b, xo);
+    if (rc) return -100 + rc;
+    MatVec(xo, pq.data());
+    for (int i = 0; i < ndof; i++) pr[i] = b[i] - pq[i];
+    if ((rc = LinSolve(pr.data(), pz.data()))) return -100 + rc;
+    pp = pz;
+    double rz = Dot(pr, pz);
+    for (int it = 0; it < maxIt; it++) {
+      statPcgIts++;
+      if (Norm(pr) <= tol * bnorm) return it;
+      MatVec(pp.data(), pq.data());
+      double alpha = rz / Dot(pp, pq);
+      for (int i = 0; i < ndof; i++) { xo[i] += alpha * pp[i]; pr[i] -= alpha * pq[i]; }
+      if ((rc = LinSolve(pr.data(), pz.data()))) return -100 + rc;
+      double rzNew = Dot(pr, pz);
+      double beta = rzNew / rz;
+      rz = rzNew;
+      for (int i = 0; i < ndof; i++) pp[i] = pz[i] + beta * pp[i];
+    }
+    return Norm(pr) <= tol * bnorm ? maxIt : -1;
+  }
+
+  // SolveLinear (Forging3DSolver.cs:965-1019)
+  int SolveLinear(const double* b, double* xo) {
+    if (haveFactor && pcgSkip > 0) {
+      pcgSkip--;
+    } else if (haveFactor) {
+      statPcgCalls++;
+      int its = Pcg(b, xo, 1e-7, P.pcg_budget);
+      if (its >= 0) { pcgFailures = 0; return 0; }
+      if (its < -1) { haveFactor = false; err = "sparse solve failed"; return -5; }
+      if (++pcgFailures >= 2) pcgSkip = std::min(64, 1 << (pcgFailures - 1));
+    }
+    if (needAnalyze) {
+      int rc = LinAnalyze();
+      if (rc) { err = "analyze failed"; return rc; }
+      needAnalyze = false;
+    }
+    int rc = LinFactorize();
+    if (rc) { haveFactor = false; err = "Sparse factorisation failed (not positive definite)"; return rc; }
+    rc = LinSolve(b, xo);
+    if (rc) { haveFactor = false; err = "Sparse solve failed"; return rc; }
+    haveFactor = true;
+    return 0;
+  }
+
+  // SolveVelocity (Forging3DSolver.cs:875-948); returns iterations (>= 1) or a negative error code
+  int SolveVelocity(int& converged) {
+    converged = 0;
+    lastRel = INFINITY;
+    statFactor = statPcgIts = statPcgCalls = statSolves = statAssemblies = 0;
+    int it;
+    int halvedInARow = 0;
+    const int NewtonLimit = 20;
+    for (it = 1; it <= P.max_iterations; it++) {
+      bool newton = (warm || it > 2) && it <= NewtonLimit && !(P.adaptive_newton && halvedInARow >= 2);
+      double pw;
+      double e0 = Assemble(true, newton, v.data(), pw);
+      double maxDiag = 0;
+      for (int j = 0; j < ndof; j++)
+        for (int p = colPtr[j]; p < colPtr[j + 1]; p++) {
+          int i = rowIdx[p];
+          if (fixedv[i] || fixedv[j]) val[p] = i == j ? 1 : 0;
+          else if (i == j) maxDiag = std::max(maxDiag, val[p]);
+        }
+      double reg = 1e-10 * maxDiag;
+      for (int j = 0; j < ndof; j++) if (!fixedv[j]) val[Pos(j / 3, j % 3, j / 3, j % 3)] += reg;
+      for (int d = 0; d < ndof; d++) if (fixedv[d]) r[d] = 0;
+      for (int d = 0; d < ndof; d++) r[d] = -r[d];
+      int rc = SolveLinear(r.data(), dv.data());
+      if (rc) return rc < 0 ? rc : -rc;
+      double slope = 0;
+      for (int d = 0; d < ndof; d++) slope -= r[d] * dv[d];
+      double beta = 1;
+      for (int ls = 0; ls < 8; ls++) {
+        for (int d = 0; d < ndof; d++) vTrial[d] = v[d] + beta * dv[d];
+        double e1 = Assemble(false, false, vTrial.data(), pw);
+        if (e1 <= e0 + 1e-4 * beta * slope + 1e-11 * std::fabs(e0)) break;
+        double curv = (e1 - e0 - slope * beta) / (beta * beta);
+        beta = P.quadratic_ls && curv > 0 ? std::min(0.7 * beta, std::max(0.1 * beta, -slope / (2 * curv))) : 0.5 * beta;
+      }
+      v = vTrial;
+      halvedInARow = newton && beta < 1 ? halvedInARow + 1 : 0;
+      double rel = beta * Norm(dv) / std::max(Norm(v), 1e-30);
+      lastRel = rel;
+      if (std::isnan(rel)) { err = "Velocity solution became NaN"; return -6; }
+      if (rel < P.tolerance) { converged = 1; return it; }
+    }
+    return it - 1;
+  }
+
+  // FinalEvaluation (Forging3DSolver.cs:1402-1451)
+  double FinalEvaluation(double& reaction) {
+    double power;
+    Assemble(false, false, v.data(), power);
+    vgPrev = vg;
+    reaction = 0;
+    double maxB = 0, maxT = 0;
+    for (int n = 0; n < nn; n++) {
+      vg[n] = GlobalVelocity(v.data(), n);
+      if (contact[n] == FREE) continue;
+      V3 rg = ToGlobal(n, r[3 * n], r[3 * n + 1], r[3 * n + 2]);
+      if (contact[n] == TOP) reaction -= rg.z;
+      if (dieAxis[n] < 0) continue;
+      double comp = r[3 * n + dieAxis[n]];
+      if (contact[n] == TOP) maxT = std::max(maxT, comp); else maxB = std::max(maxB, comp);
+    }
+    int keptB = 0, keptT = 0;
+    for (int n = 0; n < nn; n++) {
+      if (contact[n] == FREE || dieAxis[n] < 0) continue;
+      double comp = r[3 * n + dieAxis[n]];
+      double thr = 0.02 * (contact[n] == TOP ? maxT : maxB);
+      if (comp < -thr) release[n] = 1;
+      else if (contact[n] == TOP) keptT++;
+      else keptB++;
+    }
+    if (keptB == 0 || keptT == 0)
+      for (int n = 0; n < nn; n++)
+        if ((keptT == 0 && contact[n] == TOP) || (keptB == 0 && contact[n] == BOTTOM)) release[n] = 0;
+    std::vector<double> nodal(nn);
+    for (int n = 0; n < nn; n++) {
+      double ev = 0;
+      for (int p = gStart[n]; p < gStart[n + 1]; p++) ev += gCoef[p].dot(vg[gNode[p]]);
+      nodal[n] = kPen * ev;
+    }
+    for (int e = 0; e < ne; e++)
+      elMean[e] = 0.25 * (nodal[tets[4 * e]] + nodal[tets[4 * e + 1]] + nodal[tets[4 * e + 2]] + nodal[tets[4 * e + 3]]);
+    return power;
+  }
+
+  // AccumulateDamage (Forging3DSolver.cs:1844-1855)
+  void AccumulateDamage(double dt) {
+    for (int e = 0; e < ne; e++) {
+      double sig = elSig[e];
+      if (!(sig > 0) || elRate[e] <= rateCut) continue;
+      int o = 6 * e;
+      double m = elMean[e];
+      double s1 = max3(elDev[o] + m, elDev[o + 1] + m, elDev[o + 2] + m, elDev[o + 3], elDev[o + 4], elDev[o + 5]);
+      if (s1 > 0) dmg[e] += s1 / sig * elRate[e] * dt;
+    }
+  }
+
+  static double Quality(const V3& a, const V3& b, const V3& c, const V3& d) {
+    double l2 = ((b - a).dot(b - a) + (c - a).dot(c - a) + (d - a).dot(d - a) + (c - b).dot(c - b) + (d - b).dot(d - b) + (d - c).dot(d - c)) / 6.0;
+    double V = (b - a).dot((c - a).cross(d - a)) / 6.0;
+    return l2 > 0 ? 6 * std::sqrt(2.0) * V / std::pow(l2, 1.5) : -1;
+  }
+
+  // TryUpdate (Forging3DSolver.cs:1455-1537), contact and placement only: the placement repair
+  // (ImprovePlacement/SmoothInterior) is host geometry outside the solve step and is not restated.
+  int TryUpdate(double dt) {
+    double gapT = std::max(0.0, gap - P.ram * dt);
+    if (gapT < 1e-9) gapT = 0;
+    std::vector<V3> xT(nn);
+    std::vector<int> cT(nn, FREE);
+    double maxPenRel = 0;
+    for (int n = 0; n < nn; n++) {
+      V3 p = Sym(x[n] + dt * vg[n], n);
+      int c = contact[n];
+      if (c != FREE && !release[n]) {
+        const Die& s = Surface(c);
+        double off = Offset(c, gapT);
+        if (s.clearance(p, off) > 0.05 * hLocal[n]) cT[n] = FREE;
+        else { p = Sym(s.project(p, off, nullptr), n); cT[n] = c; }
+      } else if (isSurf[n]) {
+        double cb = bottom.clearance(p, 0), ctp = top.clearance(p, gapT);
+        if (cb < 0 || ctp < 0) {
+          int hit = cb <= ctp ? BOTTOM : TOP;
+          V3 q = Sym(Surface(hit).project(p, Offset(hit, gapT), nullptr), n);
+          maxPenRel = std::max(maxPenRel, (q - p).len() / hLocal[n]);
+          p = q;
+          cT[n] = hit;
+        }
+      }
+      xT[n] = p;
+    }
+    if (maxPenRel > 0.5) { err = "die penetration exceeds half the local element size"; return 1; }
+    for (int e = 0; e < ne; e++)
+      if (Quality(xT[tets[4 * e]], xT[tets[4 * e + 1]], xT[tets[4 * e + 2]], xT[tets[4 * e + 3]]) <= 0.005) {
+        err = "tetrahedron collapsed";
+        return 1;
+      }
+    x = xT;
+    contact = cT;
+    gap = gapT;
+    time_ += dt;
+    return 0;
+  }
+
+  // LayerLimitedIncrement (Forging3DSolver.cs:236-249)
+  double LayerLimitedIncrement() const {
+    double thinnest = INFINITY;
+    for (int n = 0; n < nn; n++) {
+      if (contact[n] != TOP) continue;
+      double zb = bottom.surface_z(x[n].x, x[n].y), zt = top.surface_z(x[n].x, x[n].y) + gap;
+      if (std::isfinite(zb) && std::isfinite(zt) && zt > zb) thinnest = std::min(thinnest, zt - zb);
+    }
+    return std::isfinite(thinnest) ? std::max(0.1 * thinnest, P.stroke_increment / 32) : INFINITY;
+  }
+
+  // One increment (Forging3DSolver.cs:540-638) on a fixed mesh: no remeshing, no recovery ladder.
+  int Step(double& load, double& reaction, int& iterations) {
+    int rc = PrepareStep();
+    if (rc) return rc;
+    int converged = 0;
+    iterations = SolveVelocity(converged);
+    if (iterations < 0) return iterations;
+    if (!converged && lastRel >= 1e-2) { err = "Velocity solution did not converge"; return -7; }
+    double power = FinalEvaluation(reaction);
+    load = power / P.ram * P.sym_factor;
+    reaction *= P.sym_factor;
+    double dsx = std::min(std::min(ds, P.stroke_increment), LayerLimitedIncrement());
+    double dt = std::min(dsx, gap) / P.ram;
+    if (gap - P.ram * dt < 0.05 * dsx) dt = gap / P.ram;
+    bool done = false;
+    for (int attempt = 0; attempt <= 5 && !done; attempt++) {
+      double dta = dt / (1 << attempt);
+      if (TryUpdate(dta) == 0) { dt = dta; done = true; }
+    }
+    if (!done) return -8;
+    for (int e = 0; e < ne; e++) eps[e] += elRate[e] * dt;
+    AccumulateDamage(dt);
+    warm = true;
+    lastDt = dt;
+    return 0;
+  }
+  double lastDt = 0;
+};
+
+}  // namespace
+
+extern "C" {
+
+void* orc_create(const OrcParams* prm, int nn, int ne, const int* tets, const double* x, const unsigned char* onX0,
+                 const unsigned char* onY0, double gap0) {
+  auto* o = new Oracle();
+  o->P = *prm;
+  o->law = Law{prm->C, prm->N, prm->M, prm->Eps0};
+  o->bottom.side = 0; o->bottom.kind = prm->bottom_kind;
+  o->top.side = 1; o->top.kind = prm->top_kind;
+  for (int k = 0; k < 8; k++) { o->bottom.p[k] = prm->bottom_p[k]; o->top.p[k] = prm->top_p[k]; }
+  o->nn = nn; o->ne = ne; o->ndof = 3 * nn;
+  o->tets.assign(tets, tets + 4 * (size_t)ne);
+  o->x.resize(nn);
+  for (int n = 0; n < nn; n++) o->x[n] = V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]);
+  o->onX0.assign(onX0, onX0 + nn);
+  o->onY0.assign(onY0, onY0 + nn);
+  o->vg.assign(nn, V3());
+  o->eps.assign(ne, 0); o->dmg.assign(ne, 0); o->elDev.assign(6 * (size_t)ne, 0);
+  o->elSig.assign(ne, 0); o->elRate.assign(ne, 0); o->elMean.assign(ne, 0);
+  o->grad.assign(12 * (size_t)ne, 0); o->vol.assign(ne, 0);
+  o->contact.assign(nn, FREE); o->nFix.assign(nn, 0); o->dieAxis.assign(nn, -1);
+  o->fixedv.assign(o->ndof, 0); o->fixedVal.assign(o->ndof, 0); o->release.assign(nn, 0);
+  o->kFric.assign(nn, 0); o->aFric.assign(nn, 0);
+  o->ax0.assign(nn, V3()); o->ax1.assign(nn, V3()); o->ax2.assign(nn, V3());
+  o->v.assign(o->ndof, 0); o->r.assign(o->ndof, 0); o->dv.assign(o->ndof, 0); o->vTrial.assign(o->ndof, 0);
+  o->rvol.assign(o->ndof, 0); o->vNode.assign(nn, 0);
+  o->pr.assign(o->ndof, 0); o->pz.assign(o->ndof, 0); o->pp.assign(o->ndof, 0); o->pq.assign(o->ndof, 0);
+  o->eR.assign(12 * (size_t)ne, 0); o->eEnergy.assign(ne, 0); o->ePower.assign(ne, 0);
+  // constants (Forging3DSolver.cs:144-150)
+  o->rateRef = prm->ram / prm->size;
+  o->rateCut = 1e-3 * o->rateRef;
+  o->sigRef = o->law.stress(0, o->rateRef);
+  o->kPen = prm->penalty_factor * o->sigRef / o->rateRef;
+  o->u0 = 5e-3 * prm->ram;
+  o->gap = o->gap0 = gap0;
+  o->ds = prm->stroke_increment;
+  o->BuildTopology();
+  o->LocalLengths();
+  o->BuildPattern();
+  std::fill(o->elSig.begin(), o->elSig.end(), o->sigRef);
+  return o;
+}
+
+void orc_destroy(void* h) { delete (Oracle*)h; }
+
+const char* orc_error(void* h) { return ((Oracle*)h)->err.c_str(); }
+
+void orc_sizes(void* h, long long* out) {
+  auto* o = (Oracle*)h;
+  out[0] = o->nn; out[1] = o->ne; out[2] = o->ndof; out[3] = (long long)o->rowIdx.size();
+  out[4] = (long long)o->fsym.size(); out[5] = (long long)o->gNode.size();
+}
+
+void orc_constants(void* h, double* out) {
+  auto* o = (Oracle*)h;
+  out[0] = o->rateRef; out[1] = o->rateCut; out[2] = o->sigRef; out[3] = o->kPen; out[4] = o->u0; out[5] = o->gap;
+}
+
+void orc_get_pattern(void* h, int* colPtr, int* rowIdx, int* nbrStart, int* nbr) {
+  auto* o = (Oracle*)h;
+  std::copy(o->colPtr.begin(), o->colPtr.end(), colPtr);
+  std::copy(o->rowIdx.begin(), o->rowIdx.end(), rowIdx);
+  if (nbrStart) std::copy(o->nbrStart.begin(), o->nbrStart.end(), nbrStart);
+  if (nbr) std::copy(o->nbr.begin(), o->nbr.end(), nbr);
+}
+
+void orc_get_topology(void* h, int* bfaces, int* owner, unsigned char* fsym, unsigned char* isSurf) {
+  auto* o = (Oracle*)h;
+  std::copy(o->bfaces.begin(), o->bfaces.end(), bfaces);
+  std::copy(o->bowner.begin(), o->bowner.end(), owner);
+  for (size_t k = 0; k < o->fsym.size(); k++) fsym[k] = o->fsym[k];
+  for (int n = 0; n < o->nn; n++) isSurf[n] = o->isSurf[n];
+}
+
+// Initial contact and velocity guess (Forging3DSolver.cs:158-182) for a vertical-cylinder billet.
+void orc_initial(void* h, double zmin, double cx, double cy) {
+  auto* o = (Oracle*)h;
+  double tol = 0.05 * o->P.h;
+  int nb = 0, nt = 0;
+  for (int n = 0; n < o->nn; n++) {
+    if (!o->isSurf[n]) continue;
+    if (o->bottom.clearance(o->x[n], 0) < tol) { o->contact[n] = BOTTOM; nb++; }
+    else if (o->top.clearance(o->x[n], o->gap) < tol) { o->contact[n] = TOP; nt++; }
+  }
+  for (int side = BOTTOM; side <= TOP; side++) {
+    if ((side == BOTTOM && nb) || (side == TOP && nt)) continue;
+    int best = -1;
+    double bestC = 1e300;
+    for (int n = 0; n < o->nn; n++) {
+      if (!o->isSurf[n]) continue;
+      double c = side == BOTTOM ? o->bottom.clearance(o->x[n], 0) : o->top.clearance(o->x[n], o->gap);
+      if (c < bestC) { bestC = c; best = n; }
+    }
+    if (best >= 0) o->contact[best] = side;
+  }
+  for (int n = 0; n < o->nn; n++)
+    if (o->contact[n] != FREE) o->x[n] = o->Sym(o->Surface(o->contact[n]).project(o->x[n], o->Offset(o->contact[n], o->gap), nullptr), n);
+  for (int n = 0; n < o->nn; n++) {
+    V3 p = o->x[n];
+    o->vg[n] = V3(0.5 * o->rateRef * (p.x - cx), 0.5 * o->rateRef * (p.y - cy), -o->rateRef * (p.z - zmin));
+  }
+}
+
+void orc_get_state(void* h, double* x, double* vg, int* contact, double* eps, double* elSig, double* elRate, double* gap,
+                   unsigned char* warm) {
+  auto* o = (Oracle*)h;
+  for (int n = 0; n < o->nn; n++) {
+    if (x) { x[3 * n] = o->x[n].x; x[3 * n + 1] = o->x[n].y; x[3 * n + 2] = o->x[n].z; }
+    if (vg) { vg[3 * n] = o->vg[n].x; vg[3 * n + 1] = o->vg[n].y; vg[3 * n + 2] = o->vg[n].z; }
+    if (contact) contact[n] = o->contact[n];
+  }
+  if (eps) std::copy(o->eps.begin(), o->eps.end(), eps);
+  if (elSig) std::copy(o->elSig.begin(), o->elSig.end(), elSig);
+  if (elRate) std::copy(o->elRate.begin(), o->elRate.end(), elRate);
+  if (gap) *gap = o->gap;
+  if (warm) *warm = o->warm;
+}
+
+void orc_set_state(void* h, const double* x, const double* vg, const int* contact, const double* eps, const double* elSig,
+                   double gap, int warm) {
+  auto* o = (Oracle*)h;
+  for (int n = 0; n < o->nn; n++) {
+    if (x) o->x[n] = V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]);
+    if (vg) o->vg[n] = V3(vg[3 * n], vg[3 * n + 1], vg[3 * n + 2]);
+    if (contact) o->contact[n] = contact[n];
+  }
+  if (eps) std::copy(eps, eps + o->ne, o->eps.begin());
+  if (elSig) std::copy(elSig, elSig + o->ne, o->elSig.begin());
+  o->gap = gap;
+  o->warm = warm != 0;
+}
+
+int orc_prepare(void* h) { return ((Oracle*)h)->PrepareStep(); }
+
+void orc_get_frames(void* h, double* ax, int* nFix, int* dieAxis, unsigned char* fixedv, double* fixedVal) {
+  auto* o = (Oracle*)h;
+  for (int n = 0; n < o->nn; n++) {
+    V3 e[3] = {o->ax0[n], o->ax1[n], o->ax2[n]};
+    for (int k = 0; k < 3; k++) { ax[9 * n + 3 * k] = e[k].x; ax[9 * n + 3 * k + 1] = e[k].y; ax[9 * n + 3 * k + 2] = e[k].z; }
+    nFix[n] = o->nFix[n];
+    dieAxis[n] = o->dieAxis[n];
+  }
+  for (int d = 0; d < o->ndof; d++) { fixedv[d] = o->fixedv[d]; fixedVal[d] = o->fixedVal[d]; }
+}
+
+void orc_get_geometry(void* h, double* grad, double* vol, double* vNode, double* kvol, double* aFric, double* kFric) {
+  auto* o = (Oracle*)h;
+  if (grad) std::copy(o->grad.begin(), o->grad.end(), grad);
+  if (vol) std::copy(o->vol.begin(), o->vol.end(), vol);
+  if (vNode) std::copy(o->vNode.begin(), o->vNode.end(), vNode);
+  if (kvol) std::copy(o->kvol.begin(), o->kvol.end(), kvol);
+  if (aFric) std::copy(o->aFric.begin(), o->aFric.end(), aFric);
+  if (kFric) std::copy(o->kFric.begin(), o->kFric.end(), kFric);
+}
+
+void orc_get_v(void* h, double* v) { auto* o = (Oracle*)h; std::copy(o->v.begin(), o->v.end(), v); }
+void orc_set_v(void* h, const double* v) { auto* o = (Oracle*)h; std::copy(v, v + o->ndof, o->v.begin()); }
+
+double orc_assemble(void* h, int matrix, int newton, const double* v, double* r, double* val, double* power) {
+  auto* o = (Oracle*)h;
+  double pw;
+  double e = o->Assemble(matrix != 0, newton != 0, v, pw);
+  if (r) std::copy(o->r.begin(), o->r.end(), r);
+  if (val && matrix) std::copy(o->val.begin(), o->val.end(), val);
+  if (power) *power = pw;
+  return e;
+}
+
+void orc_get_element_state(void* h, double* elSig, double* elRate, double* elDev, double* elMean, double* dmg) {
+  auto* o = (Oracle*)h;
+  if (elSig) std::copy(o->elSig.begin(), o->elSig.end(), elSig);
+  if (elRate) std::copy(o->elRate.begin(), o->elRate.end(), elRate);
+  if (elDev) std::copy(o->elDev.begin(), o->elDev.end(), elDev);
+  if (elMean) std::copy(o->elMean.begin(), o->elMean.end(), elMean);
+  if (dmg) std::copy(o->dmg.begin(), o->dmg.end(), dmg);
+}
+
+// stats: [factorizations, pcg iterations, pcg calls, solves, assemblies]
+int orc_solve_velocity(void* h, int* converged, double* lastRel, int* stats) {
+  auto* o = (Oracle*)h;
+  int cv = 0;
+  int it = o->SolveVelocity(cv);
+  *converged = cv;
+  *lastRel = o->lastRel;
+  if (stats) { stats[0] = o->statFactor; stats[1] = o->statPcgIts; stats[2] = o->statPcgCalls; stats[3] = o->statSolves; stats[4] = o->statAssemblies; }
+  return it;
+}
+
+double orc_final(void* h, double* reaction) { return ((Oracle*)h)->FinalEvaluation(*reaction); }
+
+int orc_step(void* h, double* out) {
+  auto* o = (Oracle*)h;
+  double load = 0, reaction = 0;
+  int its = 0;
+  int rc = o->Step(load, reaction, its);
+  out[0] = load; out[1] = reaction; out[2] = its; out[3] = o->lastDt; out[4] = o->gap;
+  out[5] = o->statFactor; out[6] = o->statPcgIts; out[7] = o->statAssemblies;
+  return rc;
+}
+
+void orc_set_perm(void* h, const int* perm) {
+  auto* o = (Oracle*)h;
+  if (perm) { o->userPerm.assign(perm, perm + o->ndof); o->useUserPerm = true; }
+  else o->useUserPerm = false;
+  o->needAnalyze = true;
+  o->haveFactor = false;
+}
+
+void orc_reset_factor(void* h) { auto* o = (Oracle*)h; o->haveFactor = false; o->pcgFailures = o->pcgSkip = 0; }
+
+long long orc_factor_nnz(void* h) { return ((Oracle*)h)->FactorNonZeros(); }
+
+// ---- stand-alone sparse Cholesky (full symmetric CSC in, FP64) for tests and the CPU baseline
+void* orc_chol_create(int n, const int* colPtr, const int* rowIdx, const int* perm, int* parent_out, int* colcount_out, int* rc) {
+  auto* c = new SparseChol();
+  std::vector<int> cc;
+  *rc = c->analyze(n, colPtr, rowIdx, perm, &cc);
+  if (parent_out) std::copy(c->parent.begin(), c->parent.end(), parent_out);
+  if (colcount_out) std::copy(cc.begin(), cc.end(), colcount_out);
+  return c;
+}
+int orc_chol_factor(void* c, const double* vals) { return ((SparseChol*)c)->factorize(vals); }
+void orc_chol_solve(void* c, const double* b, double* x) { ((SparseChol*)c)->solve(b, x); }
+long long orc_chol_nnz(void* c) { return ((SparseChol*)c)->nnz(); }
+void orc_chol_destroy(void* c) { delete (SparseChol*)c; }
+
+// Elimination tree of P A P^T by the textbook definition (parent(j) = min{i > j : L(i,j) != 0}) from a brute-force
+// symbolic elimination on a dense boolean matrix. Small n only: this pins the sparse symbolic paths.
+int orc_brute_symbolic(int n, const int* colPtr, const int* rowIdx, const int* perm, int* parent_out, int* colcount_out) {
+  if (n > 6000) return -1;
+  std::vector<int> iperm(n);
+  for (int k = 0; k < n; k++) iperm[perm[k]] = k;
+  std::vector<std::vector<char>> M(n, std::vector<char>(n, 0));
+  for (int j = 0; j < n; j++)
+    for (int p = colPtr[j]; p < colPtr[j + 1]; p++) {
+      int a = iperm[rowIdx[p]], b = iperm[j];
+      M[a][b] = M[b][a] = 1;
+    }
+  for (int k = 0; k < n; k++) {
+    std::vector<int> nz;
+    for (int i = k + 1; i < n; i++) if (M[i][k]) nz.push_back(i);
+    for (int a : nz) for (int b : nz) M[a][b] = 1;
+    parent_out[k] = nz.empty() ? -1 : nz[0];
+    colcount_out[k] = 1 + (int)nz.size();
+  }
+  return 0;
+}
+
+// Minimum-degree node ordering of a full symmetric CSC pattern whose 3 consecutive columns form a node.
+void orc_min_degree_nodes(int nn, const int* nbrStart, const int* nbr, int* perm_out) {
+  std::vector<int> adjp(nn + 1, 0), adj;
+  for (int n = 0; n < nn; n++) {
+    for (int q = nbrStart[n]; q < nbrStart[n + 1]; q++) if (nbr[q] != n) adj.push_back(nbr[q]);
+    adjp[n + 1] = (int)adj.size();
+  }
+  std::vector<int> w(nn, 3);
+  std::vector<int> ord = min_degree(nn, adjp, adj, w);
+  for (int k = 0; k < nn; k++) for (int c = 0; c < 3; c++) perm_out[3 * k + c] = 3 * ord[k] + c;
+}
+
+// ---- MKL PARDISO on a full symmetric CSC (the reference's MklPardisoSolver settings, MklPardisoSolver.cs:20-99)
+void* orc_pardiso_create(int n, const int* colPtr, const int* rowIdx, const int* perm, int want_perm, int* rc) {
+  auto* p = new PardisoDL();
+  if (!p->ok()) { *rc = -10; return p; }
+  p->want_perm = want_perm;
+  *rc = p->analyze(n, colPtr, rowIdx, perm);
+  return p;
+}
+int orc_pardiso_factor(void* p, const double* vals) { return ((PardisoDL*)p)->factorize(vals); }
+int orc_pardiso_solve(void* p, const double* b, double* x) { return ((PardisoDL*)p)->solve(b, x); }
+void orc_pardiso_info(void* p, long long* out) {
+  auto* q = (PardisoDL*)p;
+  out[0] = q->nnz_l; out[1] = q->mflops; out[2] = q->threads;
+}
+void orc_pardiso_perm(void* p, int* perm) { auto* q = (PardisoDL*)p; std::copy(q->perm.begin(), q->perm.end(), perm); }
+void orc_pardiso_destroy(void* p) { delete (PardisoDL*)p; }
+int orc_pardiso_available() { PardisoDL p; return p.ok() ? 1 : 0; }
+void orc_set_threads(int n) { PardisoDL::set_threads(n); }
+
+}  // extern "C"
