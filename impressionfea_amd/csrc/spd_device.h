@@ -1,0 +1,89 @@
+// Device supernodal Cholesky (multifrontal, level scheduled) for the ISpdSolver slot and the device-resident step.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "symbolic.h"
+
+namespace ifea {
+
+// Device view of the symbolic structure.
+struct SymDev {
+  const int* sn_first;
+  const long long* sn_rowptr;
+  const int* sn_rows;
+  const long long* Loff;
+  const long long* Uoff;
+  const long long* Voff;
+  const long long* Woff;
+  const long long* rel_off;
+  const int* rel;
+  const int* child_ptr;
+  const int* child_list;
+};
+
+struct LaunchList {   // tasks of one launch: [begin, end) into a task array
+  long long begin = 0, end = 0;
+  int c0 = 0;         // factor step column (panel / syrk)
+};
+
+class SpdDevice {
+ public:
+  explicit SpdDevice(hipStream_t stream) : stream_(stream) {}
+  ~SpdDevice();
+
+  // Host symbolic + device structures. perm may be null (nested dissection).
+  void analyze(int n, const int* colPtr, const int* rowIdx, const int* perm, const SymbolicOptions& opt);
+  bool analyzed() const { return analyzed_; }
+  const Symbolic& symbolic() const { return S_; }
+
+  // Numeric factorization from full-CSC values already on the device. precision: 0 fp64, 1 fp32 storage.
+  // Returns 0 or IFEA_ERR_NOT_SPD (-4).
+  int factorize_device(const double* d_vals, int precision);
+  bool factored() const { return factored_; }
+  int precision() const { return prec_; }
+
+  // x = A^{-1} b with the current factor; vectors in the original (unpermuted) order, on the device.
+  // d_x may alias d_b.
+  void solve_device(const double* d_b, double* d_x);
+
+  double last_factor_ms() const { return last_factor_ms_; }
+  size_t device_bytes() const;
+
+ private:
+  template <typename T> int factorize_t(const double* d_vals);
+  template <typename T> void solve_t(const double* d_b, double* d_x);
+
+  hipStream_t stream_;
+  Symbolic S_;
+  bool analyzed_ = false, factored_ = false;
+  int prec_ = 0;
+  double last_factor_ms_ = 0;
+  // device symbolic
+  DBuf<int> perm_, sn_first_, sn_rows_, rel_, child_ptr_, child_list_;
+  DBuf<long long> sn_rowptr_, Loff_, Uoff_, Voff_, Woff_, rel_off_;
+  DBuf<int> amap_src_;
+  DBuf<long long> amap_dst_;
+  // task arrays (int4: a, b, c, d)
+  DBuf<int4> ext_tasks_, panel_tasks_, syrk_tasks_, zero_tasks_;
+  DBuf<int> level_sn_;
+  // launch schedule
+  struct Level {
+    std::vector<LaunchList> ext_rounds;
+    LaunchList zero;
+    std::vector<LaunchList> panel_steps, syrk_steps;
+    long long sn_begin = 0, sn_end = 0;  // into level_sn_
+    long long max_m = 0;
+  };
+  std::vector<Level> levels_;
+  // numeric storage
+  DBuf<unsigned char> L_, U_;
+  DBuf<double> V_, W_, y_;
+  DBuf<int> err_;
+  SymDev sd_{};
+};
+
+}  // namespace ifea

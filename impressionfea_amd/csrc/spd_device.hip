@@ -1,0 +1,509 @@
+// Device supernodal Cholesky for MI355X (gfx950): multifrontal, level-scheduled, deterministic.
+//
+// Replaces the numeric phases the reference delegates to cuDSS (CudssSolver.cs:126-145) and MKL PARDISO
+// (MklPardisoSolver.cs:74-99). Per elimination-tree level:
+//   extend-add   children's update blocks into the parent front, one child rank per launch (no atomics),
+//   panel        FP64 Cholesky of a 64-wide diagonal block in one wave + FP64 TRSM of the rows below,
+//   syrk         trailing update of the front with the panel on MFMA (FP64 16x16x4, or FP32 16x16x4 with FP32
+//                accumulation in mixed precision).
+// Solves run a multifrontal forward sweep (update vectors passed to the parent in fixed child order) and a gather
+// backward sweep, in FP64 whatever the factor precision.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "spd_device.h"
+
+namespace ifea {
+
+constexpr int NB = 64;          // panel width
+constexpr int PANEL_ROWS = 256; // rows per panel task (one per thread)
+constexpr int EXT_TARGET = 8192;
+constexpr long long ZERO_CHUNK = 1 << 16;
+
+// ------------------------------------------------------------------------------------------------ MFMA wrappers
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // item r of lane l holds D[row][l % 16]; the row layout differs between the f64 and f32 instructions
+  static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+__device__ __forceinline__ int sn_k(const SymDev& sd, int s) { return sd.sn_first[s + 1] - sd.sn_first[s]; }
+__device__ __forceinline__ long long sn_m(const SymDev& sd, int s) { return sd.sn_rowptr[s + 1] - sd.sn_rowptr[s]; }
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------------------------------------ kernels
+template <typename T>
+__global__ void k_scatter(const double* __restrict__ vals, const int* __restrict__ src, const long long* __restrict__ dst,
+                          long long cnt, T* __restrict__ L) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k < cnt) L[dst[k]] = (T)vals[src[k]];
+}
+
+// task: (supernode s, chunk index): zero one chunk of the supernode's update block before its level starts
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_u(const int4* __restrict__ tasks, SymDev sd, T* __restrict__ U) {
+  int4 tk = tasks[blockIdx.x];
+  int s = tk.x;
+  long long mu = sn_m(sd, s) - sn_k(sd, s);
+  long long size = mu * mu;
+  long long b = (long long)tk.y * ZERO_CHUNK, e = min(size, b + ZERO_CHUNK);
+  T* u = U + sd.Uoff[s];
+  for (long long i = b + threadIdx.x; i < e; i += blockDim.x) u[i] = T(0);
+}
+
+// task: (child c, first column, end column, parent p) of the child's update block
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend(const int4* __restrict__ tasks, SymDev sd, T* __restrict__ L,
+                                                T* __restrict__ U) {
+  int4 tk = tasks[blockIdx.x];
+  int c = tk.x, p = tk.w;
+  int kc = sn_k(sd, c);
+  int mu = (int)(sn_m(sd, c) - kc);
+  int kp = sn_k(sd, p);
+  long long mp = sn_m(sd, p);
+  long long mpu = mp - kp;
+  const T* Uc = U + sd.Uoff[c];
+  const int* rel = sd.rel + sd.rel_off[c];
+  T* Lp = L + sd.Loff[p];
+  T* Up = U + sd.Uoff[p];
+  for (int j = tk.y; j < tk.z; j++) {
+    int pc = rel[j];
+    T* dcol = pc < kp ? Lp + (long long)pc * mp : Up + (long long)(pc - kp) * mpu - kp;
+    const T* scol = Uc + (long long)j * mu;
+    for (int i = j + threadIdx.x; i < mu; i += blockDim.x) dcol[rel[i]] += scol[i];
+  }
+}
+
+// task: (supernode s, first row, end row, writes-diagonal flag). c0 = first column of the 64-wide block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
+                                               int* __restrict__ err) {
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int w = min(NB, k - c0);
+  T* P = L + sd.Loff[s];
+  __shared__ double D[NB][NB + 1];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
+    int j = idx / NB, i = idx % NB;
+    double v;
+    if (i < w && j < w) v = (i >= j) ? (double)P[(long long)(c0 + j) * m + c0 + i] : 0.0;
+    else v = (i == j) ? 1.0 : 0.0;
+    D[i][j] = v;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    // one wave factors the block in LDS: lane i updates row i; column j is read back by broadcast
+    const int i = tid;
+    bool bad = false;
+    for (int j = 0; j < NB; j++) {
+      double piv = D[j][j];
+      bad |= !(piv > 0.0) || !isfinite(piv);
+      double d = sqrt(piv > 0.0 ? piv : 1.0);
+      double lij = (i > j) ? D[i][j] / d : 0.0;
+      wave_sync_lds();
+      if (i == j) D[i][j] = d;
+      else if (i > j) D[i][j] = lij;
+      wave_sync_lds();
+      for (int q = j + 1; q <= i; q++) D[i][q] -= lij * D[q][j];
+      wave_sync_lds();
+    }
+    for (int q = i + 1; q < NB; q++) D[i][q] = 0.0;
+    if (tk.w) {
+      for (int q = 0; q < w; q++)
+        if (i < w && i >= q) P[(long long)(c0 + q) * m + c0 + i] = (T)D[i][q];
+      if (bad && i == 0) atomicOr(err, 1);
+    }
+  }
+  __syncthreads();
+  // rows below the block: X = A L11^{-T}, one row per thread
+  const long long row = (long long)tk.y + tid;
+  const bool valid = row < tk.z;
+  double x[NB];
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    double a = (q < w && valid) ? (double)P[(long long)(c0 + q) * m + row] : 0.0;
+#pragma unroll
+    for (int p = 0; p < q; p++) a -= x[p] * D[q][p];
+    x[q] = a / D[q][q];
+    if (q < w && valid) P[(long long)(c0 + q) * m + row] = (T)x[q];
+  }
+}
+
+// task: (supernode s, row tile I, column tile J, 0), I >= J; tiles of 64 over the trailing part below the block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
+                                              T* __restrict__ U) {
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int w = min(NB, k - c0);
+  const long long rb = c0 + w;
+  const T* P = L + sd.Loff[s];
+  __shared__ T Xi[NB][64 + 4];
+  __shared__ T Xj[NB][64 + 4];
+  const long long i0 = rb + 64LL * tk.y, j0 = rb + 64LL * tk.z;
+  for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
+    int q = idx >> 6, rr = idx & 63;
+    long long ri = i0 + rr, rj = j0 + rr;
+    const T* cp = P + (long long)(c0 + q) * m;
+    Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
+    Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ib = (wv >> 1) * 32, jb = (wv & 1) * 32;
+  typedef typename Mfma<T>::v4 v4;
+  v4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+  const int kq = (w + 3) & ~3;
+  const int l16 = lane & 15;
+  for (int kk = 0; kk < kq; kk += 4) {
+    const int q = kk + (lane >> 4);
+    T a0 = Xj[q][jb + l16], a1 = Xj[q][jb + 16 + l16];
+    T b0 = Xi[q][ib + l16], b1 = Xi[q][ib + 16 + l16];
+    acc00 = Mfma<T>::op(a0, b0, acc00);
+    acc01 = Mfma<T>::op(a0, b1, acc01);
+    acc10 = Mfma<T>::op(a1, b0, acc10);
+    acc11 = Mfma<T>::op(a1, b1, acc11);
+  }
+  // D[a][b] = sum_q Xj[a][q] Xi[b][q] = contribution to C(i = i0+ib+16bi+b, j = j0+jb+16ai+a)
+  v4 accs[2][2] = {{acc00, acc01}, {acc10, acc11}};
+  T* Ls = L + sd.Loff[s];
+  T* Us = U + sd.Uoff[s];
+  const long long mu = m - k;
+#pragma unroll
+  for (int ai = 0; ai < 2; ai++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        long long j = j0 + jb + 16 * ai + Mfma<T>::row(lane, r);
+        long long i = i0 + ib + 16 * bi + l16;
+        if (i < m && j < m && i >= j) {
+          T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
+          *cp -= accs[ai][bi][r];
+        }
+      }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd(const int* __restrict__ sns, SymDev sd, const T* __restrict__ L,
+                                             double* __restrict__ y, double* __restrict__ V, double* __restrict__ Wg,
+                                             int cap) {
+  extern __shared__ double shw[];
+  const int s = sns[blockIdx.x];
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  double* w = (m <= cap) ? shw : Wg + sd.Woff[s];
+  const int tid = threadIdx.x, bd = blockDim.x;
+  for (long long i = tid; i < m; i += bd) w[i] = i < k ? y[f + i] : 0.0;
+  __syncthreads();
+  for (int cp = sd.child_ptr[s]; cp < sd.child_ptr[s + 1]; cp++) {
+    int c = sd.child_list[cp];
+    int mu = (int)(sn_m(sd, c) - sn_k(sd, c));
+    const int* rel = sd.rel + sd.rel_off[c];
+    const double* vc = V + sd.Voff[c];
+    for (int i = tid; i < mu; i += bd) w[rel[i]] += vc[i];
+    __syncthreads();
+  }
+  const T* P = L + sd.Loff[s];
+  for (int q = 0; q < k; q++) {
+    const T* cq = P + (long long)q * m;
+    double xq = w[q] / (double)cq[q];
+    if (tid == 0) y[f + q] = xq;
+    for (long long i = q + 1 + tid; i < m; i += bd) w[i] -= (double)cq[i] * xq;
+    __syncthreads();
+  }
+  double* vs = V + sd.Voff[s];
+  for (long long i = k + tid; i < m; i += bd) vs[i - k] = w[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev sd, const T* __restrict__ L,
+                                             double* __restrict__ y, double* __restrict__ Wg, int cap) {
+  extern __shared__ double shz[];
+  const int s = sns[blockIdx.x];
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int* rows = sd.sn_rows + sd.sn_rowptr[s];
+  double* z = (k <= cap) ? shz : Wg + sd.Woff[s];
+  const T* P = L + sd.Loff[s];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int q = wv; q < k; q += nw) {
+    const T* cq = P + (long long)q * m;
+    double acc = 0.0;
+    for (long long i = k + lane; i < m; i += 64) acc += (double)cq[i] * y[rows[i]];
+    acc = wave_sum(acc);
+    if (lane == 0) z[q] = y[f + q] - acc;
+  }
+  __syncthreads();
+  for (int q = k - 1; q >= 0; q--) {
+    double xq = z[q] / (double)P[(long long)q * m + q];
+    if (threadIdx.x == 0) y[f + q] = xq;
+    for (int p = threadIdx.x; p < q; p += blockDim.x) z[p] -= (double)P[(long long)p * m + q] * xq;
+    __syncthreads();
+  }
+}
+
+__global__ void k_perm_in(const double* __restrict__ b, const int* __restrict__ perm, double* __restrict__ y, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[k] = b[perm[k]];
+}
+__global__ void k_perm_out(const double* __restrict__ y, const int* __restrict__ perm, double* __restrict__ x, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[perm[k]] = y[k];
+}
+
+// ------------------------------------------------------------------------------------------------ host
+SpdDevice::~SpdDevice() = default;
+
+size_t SpdDevice::device_bytes() const {
+  return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
+         sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() + zero_tasks_.bytes();
+}
+
+void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* perm, const SymbolicOptions& opt) {
+  analyzed_ = false;
+  factored_ = false;
+  int rc = symbolic_analyze(n, colPtr, rowIdx, perm, opt, S_);
+  if (rc) throw Error(rc == -2 ? -2 : (rc == -1 ? -1 : -3), "symbolic analysis failed: " + S_.error);
+  const Symbolic& S = S_;
+  const int ns = S.ns;
+  // children lists (ascending)
+  std::vector<int> cptr(ns + 1, 0), clist;
+  for (int s = 0; s < ns; s++) if (S.sn_parent[s] >= 0) cptr[S.sn_parent[s] + 1]++;
+  for (int s = 0; s < ns; s++) cptr[s + 1] += cptr[s];
+  clist.resize(cptr[ns]);
+  {
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+    for (int s = 0; s < ns; s++) if (S.sn_parent[s] >= 0) clist[fill[S.sn_parent[s]]++] = s;
+  }
+  std::vector<long long> Woff(ns + 1, 0);
+  for (int s = 0; s < ns; s++) Woff[s + 1] = Woff[s] + (S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+  // level schedule and task lists
+  int nl = S.max_level + 1;
+  levels_.assign(nl, Level());
+  std::vector<std::vector<int>> bylevel(nl);
+  for (int s = 0; s < ns; s++) bylevel[S.sn_level[s]].push_back(s);
+  std::vector<int> lvl_sn;
+  std::vector<int4> ext, pan, syr, zer;
+  for (int l = 0; l < nl; l++) {
+    Level& L = levels_[l];
+    L.sn_begin = (long long)lvl_sn.size();
+    for (int s : bylevel[l]) {
+      lvl_sn.push_back(s);
+      L.max_m = std::max<long long>(L.max_m, S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+    }
+    L.sn_end = (long long)lvl_sn.size();
+    L.zero.begin = (long long)zer.size();
+    for (int s : bylevel[l]) {
+      long long mu = (S.sn_rowptr[s + 1] - S.sn_rowptr[s]) - (S.sn_first[s + 1] - S.sn_first[s]);
+      long long chunks = (mu * mu + ZERO_CHUNK - 1) / ZERO_CHUNK;
+      for (long long c = 0; c < chunks; c++) zer.push_back(make_int4(s, (int)c, 0, 0));
+    }
+    L.zero.end = (long long)zer.size();
+    // extend-add rounds: round r = r-th child of every parent at this level
+    int maxc = 0;
+    for (int s : bylevel[l]) maxc = std::max(maxc, cptr[s + 1] - cptr[s]);
+    for (int r = 0; r < maxc; r++) {
+      LaunchList ll;
+      ll.begin = (long long)ext.size();
+      for (int s : bylevel[l]) {
+        if (cptr[s + 1] - cptr[s] <= r) continue;
+        int c = clist[cptr[s] + r];
+        int kc = S.sn_first[c + 1] - S.sn_first[c];
+        int mu = (int)(S.sn_rowptr[c + 1] - S.sn_rowptr[c] - kc);
+        int j = 0;
+        while (j < mu) {
+          int j1 = j;
+          long long work = 0;
+          while (j1 < mu && work < EXT_TARGET) { work += mu - j1; j1++; }
+          ext.push_back(make_int4(c, j, j1, s));
+          j = j1;
+        }
+      }
+      ll.end = (long long)ext.size();
+      if (ll.end > ll.begin) L.ext_rounds.push_back(ll);
+    }
+    // factor steps
+    int maxk = 0;
+    for (int s : bylevel[l]) maxk = std::max(maxk, S.sn_first[s + 1] - S.sn_first[s]);
+    for (int c0 = 0; c0 < maxk; c0 += NB) {
+      LaunchList lp, ls;
+      lp.begin = (long long)pan.size();
+      ls.begin = (long long)syr.size();
+      lp.c0 = ls.c0 = c0;
+      for (int s : bylevel[l]) {
+        int k = S.sn_first[s + 1] - S.sn_first[s];
+        if (k <= c0) continue;
+        long long m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+        int w = std::min(NB, k - c0);
+        long long rb = c0 + w;
+        long long nrows = m - rb;
+        long long chunks = std::max<long long>(1, (nrows + PANEL_ROWS - 1) / PANEL_ROWS);
+        for (long long ci = 0; ci < chunks; ci++) {
+          long long r0 = rb + ci * PANEL_ROWS, r1 = std::min<long long>(m, r0 + PANEL_ROWS);
+          pan.push_back(make_int4(s, (int)r0, (int)std::max(r0, r1), ci == 0 ? 1 : 0));
+        }
+        long long tiles = (nrows + 63) / 64;
+        for (long long I = 0; I < tiles; I++)
+          for (long long J = 0; J <= I; J++) syr.push_back(make_int4(s, (int)I, (int)J, 0));
+      }
+      lp.end = (long long)pan.size();
+      ls.end = (long long)syr.size();
+      L.panel_steps.push_back(lp);
+      L.syrk_steps.push_back(ls);
+    }
+  }
+  // upload
+  perm_.upload(S.perm, stream_);
+  sn_first_.upload(S.sn_first, stream_);
+  sn_rowptr_.upload(S.sn_rowptr, stream_);
+  sn_rows_.upload(S.sn_rows, stream_);
+  Loff_.upload(S.Loff, stream_);
+  Uoff_.upload(S.Uoff, stream_);
+  Voff_.upload(S.Voff, stream_);
+  Woff_.upload(Woff, stream_);
+  rel_off_.upload(S.rel_off, stream_);
+  if (S.rel.empty()) rel_.alloc(1); else rel_.upload(S.rel, stream_);
+  child_ptr_.upload(cptr, stream_);
+  if (clist.empty()) child_list_.alloc(1); else child_list_.upload(clist, stream_);
+  amap_src_.upload(S.amap_src, stream_);
+  amap_dst_.upload(S.amap_dst, stream_);
+  if (ext.empty()) ext_tasks_.alloc(1); else ext_tasks_.upload(ext, stream_);
+  if (pan.empty()) panel_tasks_.alloc(1); else panel_tasks_.upload(pan, stream_);
+  if (syr.empty()) syrk_tasks_.alloc(1); else syrk_tasks_.upload(syr, stream_);
+  if (zer.empty()) zero_tasks_.alloc(1); else zero_tasks_.upload(zer, stream_);
+  level_sn_.upload(lvl_sn, stream_);
+  V_.alloc(std::max<long long>(1, S.Vsize));
+  W_.alloc(std::max<long long>(1, Woff[ns]));
+  y_.alloc(n);
+  err_.alloc(1);
+  sd_ = SymDev{sn_first_.p, sn_rowptr_.p, sn_rows_.p, Loff_.p, Uoff_.p, Voff_.p, Woff_.p, rel_off_.p, rel_.p,
+               child_ptr_.p, child_list_.p};
+  IFEA_HIP(hipStreamSynchronize(stream_));
+  // host copies not needed after upload
+  std::vector<int>().swap(S_.amap_src);
+  std::vector<long long>().swap(S_.amap_dst);
+  analyzed_ = true;
+}
+
+template <typename T>
+int SpdDevice::factorize_t(const double* d_vals) {
+  const Symbolic& S = S_;
+  size_t lbytes = (size_t)std::max<long long>(1, S.Lsize) * sizeof(T);
+  size_t ubytes = (size_t)std::max<long long>(1, S.Usize) * sizeof(T);
+  L_.alloc(lbytes);
+  U_.alloc(ubytes);
+  T* L = (T*)L_.p;
+  T* U = (T*)U_.p;
+  IFEA_HIP(hipMemsetAsync(L, 0, lbytes, stream_));
+  IFEA_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), stream_));
+  long long na = (long long)amap_src_.n;
+  if (na) hipLaunchKernelGGL(k_scatter<T>, dim3(grid_for(na, 256)), dim3(256), 0, stream_, d_vals, amap_src_.p,
+                             amap_dst_.p, na, L);
+  for (size_t l = 0; l < levels_.size(); l++) {
+    const Level& lv = levels_[l];
+    if (lv.zero.end > lv.zero.begin)
+      hipLaunchKernelGGL(k_zero_u<T>, dim3((unsigned)(lv.zero.end - lv.zero.begin)), dim3(256), 0, stream_,
+                         zero_tasks_.p + lv.zero.begin, sd_, U);
+    for (const LaunchList& r : lv.ext_rounds)
+      hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, stream_,
+                         ext_tasks_.p + r.begin, sd_, L, U);
+    for (size_t t = 0; t < lv.panel_steps.size(); t++) {
+      const LaunchList& p = lv.panel_steps[t];
+      if (p.end > p.begin)
+        hipLaunchKernelGGL(k_panel<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
+                           panel_tasks_.p + p.begin, p.c0, sd_, L, err_.p);
+      const LaunchList& q = lv.syrk_steps[t];
+      if (q.end > q.begin)
+        hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
+                           syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+    }
+  }
+  IFEA_HIP(hipGetLastError());
+  int herr = 0;
+  IFEA_HIP(hipMemcpyAsync(&herr, err_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  IFEA_HIP(hipStreamSynchronize(stream_));
+  return herr ? -4 : 0;
+}
+
+int SpdDevice::factorize_device(const double* d_vals, int precision) {
+  if (!analyzed_) throw Error(-5, "Analyze must be called first.");
+  hipEvent_t e0, e1;
+  IFEA_HIP(hipEventCreate(&e0));
+  IFEA_HIP(hipEventCreate(&e1));
+  IFEA_HIP(hipEventRecord(e0, stream_));
+  prec_ = precision;
+  int rc = precision == 1 ? factorize_t<float>(d_vals) : factorize_t<double>(d_vals);
+  IFEA_HIP(hipEventRecord(e1, stream_));
+  IFEA_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  IFEA_HIP(hipEventElapsedTime(&ms, e0, e1));
+  last_factor_ms_ = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  factored_ = rc == 0;
+  return rc;
+}
+
+template <typename T>
+void SpdDevice::solve_t(const double* d_b, double* d_x) {
+  const int n = S_.n;
+  const T* L = (const T*)L_.p;
+  hipLaunchKernelGGL(k_perm_in, dim3(grid_for(n, 256)), dim3(256), 0, stream_, d_b, perm_.p, y_.p, n);
+  const int cap = 8192;  // doubles of LDS per forward/backward work vector
+  for (size_t l = 0; l < levels_.size(); l++) {
+    const Level& lv = levels_[l];
+    unsigned cnt = (unsigned)(lv.sn_end - lv.sn_begin);
+    if (!cnt) continue;
+    size_t sh = (size_t)std::min<long long>(lv.max_m, cap) * sizeof(double);
+    hipLaunchKernelGGL(k_fwd<T>, dim3(cnt), dim3(256), sh, stream_, level_sn_.p + lv.sn_begin, sd_, L, y_.p, V_.p,
+                       W_.p, cap);
+  }
+  for (size_t l = levels_.size(); l-- > 0;) {
+    const Level& lv = levels_[l];
+    unsigned cnt = (unsigned)(lv.sn_end - lv.sn_begin);
+    if (!cnt) continue;
+    size_t sh = (size_t)std::min<long long>(lv.max_m, cap) * sizeof(double);
+    hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, stream_, level_sn_.p + lv.sn_begin, sd_, L, y_.p, W_.p,
+                       cap);
+  }
+  hipLaunchKernelGGL(k_perm_out, dim3(grid_for(n, 256)), dim3(256), 0, stream_, y_.p, perm_.p, d_x, n);
+  IFEA_HIP(hipGetLastError());
+}
+
+void SpdDevice::solve_device(const double* d_b, double* d_x) {
+  if (!factored_) throw Error(-5, "Factorize must be called first.");
+  if (prec_ == 1) solve_t<float>(d_b, d_x);
+  else solve_t<double>(d_b, d_x);
+}
+
+}  // namespace ifea

@@ -1,0 +1,68 @@
+// Device-resident solve step (see include/ifea_step.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../include/ifea_step.h"
+#include "common.h"
+#include "spd_device.h"
+#include "vec.h"
+
+namespace ifea {
+
+class StepDevice {
+ public:
+  StepDevice(int nn, int ne, const int* tets, const int* colPtr, const int* rowIdx, int nf, const int* faces,
+             const int* face_owner);
+  ~StepDevice();
+  void set_params(const ifea_step_params& p) { P_ = p; have_params_ = true; }
+  void set_perm(const int* perm);
+  void prepare(const double* x, const double* ax, const int* nfix, const double* fixed_val, const int* contact,
+               const double* vguess);
+  void set_element_state(const double* eps, const double* sigma);
+  // mode: 0 raw residual, 1 BC + negated residual (linear system), 2 energy only
+  double assemble(bool matrix, bool newton, const double* d_v, int mode, double* power);
+  void assemble_host(int matrix, int newton, const double* v, double* energy, double* power, double* r, double* val);
+  void solve_velocity(int warm, ifea_step_stats* st);
+  void final_eval(const int* die_axis, double* power, double* reaction, double* vg, double* r, unsigned char* release);
+  void advance(double dt, double* eps, double* sigma, double* rate);
+  void get_v(double* v);
+  void set_v(const double* v);
+  void info(double* out);
+
+ private:
+  int solve_linear(double* b, double* x, ifea_step_stats* st);
+  int pcg(const double* b, double* x, double tol, int maxIt, ifea_step_stats* st);
+  void lin_solve(const double* b, double* x, ifea_step_stats* st);
+  int refactor(int precision, ifea_step_stats* st);
+
+  hipStream_t s_ = nullptr;
+  std::unique_ptr<Vec> vec_;
+  std::unique_ptr<SpdDevice> spd_;
+  ifea_step_params P_{};
+  bool have_params_ = false, prepared_ = false;
+  int nn_, ne_, ndof_;
+  long long nnz_;
+  int max_deg_ = 0, max_patch_ = 0;
+  std::vector<int> user_perm_;
+  std::vector<int> h_colPtr_, h_rowIdx_;
+  std::vector<double> h_coords_;  // 3 per DOF, for the geometric ordering
+  bool need_analyze_ = true, have_factor_ = false;
+  int pcg_failures_ = 0, pcg_skip_ = 0;
+  // topology (device)
+  DBuf<int> tets_, nt_start_, nt_list_, colPtr_, rowIdx_, nbr_start_, nbr_, p_start_, p_node_, diagpos_;
+  DBuf<int> nf_start_, nf_list_, faces_, face_owner_;
+  // per increment
+  DBuf<double> x_, ax_, fixed_val_, grad_, vol_, vnode_, gcoef_, kvol_, afric_, kfric_;
+  DBuf<int> nfix_, contact_, bad_;
+  // state
+  DBuf<double> eps_, el_sig_, el_rate_, el_dev_, el_mean_;
+  // iteration
+  DBuf<double> v_, r_, dv_, vtrial_, val_, e_wm_, e_wn_, e_g_, e_energy_, e_power_, n_energy_, n_power_;
+  DBuf<double> pr_, pz_, pp_, pq_, tmp_;
+  DBuf<unsigned long long> maxdiag_;
+};
+
+}  // namespace ifea

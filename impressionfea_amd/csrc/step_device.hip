@@ -1,0 +1,884 @@
+// Device-resident solve step of the 3D rigid-viscoplastic solver (ImpressionFEA Forging3DSolver hot path) on MI355X.
+//
+// Data layout in HBM (structure of arrays, FP64):
+//   per node      x[3nn], frames ax[9nn] (rows ax0 ax1 ax2), nfix, contact, v/r/dv[3nn] in local frames
+//   per element   tets[4ne], grad[12ne], vol, eps, sigma, rate, dev[6ne], and per assembly wm = vol*mu,
+//                 wn = vol*coefN*blend, G[12ne], energy, power
+//   per pattern   the reference's full symmetric CSC (colPtr, rowIdx, val, kvol), 3x3 node blocks
+// Assembly is a two-pass, atomic-free gather: an element pass (strain rate, flow stress, G = B^T M eps) and a node
+// pass where one wave owns the 3 CSC columns of a node and adds its incident elements' rotated 3x3 blocks in
+// ascending element order, as the reference's serial scatter does (Forging3DSolver.cs:1146-1171).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "step_device.h"
+
+namespace ifea {
+
+constexpr int MAX_DEG = 192;    // two-ring neighbours per node
+constexpr int MAX_PATCH = 64;   // one-ring patch size per node
+constexpr int NODE_WAVES = 4;   // nodes per 256-thread block
+
+__device__ __forceinline__ double law_stress(const ifea_step_params& P, double strain, double rate) {
+  return P.C * pow(P.eps0 + fmax(strain, 0.0), P.n) * pow(fmax(rate, 1e-12), P.m);
+}
+
+__device__ __forceinline__ int bsearch_lds(const int* a, int n, int key) {
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int v = a[mid];
+    if (v == key) return mid;
+    if (v < key) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return __shfl(v, 0, 64);
+}
+
+// ------------------------------------------------------------------------------- PrepareStep (:730-753)
+__global__ void k_geom(int ne, const int* __restrict__ tets, const double* __restrict__ x, double* __restrict__ grad,
+                       double* __restrict__ vol, int* __restrict__ bad) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int* t = tets + 4 * e;
+  double p0x = x[3 * t[0]], p0y = x[3 * t[0] + 1], p0z = x[3 * t[0] + 2];
+  double ax = x[3 * t[1]] - p0x, ay = x[3 * t[1] + 1] - p0y, az = x[3 * t[1] + 2] - p0z;
+  double bx = x[3 * t[2]] - p0x, by = x[3 * t[2] + 1] - p0y, bz = x[3 * t[2] + 2] - p0z;
+  double cx = x[3 * t[3]] - p0x, cy = x[3 * t[3] + 1] - p0y, cz = x[3 * t[3] + 2] - p0z;
+  // b x c, c x a, a x b
+  double bcx = by * cz - bz * cy, bcy = bz * cx - bx * cz, bcz = bx * cy - by * cx;
+  double cax = cy * az - cz * ay, cay = cz * ax - cx * az, caz = cx * ay - cy * ax;
+  double abx = ay * bz - az * by, aby = az * bx - ax * bz, abz = ax * by - ay * bx;
+  double det = ax * bcx + ay * bcy + az * bcz;
+  if (!(det > 0)) atomicMin(bad, e);
+  double inv = 1.0 / det;
+  double g1x = inv * bcx, g1y = inv * bcy, g1z = inv * bcz;
+  double g2x = inv * cax, g2y = inv * cay, g2z = inv * caz;
+  double g3x = inv * abx, g3y = inv * aby, g3z = inv * abz;
+  double* g = grad + 12 * (long long)e;
+  g[0] = -(g1x + g2x + g3x); g[1] = -(g1y + g2y + g3y); g[2] = -(g1z + g2z + g3z);
+  g[3] = g1x; g[4] = g1y; g[5] = g1z;
+  g[6] = g2x; g[7] = g2y; g[8] = g2z;
+  g[9] = g3x; g[10] = g3y; g[11] = g3z;
+  vol[e] = det / 6;
+}
+
+// BuildAnp (:761-823): nodal volumes, nodal gradient operator, penalty matrix
+__global__ void k_vnode(int nn, const int* __restrict__ nts, const int* __restrict__ ntl, const double* __restrict__ vol,
+                        double* __restrict__ vnode) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double s = 0.0;
+  for (int q = nts[n]; q < nts[n + 1]; q++) s += vol[ntl[q]] / 4;
+  vnode[n] = s;
+}
+
+__global__ void k_gcoef(int nn, const int* __restrict__ tets, const int* __restrict__ nts, const int* __restrict__ ntl,
+                        const int* __restrict__ ps, const int* __restrict__ pn, const double* __restrict__ vol,
+                        const double* __restrict__ vnode, const double* __restrict__ grad, double* __restrict__ gcoef) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const int p0 = ps[n], np = ps[n + 1] - p0;
+  for (int t = 0; t < 3 * np; t++) gcoef[3 * (long long)p0 + t] = 0.0;
+  for (int q = nts[n]; q < nts[n + 1]; q++) {
+    int e = ntl[q];
+    double w = vol[e] / 4 / vnode[n];
+    for (int a = 0; a < 4; a++) {
+      int m = tets[4 * e + a];
+      int lo = 0, hi = np - 1, sl = 0;
+      while (lo <= hi) { int mid = (lo + hi) >> 1; int v = pn[p0 + mid]; if (v == m) { sl = mid; break; } if (v < m) lo = mid + 1; else hi = mid - 1; }
+      const double* g = grad + 12 * (long long)e + 3 * a;
+      double* c = gcoef + 3 * ((long long)p0 + sl);
+      c[0] += w * g[0]; c[1] += w * g[1]; c[2] += w * g[2];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_kvol(int nn, double kpen, const int* __restrict__ colPtr,
+                                              const int* __restrict__ nbs, const int* __restrict__ nb,
+                                              const int* __restrict__ ps, const int* __restrict__ pn,
+                                              const double* __restrict__ ax, const double* __restrict__ vnode,
+                                              const double* __restrict__ gcoef, double* __restrict__ kvol) {
+  __shared__ int s_nbr[NODE_WAVES][MAX_DEG];
+  __shared__ double s_acc[NODE_WAVES][9 * MAX_DEG];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j = blockIdx.x * NODE_WAVES + wv;
+  if (j >= nn) return;
+  const int deg = nbs[j + 1] - nbs[j];
+  int* nbr = s_nbr[wv];
+  double* acc = s_acc[wv];
+  for (int q = lane; q < deg; q += 64) nbr[q] = nb[nbs[j] + q];
+  for (int q = lane; q < 9 * deg; q += 64) acc[q] = 0.0;
+  wave_sync_lds();
+  const double* Rj = ax + 9 * (long long)j;
+  const int d3 = 3 * deg;
+  for (int pp = ps[j]; pp < ps[j + 1]; pp++) {
+    const int n = pn[pp];
+    const double kv = kpen * vnode[n];
+    const int p0 = ps[n], np = ps[n + 1] - p0;
+    int sj = 0;
+    {
+      int lo = 0, hi = np - 1;
+      while (lo <= hi) { int mid = (lo + hi) >> 1; int v = pn[p0 + mid]; if (v == j) { sj = mid; break; } if (v < j) lo = mid + 1; else hi = mid - 1; }
+    }
+    const double* gj = gcoef + 3 * ((long long)p0 + sj);
+    double cj[3];
+    for (int c = 0; c < 3; c++) cj[c] = gj[0] * Rj[3 * c] + gj[1] * Rj[3 * c + 1] + gj[2] * Rj[3 * c + 2];
+    if (lane < np) {
+      const int i = pn[p0 + lane];
+      const double* gi = gcoef + 3 * ((long long)p0 + lane);
+      const double* Ri = ax + 9 * (long long)i;
+      double ci[3];
+      for (int r = 0; r < 3; r++) ci[r] = gi[0] * Ri[3 * r] + gi[1] * Ri[3 * r + 1] + gi[2] * Ri[3 * r + 2];
+      const int pos = bsearch_lds(nbr, deg, i);
+      if (pos >= 0) {
+        for (int c = 0; c < 3; c++)
+          for (int r = 0; r < 3; r++) acc[c * d3 + 3 * pos + r] += kv * (ci[r] * cj[c]);
+      }
+    }
+    wave_sync_lds();
+  }
+  for (int c = 0; c < 3; c++) {
+    double* dst = kvol + colPtr[3 * j + c];
+    for (int q = lane; q < d3; q += 64) dst[q] = acc[c * d3 + q];
+  }
+}
+
+// BuildFriction (:829-850): faces of each node in the reference's face order
+__global__ void k_fric(int nn, const int* __restrict__ nfs, const int* __restrict__ nfl, const int* __restrict__ faces,
+                       const int* __restrict__ owner, const double* __restrict__ x, const double* __restrict__ sig,
+                       double* __restrict__ afric, double* __restrict__ kfric) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double a = 0.0, ks = 0.0;
+  int cnt = 0;
+  for (int q = nfs[n]; q < nfs[n + 1]; q++) {
+    int f = nfl[q];
+    int ia = faces[3 * f], ib = faces[3 * f + 1], ic = faces[3 * f + 2];
+    double ux = x[3 * ib] - x[3 * ia], uy = x[3 * ib + 1] - x[3 * ia + 1], uz = x[3 * ib + 2] - x[3 * ia + 2];
+    double wx = x[3 * ic] - x[3 * ia], wy = x[3 * ic + 1] - x[3 * ia + 1], wz = x[3 * ic + 2] - x[3 * ia + 2];
+    double cx = uy * wz - uz * wy, cy = uz * wx - ux * wz, cz = ux * wy - uy * wx;
+    double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+    a += area / 3;
+    ks += sig[owner[f]] / sqrt(3.0);
+    cnt++;
+  }
+  afric[n] = a;
+  kfric[n] = cnt > 0 ? ks / cnt : 0.0;
+}
+
+// LoadVelocityGuess (:854-869)
+__global__ void k_vguess(int nn, const double* __restrict__ ax, const int* __restrict__ nfix,
+                         const double* __restrict__ fixed_val, const double* __restrict__ vg, double* __restrict__ v) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const double* R = ax + 9 * (long long)n;
+  for (int q = 0; q < 3; q++) {
+    double l = vg[3 * n] * R[3 * q] + vg[3 * n + 1] * R[3 * q + 1] + vg[3 * n + 2] * R[3 * q + 2];
+    v[3 * n + q] = q < nfix[n] ? fixed_val[3 * n + q] : l;
+  }
+}
+
+// ------------------------------------------------------------------------------- Kernel (:1312-1400), element pass
+__global__ __launch_bounds__(256) void k_elem(int ne, ifea_step_params P, int newton, const int* __restrict__ tets,
+                                              const double* __restrict__ ax, const double* __restrict__ v,
+                                              const double* __restrict__ grad, const double* __restrict__ vol,
+                                              const double* __restrict__ eps, double* __restrict__ wm_out,
+                                              double* __restrict__ wn_out, double* __restrict__ g_out,
+                                              double* __restrict__ energy_out, double* __restrict__ power_out,
+                                              double* __restrict__ el_sig, double* __restrict__ el_rate,
+                                              double* __restrict__ el_dev) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double ve[12], g[12];
+  const double* ge = grad + 12 * (long long)e;
+#pragma unroll
+  for (int k = 0; k < 12; k++) g[k] = ge[k];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    int n = tets[4 * e + a];
+    const double* R = ax + 9 * (long long)n;
+    double l0 = v[3 * n], l1 = v[3 * n + 1], l2 = v[3 * n + 2];
+    // ToGlobal (:827): l0*ax0 + l1*ax1 + l2*ax2
+    ve[3 * a] = l0 * R[0] + l1 * R[3] + l2 * R[6];
+    ve[3 * a + 1] = l0 * R[1] + l1 * R[4] + l2 * R[7];
+    ve[3 * a + 2] = l0 * R[2] + l1 * R[5] + l2 * R[8];
+  }
+  // eps = B ve (engineering shear)
+  double e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double gx = g[3 * a], gy = g[3 * a + 1], gz = g[3 * a + 2];
+    double vx = ve[3 * a], vy = ve[3 * a + 1], vz = ve[3 * a + 2];
+    e0 += gx * vx;
+    e1 += gy * vy;
+    e2 += gz * vz;
+    e3 += gy * vx + gx * vy;
+    e4 += gz * vy + gy * vz;
+    e5 += gz * vx + gx * vz;
+  }
+  double ev3 = (e0 + e1 + e2) / 3;
+  double d0 = e0 - ev3, d1 = e1 - ev3, d2 = e2 - ev3;
+  double rate = sqrt(fmax(2.0 / 3.0 * (d0 * d0 + d1 * d1 + d2 * d2) + (e3 * e3 + e4 * e4 + e5 * e5) / 3.0, 0.0));
+  double me0 = 2.0 / 3.0 * d0, me1 = 2.0 / 3.0 * d1, me2 = 2.0 / 3.0 * d2, me3 = e3 / 3, me4 = e4 / 3, me5 = e5 / 3;
+  double strain = eps[e];
+  double sigCut = law_stress(P, strain, P.rate_cut);
+  double mu0 = sigCut / P.rate_cut;
+  double mu, sig, wd, coefN;
+  if (rate > P.rate_cut) {
+    sig = law_stress(P, strain, rate);
+    mu = sig / rate;
+    wd = 0.5 * sigCut * P.rate_cut + (sig * rate - sigCut * P.rate_cut) / (P.m + 1);
+    coefN = (P.m - 1) * mu / (rate * rate);
+  } else {
+    mu = mu0;
+    sig = mu0 * rate;
+    wd = 0.5 * mu0 * rate * rate;
+    coefN = 0;
+  }
+  double ve_ = vol[e];
+  energy_out[e] = ve_ * wd;
+  power_out[e] = ve_ * sig * rate;
+  el_sig[e] = sig;
+  el_rate[e] = rate;
+  double* dv = el_dev + 6 * (long long)e;
+  dv[0] = mu * me0; dv[1] = mu * me1; dv[2] = mu * me2; dv[3] = mu * me3; dv[4] = mu * me4; dv[5] = mu * me5;
+  wm_out[e] = ve_ * mu;
+  wn_out[e] = newton ? ve_ * coefN * P.newton_blend : 0.0;
+  double* G = g_out + 12 * (long long)e;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double gx = g[3 * a], gy = g[3 * a + 1], gz = g[3 * a + 2];
+    G[3 * a] = gx * me0 + gy * me3 + gz * me5;
+    G[3 * a + 1] = gy * me1 + gx * me3 + gz * me4;
+    G[3 * a + 2] = gz * me2 + gy * me4 + gx * me5;
+  }
+}
+
+// ------------------------------------------------------------------------------- Assemble (:1115-1217), node pass
+// One wave per node j: its 3 CSC columns (and its 3 residual rows by symmetry). mode 0: raw residual; mode 1:
+// BC identity rows/cols, r[fixed] = 0 and r = -r (SolveVelocity :892-917); mode 2: energy only.
+__global__ __launch_bounds__(256) void k_node(int nn, ifea_step_params P, int matrix, int newton, int mode,
+                                              const int* __restrict__ tets, const int* __restrict__ nts,
+                                              const int* __restrict__ ntl, const int* __restrict__ colPtr,
+                                              const int* __restrict__ rowIdx, const int* __restrict__ nbs,
+                                              const int* __restrict__ nb, const double* __restrict__ ax,
+                                              const int* __restrict__ nfix, const int* __restrict__ contact,
+                                              const double* __restrict__ afric, const double* __restrict__ kfric,
+                                              const double* __restrict__ kvol, const double* __restrict__ v,
+                                              const double* __restrict__ grad, const double* __restrict__ wm,
+                                              const double* __restrict__ wn, const double* __restrict__ G,
+                                              double* __restrict__ r, double* __restrict__ val,
+                                              double* __restrict__ n_energy, double* __restrict__ n_power,
+                                              unsigned long long* __restrict__ maxdiag) {
+  __shared__ int s_nbr[NODE_WAVES][MAX_DEG];
+  __shared__ double s_acc[NODE_WAVES][9 * MAX_DEG];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j = blockIdx.x * NODE_WAVES + wv;
+  if (j >= nn) return;
+  const int deg = nbs[j + 1] - nbs[j];
+  const int d3 = 3 * deg;
+  int* nbr = s_nbr[wv];
+  double* acc = s_acc[wv];
+  if (matrix) {
+    for (int q = lane; q < deg; q += 64) nbr[q] = nb[nbs[j] + q];
+    for (int c = 0; c < 3; c++) {
+      const double* src = kvol + colPtr[3 * j + c];
+      for (int q = lane; q < d3; q += 64) acc[c * d3 + q] = src[q];
+    }
+    wave_sync_lds();
+  }
+  // r_vol = K_vol v (row j = column j, symmetric); energy 1/2 v.r_vol
+  double rj[3];
+  for (int c = 0; c < 3; c++) {
+    double s = 0.0;
+    for (int p = colPtr[3 * j + c] + lane; p < colPtr[3 * j + c + 1]; p += 64) s += kvol[p] * v[rowIdx[p]];
+    rj[c] = wsum(s);
+  }
+  double en = 0.5 * v[3 * j] * rj[0] + 0.5 * v[3 * j + 1] * rj[1] + 0.5 * v[3 * j + 2] * rj[2];
+  double pw = 2.0 * en;
+  const double* Rj = ax + 9 * (long long)j;
+  const int la = lane / 9, lr = (lane / 3) % 3, lc = lane % 3;
+  // elements around j in ascending order
+  for (int q = nts[j]; q < nts[j + 1]; q++) {
+    const int e = ntl[q];
+    const int* t = tets + 4 * e;
+    const int b = t[0] == j ? 0 : (t[1] == j ? 1 : (t[2] == j ? 2 : 3));
+    const double w = wm[e];
+    const double* Ge = G + 12 * (long long)e;
+    // residual: ToLocal(j, wm * G_b)
+    double reb0 = w * Ge[3 * b], reb1 = w * Ge[3 * b + 1], reb2 = w * Ge[3 * b + 2];
+    for (int c = 0; c < 3; c++) rj[c] += reb0 * Rj[3 * c] + reb1 * Rj[3 * c + 1] + reb2 * Rj[3 * c + 2];
+    if (matrix && lane < 36) {
+      const int i = t[la];
+      const double* ga = grad + 12 * (long long)e + 3 * la;
+      const double* gb = grad + 12 * (long long)e + 3 * b;
+      const double wnn = wn[e];
+      const double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+      // Ke_ab[c][d] = wm * (B^T M B)_{3a+c, 3b+d} + wn * G_{3a+c} G_{3b+d}
+      double K[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          double dl = c == d ? 1.0 : 0.0;
+          double bmb = 2.0 / 3.0 * ga[c] * gb[d] * (dl - 1.0 / 3.0) +
+                       (dl * (gab - 2.0 * ga[c] * gb[c]) + ga[d] * gb[c]) / 3.0;
+          K[c][d] = w * bmb + wnn * Ge[3 * la + c] * Ge[3 * b + d];
+        }
+      const double* Ri = ax + 9 * (long long)i;
+      // blk[r][c'] = sum_c sum_d Ri[r][c] K[c][d] Rj[c'][d]
+      double sum = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double kr = K[c][0] * Rj[3 * lc] + K[c][1] * Rj[3 * lc + 1] + K[c][2] * Rj[3 * lc + 2];
+        sum += Ri[3 * lr + c] * kr;
+      }
+      const int pos = bsearch_lds(nbr, deg, i);
+      if (pos >= 0) acc[lc * d3 + 3 * pos + lr] += sum;
+    }
+    if (matrix) wave_sync_lds();
+  }
+  // friction on the free axes of contact nodes (:1175-1214)
+  const int nf = nfix[j];
+  const int ct = contact[j];
+  double fk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  bool have_fk = false;
+  if (P.friction > 0 && ct != 0 && nf < 3) {
+    double c = P.friction * kfric[j] * afric[j] * (2 / M_PI);
+    if (c > 0) {
+      // die velocity in local axes: top die moves with (0, 0, -ram)
+      double vd[3];
+      for (int qq = 0; qq < 3; qq++) vd[qq] = ct == 2 ? -P.ram * Rj[3 * qq + 2] : 0.0;
+      double s[3] = {0, 0, 0};
+      double s2 = 0;
+      for (int qq = nf; qq < 3; qq++) { s[qq] = v[3 * j + qq] - vd[qq]; s2 += s[qq] * s[qq]; }
+      double sl = sqrt(s2), z = sl / P.u0, at = atan(z);
+      en += c * (sl * at - 0.5 * P.u0 * log(1 + z * z));
+      pw += c * at * sl;
+      double gs = sl > 1e-12 * P.u0 ? at / sl : 1 / P.u0;
+      for (int qq = nf; qq < 3; qq++) rj[qq] += c * gs * s[qq];
+      if (matrix) {
+        double tang = c / (P.u0 * (1 + z * z));
+        for (int qq = nf; qq < 3; qq++)
+          for (int p = nf; p < 3; p++) {
+            double kqp;
+            if (newton && sl > 1e-12 * P.u0) kqp = c * gs * ((qq == p ? 1 : 0) - s[qq] * s[p] / s2) + tang * s[qq] * s[p] / s2;
+            else kqp = qq == p ? c * gs : 0;
+            fk[qq][p] = kqp;
+          }
+        have_fk = true;
+      }
+    }
+  }
+  if (lane == 0) { n_energy[j] = en; n_power[j] = pw; }
+  if (mode == 2) return;
+  if (matrix) {
+    const int pj = bsearch_lds(nbr, deg, j);
+    if (have_fk && lane < 9 && pj >= 0) {
+      int qq = lane / 3, p = lane % 3;
+      acc[p * d3 + 3 * pj + qq] += fk[qq][p];
+    }
+    wave_sync_lds();
+    double dmax = 0.0;
+    if (mode == 1) {
+      // identity rows/columns for fixed DOFs, largest free diagonal
+      for (int c = 0; c < 3; c++) {
+        const bool fcol = c < nf;
+        for (int q = lane; q < d3; q += 64) {
+          const int pos = q / 3, rr = q % 3;
+          const int i = nbr[pos];
+          const bool frow = rr < nfix[i];
+          const bool diag = (i == j) && (rr == c);
+          double vv = acc[c * d3 + q];
+          if (fcol || frow) vv = diag ? 1.0 : 0.0;
+          else if (diag) dmax = fmax(dmax, vv);
+          acc[c * d3 + q] = vv;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_down(dmax, off, 64));
+      if (lane == 0) atomicMax(maxdiag, (unsigned long long)__double_as_longlong(fmax(dmax, 0.0)));
+    }
+    for (int c = 0; c < 3; c++) {
+      double* dst = val + colPtr[3 * j + c];
+      for (int q = lane; q < d3; q += 64) dst[q] = acc[c * d3 + q];
+    }
+  }
+  if (lane < 3) {
+    double rv = rj[lane];
+    if (mode == 1) rv = lane < nf ? -0.0 : -rv;
+    r[3 * j + lane] = rv;
+  }
+}
+
+__global__ void k_reg(int nn, const int* __restrict__ nfix, const int* __restrict__ diagpos,
+                      const unsigned long long* __restrict__ maxdiag, double* __restrict__ val) {
+  int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= 3 * nn) return;
+  if (d % 3 < nfix[d / 3]) return;
+  double reg = 1e-10 * __longlong_as_double((long long)*maxdiag);
+  val[diagpos[d]] += reg;
+}
+
+__global__ void k_trial(long long n, const double* __restrict__ v, double beta, const double* __restrict__ dv,
+                        double* __restrict__ vt) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) vt[i] = v[i] + beta * dv[i];
+}
+
+// FinalEvaluation (:1402-1451) node part
+__global__ void k_final_nodes(int nn, const double* __restrict__ ax, const double* __restrict__ v,
+                              const double* __restrict__ r, const int* __restrict__ contact,
+                              const int* __restrict__ die_axis, double* __restrict__ vg, double* __restrict__ rz,
+                              double* __restrict__ comp, unsigned long long* __restrict__ maxtb) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const double* R = ax + 9 * (long long)n;
+  double l0 = v[3 * n], l1 = v[3 * n + 1], l2 = v[3 * n + 2];
+  for (int c = 0; c < 3; c++) vg[3 * n + c] = l0 * R[c] + l1 * R[3 + c] + l2 * R[6 + c];
+  double rzv = 0.0, cp = 0.0;
+  int ct = contact[n];
+  if (ct != 0) {
+    double q0 = r[3 * n], q1 = r[3 * n + 1], q2 = r[3 * n + 2];
+    double gz = q0 * R[2] + q1 * R[5] + q2 * R[8];
+    if (ct == 2) rzv = -gz;
+    int da = die_axis[n];
+    if (da >= 0) {
+      cp = r[3 * n + da];
+      atomicMax(maxtb + (ct == 2 ? 0 : 1), (unsigned long long)__double_as_longlong(fmax(cp, 0.0)));
+    }
+  }
+  rz[n] = rzv;
+  comp[n] = cp;
+}
+
+__global__ void k_advance(int ne, double dt, const double* __restrict__ rate, double* __restrict__ eps) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) eps[e] += rate[e] * dt;
+}
+
+// ------------------------------------------------------------------------------- host
+StepDevice::StepDevice(int nn, int ne, const int* tets, const int* colPtr, const int* rowIdx, int nf, const int* faces,
+                       const int* face_owner)
+    : nn_(nn), ne_(ne), ndof_(3 * nn) {
+  if (nn <= 0 || ne <= 0 || !tets || !colPtr || !rowIdx) throw Error(-1, "empty mesh");
+  IFEA_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  vec_.reset(new Vec(s_));
+  spd_.reset(new SpdDevice(s_));
+  nnz_ = colPtr[ndof_];
+  h_colPtr_.assign(colPtr, colPtr + ndof_ + 1);
+  h_rowIdx_.assign(rowIdx, rowIdx + nnz_);
+  // node -> tets (ascending), TetMesh.cs:18-29
+  std::vector<int> nts(nn + 1, 0), ntl(4 * (size_t)ne);
+  for (size_t k = 0; k < 4 * (size_t)ne; k++) {
+    if (tets[k] < 0 || tets[k] >= nn) throw Error(-1, "tet node index out of range");
+    nts[tets[k] + 1]++;
+  }
+  for (int n = 0; n < nn; n++) nts[n + 1] += nts[n];
+  {
+    std::vector<int> fill(nts.begin(), nts.end() - 1);
+    for (int e = 0; e < ne; e++) for (int a = 0; a < 4; a++) ntl[fill[tets[4 * e + a]]++] = e;
+  }
+  // neighbour lists from the pattern: column 3j lists rows 3i, 3i+1, 3i+2 for i in nbr[j]
+  std::vector<int> nbs(nn + 1, 0), nbv;
+  std::vector<int> diagpos(ndof_);
+  for (int j = 0; j < nn; j++) {
+    int p0 = colPtr[3 * j], p1 = colPtr[3 * j + 1];
+    if ((p1 - p0) % 3) throw Error(-1, "pattern is not made of 3x3 node blocks");
+    int deg = (p1 - p0) / 3;
+    if (deg > MAX_DEG) throw Error(-1, "node with more than 192 pattern neighbours");
+    max_deg_ = std::max(max_deg_, deg);
+    int pj = -1;
+    for (int q = 0; q < deg; q++) {
+      int i = rowIdx[p0 + 3 * q] / 3;
+      if (i == j) pj = q;
+      nbv.push_back(i);
+    }
+    if (pj < 0) throw Error(-1, "pattern misses a diagonal block");
+    for (int c = 0; c < 3; c++) {
+      if (colPtr[3 * j + c + 1] - colPtr[3 * j + c] != 3 * deg) throw Error(-1, "pattern is not made of 3x3 node blocks");
+      diagpos[3 * j + c] = colPtr[3 * j + c] + 3 * pj + c;
+    }
+    nbs[j + 1] = (int)nbv.size();
+  }
+  // every pair of nodes of an element must be coupled in the pattern (the assembly gathers rely on it)
+  for (int e = 0; e < ne; e++)
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        int i = tets[4 * e + a], j = tets[4 * e + b];
+        if (!std::binary_search(nbv.begin() + nbs[j], nbv.begin() + nbs[j + 1], i))
+          throw Error(-1, "pattern does not couple the nodes of element " + std::to_string(e));
+      }
+  // one-ring patches (sorted)
+  std::vector<int> ps(nn + 1, 0), pnv;
+  std::vector<int> tmp;
+  for (int n = 0; n < nn; n++) {
+    tmp.clear();
+    for (int q = nts[n]; q < nts[n + 1]; q++) for (int a = 0; a < 4; a++) tmp.push_back(tets[4 * ntl[q] + a]);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    if ((int)tmp.size() > MAX_PATCH) throw Error(-1, "node patch larger than 64 nodes");
+    max_patch_ = std::max(max_patch_, (int)tmp.size());
+    pnv.insert(pnv.end(), tmp.begin(), tmp.end());
+    ps[n + 1] = (int)pnv.size();
+  }
+  // faces of each node in face order
+  std::vector<int> nfs(nn + 1, 0), nfl;
+  for (int f = 0; f < nf; f++) for (int a = 0; a < 3; a++) nfs[faces[3 * f + a] + 1]++;
+  for (int n = 0; n < nn; n++) nfs[n + 1] += nfs[n];
+  nfl.resize(nfs[nn]);
+  {
+    std::vector<int> fill(nfs.begin(), nfs.end() - 1);
+    for (int f = 0; f < nf; f++) for (int a = 0; a < 3; a++) nfl[fill[faces[3 * f + a]]++] = f;
+  }
+  tets_.upload(tets, 4 * (size_t)ne, s_);
+  nt_start_.upload(nts, s_);
+  nt_list_.upload(ntl, s_);
+  colPtr_.upload(colPtr, ndof_ + 1, s_);
+  rowIdx_.upload(rowIdx, nnz_, s_);
+  nbr_start_.upload(nbs, s_);
+  nbr_.upload(nbv, s_);
+  p_start_.upload(ps, s_);
+  p_node_.upload(pnv, s_);
+  diagpos_.upload(diagpos, s_);
+  nf_start_.upload(nfs, s_);
+  if (nfl.empty()) nf_list_.alloc(1); else nf_list_.upload(nfl, s_);
+  if (nf) { faces_.upload(faces, 3 * (size_t)nf, s_); face_owner_.upload(face_owner, nf, s_); }
+  else { faces_.alloc(3); face_owner_.alloc(1); }
+  x_.alloc(ndof_); ax_.alloc(9 * (size_t)nn); fixed_val_.alloc(ndof_);
+  grad_.alloc(12 * (size_t)ne); vol_.alloc(ne); vnode_.alloc(nn); gcoef_.alloc(3 * pnv.size());
+  kvol_.alloc(nnz_); afric_.alloc(nn); kfric_.alloc(nn);
+  nfix_.alloc(nn); contact_.alloc(nn); bad_.alloc(1);
+  eps_.alloc(ne); el_sig_.alloc(ne); el_rate_.alloc(ne); el_dev_.alloc(6 * (size_t)ne); el_mean_.alloc(ne);
+  IFEA_HIP(hipMemsetAsync(eps_.p, 0, ne * sizeof(double), s_));
+  IFEA_HIP(hipMemsetAsync(el_sig_.p, 0, ne * sizeof(double), s_));
+  IFEA_HIP(hipMemsetAsync(el_rate_.p, 0, ne * sizeof(double), s_));
+  v_.alloc(ndof_); r_.alloc(ndof_); dv_.alloc(ndof_); vtrial_.alloc(ndof_); val_.alloc(nnz_);
+  e_wm_.alloc(ne); e_wn_.alloc(ne); e_g_.alloc(12 * (size_t)ne); e_energy_.alloc(ne); e_power_.alloc(ne);
+  n_energy_.alloc(nn); n_power_.alloc(nn);
+  pr_.alloc(ndof_); pz_.alloc(ndof_); pp_.alloc(ndof_); pq_.alloc(ndof_); tmp_.alloc(3 * (size_t)nn);
+  maxdiag_.alloc(4);
+  IFEA_HIP(hipMemsetAsync(v_.p, 0, ndof_ * sizeof(double), s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+StepDevice::~StepDevice() {
+  spd_.reset();
+  vec_.reset();
+  if (s_) (void)hipStreamDestroy(s_);
+}
+
+void StepDevice::set_perm(const int* perm) {
+  if (perm) user_perm_.assign(perm, perm + ndof_);
+  else user_perm_.clear();
+  need_analyze_ = true;
+  have_factor_ = false;
+}
+
+void StepDevice::prepare(const double* x, const double* ax, const int* nfix, const double* fixed_val,
+                         const int* contact, const double* vguess) {
+  if (!have_params_) throw Error(-5, "parameters must be set before PrepareStep");
+  x_.upload(x, ndof_, s_);
+  if (need_analyze_) {
+    h_coords_.resize(3 * (size_t)ndof_);
+    for (int n = 0; n < nn_; n++)
+      for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 3; a++) h_coords_[3 * (3 * (size_t)n + c) + a] = x[3 * n + a];
+  }
+  ax_.upload(ax, 9 * (size_t)nn_, s_);
+  nfix_.upload(nfix, nn_, s_);
+  fixed_val_.upload(fixed_val, ndof_, s_);
+  contact_.upload(contact, nn_, s_);
+  IFEA_HIP(hipMemcpyAsync(tmp_.p, vguess, ndof_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  int big = 0x7fffffff;
+  IFEA_HIP(hipMemcpyAsync(bad_.p, &big, sizeof(int), hipMemcpyHostToDevice, s_));
+  hipLaunchKernelGGL(k_geom, dim3(grid_for(ne_, 256)), dim3(256), 0, s_, ne_, tets_.p, x_.p, grad_.p, vol_.p, bad_.p);
+  int hb = 0;
+  IFEA_HIP(hipMemcpyAsync(&hb, bad_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+  if (hb != 0x7fffffff) throw Error(-1, "Tetrahedron " + std::to_string(hb) + " is inverted");
+  hipLaunchKernelGGL(k_vnode, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, nt_start_.p, nt_list_.p, vol_.p, vnode_.p);
+  hipLaunchKernelGGL(k_gcoef, dim3(grid_for(nn_, 128)), dim3(128), 0, s_, nn_, tets_.p, nt_start_.p, nt_list_.p,
+                     p_start_.p, p_node_.p, vol_.p, vnode_.p, grad_.p, gcoef_.p);
+  hipLaunchKernelGGL(k_kvol, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), 0, s_, nn_, P_.kpen, colPtr_.p, nbr_start_.p,
+                     nbr_.p, p_start_.p, p_node_.p, ax_.p, vnode_.p, gcoef_.p, kvol_.p);
+  hipLaunchKernelGGL(k_fric, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, nf_start_.p, nf_list_.p, faces_.p,
+                     face_owner_.p, x_.p, el_sig_.p, afric_.p, kfric_.p);
+  hipLaunchKernelGGL(k_vguess, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, ax_.p, nfix_.p, fixed_val_.p, tmp_.p,
+                     v_.p);
+  IFEA_HIP(hipGetLastError());
+  prepared_ = true;
+}
+
+void StepDevice::set_element_state(const double* eps, const double* sigma) {
+  if (eps) IFEA_HIP(hipMemcpyAsync(eps_.p, eps, ne_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  if (sigma) IFEA_HIP(hipMemcpyAsync(el_sig_.p, sigma, ne_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+double StepDevice::assemble(bool matrix, bool newton, const double* d_v, int mode, double* power) {
+  hipLaunchKernelGGL(k_elem, dim3(grid_for(ne_, 256)), dim3(256), 0, s_, ne_, P_, newton ? 1 : 0, tets_.p, ax_.p, d_v,
+                     grad_.p, vol_.p, eps_.p, e_wm_.p, e_wn_.p, e_g_.p, e_energy_.p, e_power_.p, el_sig_.p,
+                     el_rate_.p, el_dev_.p);
+  if (mode == 1) IFEA_HIP(hipMemsetAsync(maxdiag_.p, 0, sizeof(unsigned long long), s_));
+  hipLaunchKernelGGL(k_node, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), 0, s_, nn_, P_, matrix ? 1 : 0,
+                     newton ? 1 : 0, mode, tets_.p, nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p, nbr_start_.p,
+                     nbr_.p, ax_.p, nfix_.p, contact_.p, afric_.p, kfric_.p, kvol_.p, d_v, grad_.p, e_wm_.p, e_wn_.p,
+                     e_g_.p, r_.p, val_.p, n_energy_.p, n_power_.p, maxdiag_.p);
+  if (matrix && mode == 1)
+    hipLaunchKernelGGL(k_reg, dim3(grid_for(ndof_, 256)), dim3(256), 0, s_, nn_, nfix_.p, diagpos_.p, maxdiag_.p,
+                       val_.p);
+  vec_->dot(n_energy_.p, nullptr, nn_, S_E_NODE);
+  vec_->dot(e_energy_.p, nullptr, ne_, S_E_ELEM);
+  vec_->dot(n_power_.p, nullptr, nn_, S_P_NODE);
+  vec_->dot(e_power_.p, nullptr, ne_, S_P_ELEM);
+  double sc[4];
+  vec_->read(S_E_NODE, 4, sc);
+  if (power) *power = sc[2] + sc[3];
+  return sc[0] + sc[1];
+}
+
+void StepDevice::assemble_host(int matrix, int newton, const double* v, double* energy, double* power, double* r,
+                               double* val) {
+  if (!prepared_) throw Error(-5, "PrepareStep must be called first");
+  if (v) IFEA_HIP(hipMemcpyAsync(v_.p, v, ndof_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  double pw = 0;
+  double e = assemble(matrix != 0, newton != 0, v_.p, 0, &pw);
+  if (energy) *energy = e;
+  if (power) *power = pw;
+  if (r) IFEA_HIP(hipMemcpyAsync(r, r_.p, ndof_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (val && matrix) IFEA_HIP(hipMemcpyAsync(val, val_.p, nnz_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+int StepDevice::refactor(int precision, ifea_step_stats* st) {
+  if (need_analyze_) {
+    SymbolicOptions opt;
+    if (!h_coords_.empty()) opt.coords = h_coords_.data();
+    spd_->analyze(ndof_, h_colPtr_.data(), h_rowIdx_.data(), user_perm_.empty() ? nullptr : user_perm_.data(), opt);
+    need_analyze_ = false;
+  }
+  int rc = spd_->factorize_device(val_.p, precision);
+  if (st) {
+    st->factorizations++;
+    st->ms_factor += spd_->last_factor_ms();
+  }
+  return rc;
+}
+
+void StepDevice::lin_solve(const double* b, double* x, ifea_step_stats* st) {
+  spd_->solve_device(b, x);
+  if (st) st->solves++;
+}
+
+// Pcg (:1051-1087): preconditioner = the cached factor; returns iterations or -1.
+int StepDevice::pcg(const double* b, double* x, double tol, int maxIt, ifea_step_stats* st) {
+  if (st) st->pcg_calls++;
+  vec_->dot(b, b, ndof_, S_DOT0);
+  double bn2;
+  vec_->read(S_DOT0, 1, &bn2);
+  double bnorm = std::sqrt(bn2);
+  if (bnorm == 0) {
+    IFEA_HIP(hipMemsetAsync(x, 0, ndof_ * sizeof(double), s_));
+    return 0;
+  }
+  lin_solve(b, x, st);
+  spmv_sym(s_, ndof_, colPtr_.p, rowIdx_.p, val_.p, x, pq_.p);
+  IFEA_HIP(hipMemcpyAsync(pr_.p, b, ndof_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+  axpby(s_, ndof_, -1.0, pq_.p, 1.0, pr_.p);
+  lin_solve(pr_.p, pz_.p, st);
+  IFEA_HIP(hipMemcpyAsync(pp_.p, pz_.p, ndof_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+  vec_->dot(pr_.p, pz_.p, ndof_, S_RZ);
+  for (int it = 0; it < maxIt; it++) {
+    vec_->dot(pr_.p, pr_.p, ndof_, S_DOT1);
+    double rn2;
+    vec_->read(S_DOT1, 1, &rn2);
+    if (!std::isfinite(rn2)) return -2;
+    if (std::sqrt(rn2) <= tol * bnorm) return it;
+    if (st) st->pcg_iterations++;
+    spmv_sym(s_, ndof_, colPtr_.p, rowIdx_.p, val_.p, pp_.p, pq_.p);
+    vec_->dot(pp_.p, pq_.p, ndof_, S_PQ);
+    pcg_update_x_r(s_, ndof_, vec_->slots(), pp_.p, pq_.p, x, pr_.p);
+    lin_solve(pr_.p, pz_.p, st);
+    vec_->dot(pr_.p, pz_.p, ndof_, S_RZN);
+    pcg_update_p(s_, ndof_, vec_->slots(), pz_.p, pp_.p);
+  }
+  vec_->dot(pr_.p, pr_.p, ndof_, S_DOT1);
+  double rn2;
+  vec_->read(S_DOT1, 1, &rn2);
+  return std::sqrt(rn2) <= tol * bnorm ? maxIt : -1;
+}
+
+// SolveLinear (:965-1019) with the mixed-precision refinement slot
+int StepDevice::solve_linear(double* b, double* x, ifea_step_stats* st) {
+  if (have_factor_ && pcg_skip_ > 0) {
+    pcg_skip_--;
+  } else if (have_factor_) {
+    int its = pcg(b, x, P_.pcg_tol, P_.pcg_budget, st);
+    if (its >= 0) { pcg_failures_ = 0; return 0; }
+    if (++pcg_failures_ >= 2) pcg_skip_ = std::min(64, 1 << (pcg_failures_ - 1));
+  }
+  int prec = P_.precision == 1 ? 1 : 0;
+  int rc = refactor(prec, st);
+  if (rc) { have_factor_ = false; return rc; }
+  have_factor_ = true;
+  if (prec == 1) {
+    // FP32 factor: refine in FP64 (PCG against the FP64 matrix); fall back to an FP64 factor when it stalls
+    int its = pcg(b, x, P_.pcg_tol, std::max(P_.pcg_budget, 30), st);
+    if (its >= 0) return 0;
+    if (st) st->fp64_fallbacks++;
+    rc = refactor(0, st);
+    if (rc) { have_factor_ = false; return rc; }
+  }
+  lin_solve(b, x, st);
+  return 0;
+}
+
+// SolveVelocity (:875-948)
+void StepDevice::solve_velocity(int warm, ifea_step_stats* st) {
+  if (!prepared_) throw Error(-5, "PrepareStep must be called first");
+  ifea_step_stats local{};
+  ifea_step_stats* S = st ? st : &local;
+  std::memset(S, 0, sizeof(*S));
+  auto t0 = std::chrono::steady_clock::now();
+  int halved = 0;
+  double last_rel = INFINITY;
+  int it;
+  const int nlim = P_.newton_limit > 0 ? P_.newton_limit : 20;
+  for (it = 1; it <= P_.max_iterations; it++) {
+    bool newton = (warm || it > 2) && it <= nlim && !(P_.adaptive_newton && halved >= 2);
+    if (newton) S->newton_iterations++;
+    double pw;
+    double e0 = assemble(true, newton, v_.p, 1, &pw);
+    S->assemblies++;
+    // r now holds -r with fixed rows zeroed
+    vec_->dot(r_.p, r_.p, ndof_, S_DOT2);
+    int rc = solve_linear(r_.p, dv_.p, S);
+    if (rc) throw Error(rc == -4 ? -4 : -6, "Sparse factorisation failed (matrix not positive definite)");
+    vec_->dot(r_.p, dv_.p, ndof_, S_DOT3);
+    double sl[2];
+    vec_->read(S_DOT2, 2, sl);
+    S->last_rnorm = std::sqrt(sl[0]);
+    double slope = -sl[1];
+    double beta = 1;
+    for (int ls = 0; ls < 8; ls++) {
+      hipLaunchKernelGGL(k_trial, dim3(grid_for(ndof_, 256)), dim3(256), 0, s_, (long long)ndof_, v_.p, beta, dv_.p,
+                         vtrial_.p);
+      double p1;
+      double e1 = assemble(false, false, vtrial_.p, 2, &p1);
+      S->assemblies++;
+      if (e1 <= e0 + 1e-4 * beta * slope + 1e-11 * std::fabs(e0)) break;
+      double curv = (e1 - e0 - slope * beta) / (beta * beta);
+      beta = P_.quadratic_ls && curv > 0 ? std::min(0.7 * beta, std::max(0.1 * beta, -slope / (2 * curv))) : 0.5 * beta;
+    }
+    IFEA_HIP(hipMemcpyAsync(v_.p, vtrial_.p, ndof_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    halved = newton && beta < 1 ? halved + 1 : 0;
+    vec_->dot(dv_.p, dv_.p, ndof_, S_DOT0);
+    vec_->dot(v_.p, v_.p, ndof_, S_DOT1);
+    double nn2[2];
+    vec_->read(S_DOT0, 2, nn2);
+    double rel = beta * std::sqrt(nn2[0]) / std::max(std::sqrt(nn2[1]), 1e-30);
+    last_rel = rel;
+    if (std::isnan(rel)) throw Error(-6, "Velocity solution became NaN");
+    if (rel < P_.tolerance) { S->converged = 1; break; }
+  }
+  S->iterations = std::min(it, P_.max_iterations);
+  S->last_rel = last_rel;
+  S->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void StepDevice::final_eval(const int* die_axis, double* power, double* reaction, double* vg, double* r,
+                            unsigned char* release) {
+  double pw = 0;
+  assemble(false, false, v_.p, 0, &pw);
+  if (power) *power = pw;
+  DBuf<int> da;
+  std::vector<int> zero(nn_, -1);
+  da.upload(die_axis ? die_axis : zero.data(), nn_, s_);
+  DBuf<double> vgd, rz, comp;
+  vgd.alloc(ndof_);
+  rz.alloc(nn_);
+  comp.alloc(nn_);
+  IFEA_HIP(hipMemsetAsync(maxdiag_.p, 0, 2 * sizeof(unsigned long long), s_));
+  hipLaunchKernelGGL(k_final_nodes, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, ax_.p, v_.p, r_.p, contact_.p,
+                     da.p, vgd.p, rz.p, comp.p, maxdiag_.p);
+  vec_->dot(rz.p, nullptr, nn_, S_REACTION);
+  double rc;
+  vec_->read(S_REACTION, 1, &rc);
+  if (reaction) *reaction = rc;
+  if (vg) IFEA_HIP(hipMemcpyAsync(vg, vgd.p, ndof_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (r) IFEA_HIP(hipMemcpyAsync(r, r_.p, ndof_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (release) {
+    // tensile release (:1421-1439), small per-node pass on the host
+    std::vector<double> cp(nn_);
+    std::vector<int> ct(nn_);
+    unsigned long long mx[2];
+    IFEA_HIP(hipMemcpyAsync(cp.data(), comp.p, nn_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+    IFEA_HIP(hipMemcpyAsync(ct.data(), contact_.p, nn_ * sizeof(int), hipMemcpyDeviceToHost, s_));
+    IFEA_HIP(hipMemcpyAsync(mx, maxdiag_.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s_));
+    IFEA_HIP(hipStreamSynchronize(s_));
+    double maxT, maxB;
+    std::memcpy(&maxT, &mx[0], 8);
+    std::memcpy(&maxB, &mx[1], 8);
+    int keptB = 0, keptT = 0;
+    for (int n = 0; n < nn_; n++) {
+      release[n] = 0;
+      if (ct[n] == 0 || !die_axis || die_axis[n] < 0) continue;
+      double thr = 0.02 * (ct[n] == 2 ? maxT : maxB);
+      if (cp[n] < -thr) release[n] = 1;
+      else if (ct[n] == 2) keptT++;
+      else keptB++;
+    }
+    if (keptB == 0 || keptT == 0)
+      for (int n = 0; n < nn_; n++)
+        if ((keptT == 0 && ct[n] == 2) || (keptB == 0 && ct[n] == 1)) release[n] = 0;
+  }
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+void StepDevice::advance(double dt, double* eps, double* sigma, double* rate) {
+  hipLaunchKernelGGL(k_advance, dim3(grid_for(ne_, 256)), dim3(256), 0, s_, ne_, dt, el_rate_.p, eps_.p);
+  if (eps) IFEA_HIP(hipMemcpyAsync(eps, eps_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (sigma) IFEA_HIP(hipMemcpyAsync(sigma, el_sig_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (rate) IFEA_HIP(hipMemcpyAsync(rate, el_rate_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+void StepDevice::get_v(double* v) {
+  IFEA_HIP(hipMemcpyAsync(v, v_.p, ndof_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+void StepDevice::set_v(const double* v) {
+  IFEA_HIP(hipMemcpyAsync(v_.p, v, ndof_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+void StepDevice::info(double* out) {
+  for (int k = 0; k < 8; k++) out[k] = 0;
+  if (spd_->analyzed()) {
+    const Symbolic& S = spd_->symbolic();
+    out[0] = (double)S.nnzL;
+    out[1] = S.flops;
+    out[2] = S.ns;
+    out[3] = (double)spd_->device_bytes();
+    out[4] = spd_->last_factor_ms();
+    out[5] = (double)S.Lsize;
+    out[6] = (double)S.Usize;
+    out[7] = S.max_level + 1;
+  }
+}
+
+}  // namespace ifea

@@ -1,0 +1,419 @@
+"""Host driver of the MI355X solve step: the increment loop of the reference's Forging3DSolver on a fixed mesh.
+
+Mirrors src/ImpressionFEA.Core/ThreeD/Forging3DSolver.cs:
+  constructor constants and initial contact/velocity   :125-187
+  Step (one stroke increment, no remeshing)             :540-638
+  PrepareStep host part: nodal frames and fixed DOFs    :660-727   (vectorised numpy; die geometry is host side)
+  LayerLimitedIncrement                                 :236-249
+  TryUpdate (contact and placement)                     :1455-1537 (without the placement repair passes)
+Everything between PrepareStep's frames and the strain update runs on the GPU through include/ifea_step.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .config import StepConfig, Workload
+from .errors import MeshDistortionError, SolverDivergenceError, raise_for
+from .mesh import TetMesh, build_pattern, kuhn_cylinder
+
+FREE, BOTTOM, TOP = 0, 1, 2
+
+
+def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
+    """PrepareStep's nodal frames (Forging3DSolver.cs:669-727), vectorised over nodes.
+
+    Returns ax [nn,3,3] (rows ax0, ax1, ax2), nfix [nn], dieAxis [nn], fixedVal [3nn].
+    """
+    nn = x.shape[0]
+    dirs = np.zeros((nn, 3, 3))
+    vals = np.zeros((nn, 3))
+    k = np.zeros(nn, dtype=np.int64)
+    die_idx = np.full(nn, -1, dtype=np.int64)
+    ar = np.arange(nn)
+    m = onX0.astype(bool)
+    dirs[ar[m], k[m], 0] = 1.0
+    k[m] += 1
+    m = onY0.astype(bool)
+    dirs[ar[m], k[m], 1] = 1.0
+    k[m] += 1
+    for side, die, off in ((BOTTOM, cfg.bottom, 0.0), (TOP, cfg.top, gap)):
+        m = contact == side
+        if not m.any():
+            continue
+        _, nrm = die.project(x[m], off)
+        idx = ar[m]
+        dirs[idx, k[m]] = nrm
+        dvel = np.array([0.0, 0.0, -cfg.ram]) if side == TOP else np.zeros(3)
+        vals[idx, k[m]] = nrm @ dvel
+        die_idx[idx] = k[m]
+        k[m] += 1
+    e = np.zeros((nn, 3, 3))
+    ev = np.zeros((nn, 3))
+    mc = np.zeros(nn, dtype=np.int64)
+    die_axis = np.full(nn, -1, dtype=np.int64)
+    for q in range(3):
+        active = q < k
+        d = dirs[:, q].copy()
+        v = vals[:, q].copy()
+        for s in range(3):
+            sm = active & (s < mc)
+            proj = np.einsum("ij,ij->i", d, e[:, s])
+            d[sm] -= proj[sm, None] * e[sm, s]
+            v[sm] -= proj[sm] * ev[sm, s]
+        ln = np.sqrt(np.einsum("ij,ij->i", d, d))
+        keep = active & (ln >= 0.2)
+        ki = ar[keep]
+        e[ki, mc[keep]] = (1.0 / ln[keep])[:, None] * d[keep]
+        ev[ki, mc[keep]] = v[keep] / ln[keep]
+        hit = keep & (die_idx == q)
+        die_axis[hit] = mc[hit]
+        mc[keep] += 1
+    # completion to an orthonormal frame
+    m0 = mc == 0
+    e[m0] = np.eye(3)
+    m1 = mc == 1
+    if m1.any():
+        e0 = e[m1, 0]
+        helper = np.where((np.abs(e0[:, 0]) < 0.8)[:, None], np.array([1.0, 0, 0]), np.array([0, 1.0, 0]))
+        e1 = np.cross(e0, helper)
+        e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+        e[m1, 1] = e1
+        e[m1, 2] = np.cross(e0, e1)
+    m2 = mc == 2
+    if m2.any():
+        e2 = np.cross(e[m2, 0], e[m2, 1])
+        e[m2, 2] = e2 / np.linalg.norm(e2, axis=1, keepdims=True)
+    fixed_val = np.zeros(3 * nn)
+    fv = fixed_val.reshape(nn, 3)
+    for q in range(3):
+        mq = q < mc
+        fv[mq, q] = ev[mq, q]
+    return e, mc.astype(np.int32), die_axis.astype(np.int32), fixed_val
+
+
+def tet_quality(x, tets):
+    """TetMesh.Quality (TetMesh.cs:107-112), vectorised."""
+    a, b, c, d = (x[tets[:, i]] for i in range(4))
+    l2 = (np.sum((b - a) ** 2, 1) + np.sum((c - a) ** 2, 1) + np.sum((d - a) ** 2, 1) + np.sum((c - b) ** 2, 1)
+          + np.sum((d - b) ** 2, 1) + np.sum((d - c) ** 2, 1)) / 6.0
+    vol = np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a)) / 6.0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(l2 > 0, 6 * np.sqrt(2.0) * vol / l2 ** 1.5, -1.0)
+
+
+@dataclass
+class IncrementResult:
+    step: int
+    load: float
+    reaction: float
+    iterations: int
+    converged: bool
+    dt: float
+    gap: float
+    stats: dict
+    ms_solve: float      # PrepareStep + SolveVelocity + FinalEvaluation + strain update (the unit of work)
+    ms_update: float     # TryUpdate (host geometry), timed separately
+
+
+class Forging3DSolver:
+    """One billet on a fixed mesh, stepped by increments on the GPU (Forging3DSolver.Step without remeshing)."""
+
+    def __init__(self, cfg: StepConfig, x, tets, onX0, onY0, gap0: float, precision: int = 0, perm=None,
+                 warm_start=None):
+        self.cfg = cfg
+        self.x = np.array(x, dtype=np.float64)
+        self.mesh = TetMesh(self.x.shape[0], tets, np.asarray(onX0, bool), np.asarray(onY0, bool))
+        m = self.mesh
+        self.nn, self.ne, self.ndof = m.nn, m.ne, 3 * m.nn
+        self.nbr_start, self.nbr, self.colPtr, self.rowIdx = build_pattern(m)
+        self.nnz = int(self.colPtr[-1])
+        consts = cfg.constants()
+        self.rate_ref, self.rate_cut, self.sig_ref = consts["rateRef"], consts["rateCut"], consts["sigRef"]
+        self.k_pen, self.u0 = consts["kPen"], consts["u0"]
+        self.gap = self.gap0 = float(gap0)
+        self.ds = cfg.stroke_increment
+        self.warm = False
+        self.step_index = 0
+        self.time = 0.0
+        self.warm_start = warm_start
+        self.vg_prev = None
+        # LocalLengths (:442-466)
+        t = m.tets
+        sums = np.zeros(self.nn)
+        cnt = np.zeros(self.nn)
+        for a in range(4):
+            for b in range(a + 1, 4):
+                ln = np.linalg.norm(self.x[t[:, a]] - self.x[t[:, b]], axis=1)
+                np.add.at(sums, t[:, a], ln)
+                np.add.at(sums, t[:, b], ln)
+                np.add.at(cnt, t[:, a], 1)
+                np.add.at(cnt, t[:, b], 1)
+        self.h_local = np.where(cnt > 0, np.minimum(sums / np.maximum(cnt, 1), cfg.h), cfg.h)
+        # device step
+        L = N.lib()
+        faces = m.boundary_faces[~m.face_is_symmetry]
+        owners = m.boundary_face_owner[~m.face_is_symmetry]
+        self._keep = [N.i32(m.tets), N.i32(self.colPtr), N.i32(self.rowIdx), N.i32(faces), N.i32(owners)]
+        h = C.c_void_p()
+        rc = L.ifea_step_create(C.byref(h), self.nn, self.ne, N.ptr(self._keep[0], C.c_int), N.ptr(self._keep[1], C.c_int),
+                                N.ptr(self._keep[2], C.c_int), int(faces.shape[0]), N.ptr(self._keep[3], C.c_int),
+                                N.ptr(self._keep[4], C.c_int))
+        self._check(rc, "ifea_step_create")
+        self._h = h
+        self.precision = precision
+        self.set_params()
+        if perm is not None:
+            pm = N.i32(perm)
+            self._check(L.ifea_step_set_perm(self._h, N.ptr(pm, C.c_int)), "set_perm")
+        # state
+        self.contact = np.zeros(self.nn, dtype=np.int32)
+        self.eps = np.zeros(self.ne)
+        self.el_sig = np.full(self.ne, self.sig_ref)
+        self.el_rate = np.zeros(self.ne)
+        self.vg = np.zeros((self.nn, 3))
+        self.release = np.zeros(self.nn, dtype=bool)
+        self._state_dirty = True
+
+    @staticmethod
+    def _check(rc, what):
+        if rc:
+            raise_for(rc, what, N.last_error())
+
+    def set_params(self, precision: int | None = None):
+        if precision is not None:
+            self.precision = precision
+        c = self.cfg
+        p = N.StepParams()
+        p.C, p.n, p.m, p.eps0 = c.C, c.n, c.m, c.eps0
+        p.friction, p.ram, p.kpen, p.rate_cut, p.u0 = c.friction, c.ram, self.k_pen, self.rate_cut, self.u0
+        p.newton_blend, p.tolerance, p.pcg_tol = c.newton_blend, c.tolerance, 1e-7
+        p.max_iterations, p.pcg_budget = c.max_iterations, c.pcg_budget
+        p.quadratic_ls, p.adaptive_newton = int(c.quadratic_ls), int(c.adaptive_newton)
+        p.precision, p.newton_limit = int(self.precision), 20
+        self._check(N.lib().ifea_step_set_params(self._h, C.byref(p)), "set_params")
+
+    @classmethod
+    def from_workload(cls, wl: Workload, precision: int = 0, **kw):
+        x, tets, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=wl.seed, quarter=wl.quarter)
+        s = cls(wl.cfg, x, tets, ox, oy, wl.gap0, precision=precision, **kw)
+        s.initial(zmin=0.0)
+        return s
+
+    # ------------------------------------------------------------------ constructor tail (:158-182)
+    def initial(self, zmin: float, cx: float = 0.0, cy: float = 0.0):
+        cfg, m = self.cfg, self.mesh
+        tol = 0.05 * cfg.h
+        surf = m.is_surface_node
+        cb = cfg.bottom.clearance(self.x, 0.0)
+        ct = cfg.top.clearance(self.x, self.gap)
+        self.contact[:] = FREE
+        bot = surf & (cb < tol)
+        self.contact[bot] = BOTTOM
+        topm = surf & ~bot & (ct < tol)
+        self.contact[topm] = TOP
+        for side, clr in ((BOTTOM, cb), (TOP, ct)):
+            if not (self.contact == side).any():
+                cand = np.where(surf, clr, np.inf)
+                self.contact[int(np.argmin(cand))] = side
+        self._place_contact_nodes()
+        r = self.rate_ref
+        p = self.x
+        self.vg = np.stack([0.5 * r * (p[:, 0] - cx), 0.5 * r * (p[:, 1] - cy), -r * (p[:, 2] - zmin)], axis=1)
+
+    def _sym(self, p):
+        p = p.copy()
+        p[self.mesh.onX0, 0] = 0.0
+        p[self.mesh.onY0, 1] = 0.0
+        return p
+
+    def _place_contact_nodes(self):
+        for side, die, off in ((BOTTOM, self.cfg.bottom, 0.0), (TOP, self.cfg.top, self.gap)):
+            m = self.contact == side
+            if m.any():
+                q, _ = die.project(self.x[m], off)
+                self.x[m] = q
+        self.x = self._sym(self.x)
+
+    # ------------------------------------------------------------------ increment (:540-638)
+    def layer_limited_increment(self):
+        m = self.contact == TOP
+        if not m.any():
+            return np.inf
+        xs = self.x[m]
+        zb = self.cfg.bottom.height(xs[:, 0], xs[:, 1])
+        zt = self.cfg.top.height(xs[:, 0], xs[:, 1]) + self.gap
+        ok = np.isfinite(zb) & np.isfinite(zt) & (zt > zb)
+        if not ok.any():
+            return np.inf
+        return max(0.1 * float(np.min(zt[ok] - zb[ok])), self.cfg.stroke_increment / 32)
+
+    def prepare(self, vguess=None):
+        """PrepareStep: host frames, then the device part (geometry, ANP, friction, warm start)."""
+        ax, nfix, die_axis, fixed_val = compute_frames(self.x, self.mesh.onX0, self.mesh.onY0, self.contact, self.gap,
+                                                       self.cfg)
+        self.ax, self.nfix, self.die_axis, self.fixed_val = ax, nfix, die_axis, fixed_val
+        if vguess is None:
+            trend = self.cfg.extrapolate_start and self.warm and self.vg_prev is not None
+            vguess = 2 * self.vg - self.vg_prev if trend else self.vg
+        L = N.lib()
+        if self._state_dirty:
+            e, s = N.f64(self.eps), N.f64(self.el_sig)
+            self._check(L.ifea_step_set_element_state(self._h, N.ptr(e, C.c_double), N.ptr(s, C.c_double)), "state")
+            self._state_dirty = False
+        xs, axs, nf, fv, ct, vgs = N.f64(self.x), N.f64(ax), N.i32(nfix), N.f64(fixed_val), N.i32(self.contact), N.f64(vguess)
+        rc = L.ifea_step_prepare(self._h, N.ptr(xs, C.c_double), N.ptr(axs, C.c_double), N.ptr(nf, C.c_int),
+                                 N.ptr(fv, C.c_double), N.ptr(ct, C.c_int), N.ptr(vgs, C.c_double))
+        self._check(rc, "PrepareStep")
+
+    def solve_velocity(self):
+        st = N.StepStats()
+        self._check(N.lib().ifea_step_solve_velocity(self._h, int(self.warm), C.byref(st)), "SolveVelocity")
+        return st.as_dict()
+
+    def final_evaluation(self):
+        pw, rc = C.c_double(0), C.c_double(0)
+        vg = np.zeros((self.nn, 3))
+        rel = np.zeros(self.nn, np.uint8)
+        da = N.i32(self.die_axis)
+        self._check(N.lib().ifea_step_final(self._h, N.ptr(da, C.c_int), C.byref(pw), C.byref(rc),
+                                            N.ptr(vg, C.c_double), None, N.ptr(rel, C.c_ubyte)), "FinalEvaluation")
+        self.vg_prev = self.vg
+        self.vg = vg
+        self.release = rel.astype(bool)
+        return pw.value, rc.value
+
+    def try_update(self, dt: float) -> str:
+        """TryUpdate (:1455-1537) without ImprovePlacement/SmoothInterior; '' on success, else the reason."""
+        cfg = self.cfg
+        gapT = max(0.0, self.gap - cfg.ram * dt)
+        if gapT < 1e-9:
+            gapT = 0.0
+        p = self._sym(self.x + dt * self.vg)
+        cT = np.zeros(self.nn, np.int32)
+        # nodes in contact (and not released) stay on their die unless they left it
+        for side, die in ((BOTTOM, cfg.bottom), (TOP, cfg.top)):
+            off = gapT if side == TOP else 0.0
+            m = (self.contact == side) & ~self.release
+            if m.any():
+                clr = die.clearance(p[m], off)
+                stay = clr <= 0.05 * self.h_local[m]
+                idx = np.where(m)[0]
+                q, _ = die.project(p[idx[stay]], off)
+                p[idx[stay]] = q
+                cT[idx[stay]] = side
+        free = ~(((self.contact != FREE) & ~self.release)) & self.mesh.is_surface_node
+        cb = cfg.bottom.clearance(p, 0.0)
+        ctp = cfg.top.clearance(p, gapT)
+        hit = free & ((cb < 0) | (ctp < 0))
+        max_pen = 0.0
+        if hit.any():
+            idx = np.where(hit)[0]
+            to_bottom = cb[idx] <= ctp[idx]
+            for side, die, off, sel in ((BOTTOM, cfg.bottom, 0.0, to_bottom), (TOP, cfg.top, gapT, ~to_bottom)):
+                ii = idx[sel]
+                if ii.size == 0:
+                    continue
+                q, _ = die.project(p[ii], off)
+                pen = np.linalg.norm(q - p[ii], axis=1) / self.h_local[ii]
+                max_pen = max(max_pen, float(pen.max()))
+                p[ii] = q
+                cT[ii] = side
+        p = self._sym(p)
+        if max_pen > 0.5:
+            return "die penetration exceeds half the local element size"
+        q = tet_quality(p, self.mesh.tets)
+        if (q <= 0.005).any():
+            return f"tetrahedron {int(np.argmax(q <= 0.005))} collapsed"
+        self.x = p
+        self.contact = cT
+        self.gap = gapT
+        self.time += dt
+        return ""
+
+    def step(self) -> IncrementResult:
+        """One stroke increment (Forging3DSolver.Step :540-638) on the fixed mesh."""
+        cfg = self.cfg
+        self.step_index += 1
+        t0 = time.perf_counter()
+        vguess = self.warm_start(self) if (self.warm_start is not None and self.warm) else None
+        self.prepare(vguess)
+        st = self.solve_velocity()
+        if not st["converged"] and st["last_rel"] >= 1e-2:
+            raise SolverDivergenceError(f"Velocity solution did not converge at step {self.step_index}: "
+                                        f"|dv|/|v| = {st['last_rel']:.2E}.")
+        power, reaction = self.final_evaluation()
+        load = power / cfg.ram * cfg.sym_factor
+        reaction *= cfg.sym_factor
+        t1 = time.perf_counter()
+        ds = min(min(self.ds, cfg.stroke_increment), self.layer_limited_increment())
+        dt = min(ds, self.gap) / cfg.ram
+        if self.gap - cfg.ram * dt < 0.05 * ds:
+            dt = self.gap / cfg.ram
+        why = ""
+        for attempt in range(6):
+            dta = dt / (1 << attempt)
+            why = self.try_update(dta)
+            if not why:
+                dt = dta
+                break
+        if why:
+            raise MeshDistortionError(f"Mesh distortion at step {self.step_index}: {why}.")
+        t2 = time.perf_counter()
+        eps = np.zeros(self.ne)
+        sig = np.zeros(self.ne)
+        rate = np.zeros(self.ne)
+        self._check(N.lib().ifea_step_advance(self._h, dt, N.ptr(eps, C.c_double), N.ptr(sig, C.c_double),
+                                              N.ptr(rate, C.c_double)), "advance")
+        self.eps, self.el_sig, self.el_rate = eps, sig, rate
+        self.warm = True
+        t3 = time.perf_counter()
+        return IncrementResult(self.step_index, load, reaction, st["iterations"], bool(st["converged"]), dt, self.gap,
+                               st, 1e3 * ((t1 - t0) + (t3 - t2)), 1e3 * (t2 - t1))
+
+    # ------------------------------------------------------------------ parity helpers
+    def assemble(self, v, matrix=True, newton=True):
+        e, pw = C.c_double(0), C.c_double(0)
+        r = np.zeros(self.ndof)
+        val = np.zeros(self.nnz) if matrix else None
+        vv = N.f64(v)
+        self._check(N.lib().ifea_step_assemble(self._h, int(matrix), int(newton), N.ptr(vv, C.c_double), C.byref(e),
+                                               C.byref(pw), N.ptr(r, C.c_double), N.ptr(val, C.c_double)), "Assemble")
+        return e.value, pw.value, r, val
+
+    def get_v(self):
+        v = np.zeros(self.ndof)
+        self._check(N.lib().ifea_step_get_v(self._h, N.ptr(v, C.c_double)), "get_v")
+        return v
+
+    def set_v(self, v):
+        vv = N.f64(v)
+        self._check(N.lib().ifea_step_set_v(self._h, N.ptr(vv, C.c_double)), "set_v")
+
+    def set_element_state(self, eps=None, sigma=None):
+        if eps is not None:
+            self.eps = np.array(eps, dtype=np.float64)
+        if sigma is not None:
+            self.el_sig = np.array(sigma, dtype=np.float64)
+        self._state_dirty = True
+
+    def info(self):
+        out = np.zeros(8)
+        self._check(N.lib().ifea_step_info(self._h, N.ptr(out, C.c_double)), "info")
+        return dict(nnz_l=int(out[0]), flops=out[1], supernodes=int(out[2]), device_bytes=int(out[3]),
+                    factor_ms=out[4], l_words=int(out[5]), u_words=int(out[6]), levels=int(out[7]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().ifea_step_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

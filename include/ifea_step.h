@@ -1,0 +1,100 @@
+/* ifea_step.h — device-resident solve step of the 3D rigid-viscoplastic forging solver on MI355X.
+ *
+ * The reference keeps assembly private to Forging3DSolver (src/ImpressionFEA.Core/ThreeD/Forging3DSolver.cs:1115),
+ * so this is a new seam with no counterpart in the reference: the host (C#, or the Python driver in this package)
+ * keeps die geometry, nodal frames and contact (PrepareStep's host part, Forging3DSolver.cs:669-727) and the geometry
+ * update (TryUpdate, :1455-1537); the library owns everything else of one increment on the GPU:
+ *   element geometry and the ANP penalty matrix        (PrepareStep :730-753, BuildAnp :761-823)
+ *   friction areas / shear stresses                    (BuildFriction :829-850)
+ *   the warm start                                     (LoadVelocityGuess :854-869)
+ *   assembly of residual and tangent                   (Assemble/ElementWork/Kernel :1115-1400)
+ *   the Picard->Newton loop with line search, PCG with the cached factor and the refactorisation policy
+ *                                                      (SolveVelocity :875-948, SolveLinear/Pcg :965-1087)
+ *   the final residual, ram power and reaction         (FinalEvaluation :1402-1451) and the strain update (:626)
+ * Arrays are caller-owned and read during the call only. Return codes as in ifea_spd.h.
+ */
+#ifndef IFEA_STEP_H
+#define IFEA_STEP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ifea_step* ifea_step_t;
+
+typedef struct ifea_step_params {
+  double C, n, m, eps0;      /* flow law sigma = C (eps0 + strain)^n rate^m (FlowStressLaw.cs:8-9) */
+  double friction;           /* shear friction factor (SimulationConfig3D.FrictionFactor) */
+  double ram;                /* ram speed, mm/s */
+  double kpen;               /* absolute ANP penalty = PenaltyFactor * sigma_ref / rate_ref (Forging3DSolver.cs:149) */
+  double rate_cut;           /* strain-rate cut-off (Forging3DSolver.cs:147) */
+  double u0;                 /* friction regularisation velocity (Forging3DSolver.cs:150) */
+  double newton_blend;       /* Forging3DSolver.NewtonBlend */
+  double tolerance;          /* beta*|dv|/|v| convergence tolerance */
+  double pcg_tol;            /* PCG relative residual target (1e-7 in the reference, :989) */
+  int max_iterations;        /* SimulationConfig3D.MaxIterations */
+  int pcg_budget;            /* Forging3DSolver.PcgBudget */
+  int quadratic_ls;          /* Forging3DSolver.QuadraticLineSearch */
+  int adaptive_newton;       /* Forging3DSolver.AdaptiveNewton */
+  int precision;             /* 0: FP64 factor. 1: FP32 supernode updates (MFMA) + FP64 POTRF/residual/PCG, with an
+                                FP64 refactorisation when PCG on the FP32 factor stalls */
+  int newton_limit;          /* Newton iterations before Picard (20, Forging3DSolver.cs:953) */
+} ifea_step_params;
+
+/* Statistics of one solve_velocity call. */
+typedef struct ifea_step_stats {
+  int iterations, converged, newton_iterations;
+  int factorizations, fp64_fallbacks, pcg_calls, pcg_iterations, solves, assemblies;
+  double last_rel, last_rnorm;
+  double ms_assembly, ms_factor, ms_solve, ms_total;
+} ifea_step_stats;
+
+/* Topology and pattern, once per mesh (Allocate/BuildPattern, Forging3DSolver.cs:403-521, TetMesh.cs:9-64).
+ * tets[4*ne]; colPtr/rowIdx: the full symmetric CSC two-ring pattern (3x3 node blocks, sorted);
+ * faces[3*nf] / face_owner[nf]: non-symmetry boundary faces in the reference's order. */
+int ifea_step_create(ifea_step_t* h, int nn, int ne, const int* tets, const int* colPtr, const int* rowIdx,
+                     int nf, const int* faces, const int* face_owner);
+int ifea_step_set_params(ifea_step_t h, const ifea_step_params* p);
+
+/* Optional caller ordering for the factor (see ifea_spd_analyze_perm). */
+int ifea_step_set_perm(ifea_step_t h, const int* perm);
+
+/* Per increment (PrepareStep): positions x[3nn]; nodal frames ax[9nn] (rows ax0, ax1, ax2 per node); nfix[nn]
+ * (leading fixed local DOFs); fixed_val[3nn]; contact[nn] (0 free, 1 bottom die, 2 top die); vguess[3nn] the
+ * global-axis start velocity (the warm-start slot, LoadVelocityGuess). Returns IFEA_ERR_ARG with message
+ * "Tetrahedron e is inverted" on det <= 0 (MeshDistortionException). */
+int ifea_step_prepare(ifea_step_t h, const double* x, const double* ax, const int* nfix, const double* fixed_val,
+                      const int* contact, const double* vguess);
+
+/* Element state: accumulated strain eps[ne] and flow stress sigma[ne] (used by BuildFriction). Either may be NULL. */
+int ifea_step_set_element_state(ifea_step_t h, const double* eps, const double* sigma);
+
+/* One Assemble call at the given local-frame velocity v[3nn] (or the current one if NULL): returns Pi and power;
+ * r[3nn] / val[nnz] are copied out when not NULL (matrix=1 assembles the tangent). Used by parity tests. */
+int ifea_step_assemble(ifea_step_t h, int matrix, int newton, const double* v, double* energy, double* power,
+                       double* r, double* val);
+
+/* SolveVelocity: warm = the reference's _warm flag (Newton from the first iteration). */
+int ifea_step_solve_velocity(ifea_step_t h, int warm, ifea_step_stats* stats);
+
+/* FinalEvaluation: power (ram load = power / ram * symmetry factor), reaction, global nodal velocities vg[3nn],
+ * residual r[3nn] (local frames), release[nn] flags. Any output pointer may be NULL. */
+int ifea_step_final(ifea_step_t h, const int* die_axis, double* power, double* reaction, double* vg, double* r,
+                    unsigned char* release);
+
+/* eps += rate * dt (Forging3DSolver.cs:626) on the device; copies out eps, sigma, rate when not NULL. */
+int ifea_step_advance(ifea_step_t h, double dt, double* eps, double* sigma, double* rate);
+
+/* Current local-frame velocity v[3nn] (out) / set it (in). */
+int ifea_step_get_v(ifea_step_t h, double* v);
+int ifea_step_set_v(ifea_step_t h, const double* v);
+
+/* Device and factor statistics: [0] nnz(L), [1] factor flops, [2] supernodes, [3] device bytes, [4] last factor ms */
+int ifea_step_info(ifea_step_t h, double* out8);
+
+void ifea_step_destroy(ifea_step_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
