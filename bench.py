@@ -1,0 +1,199 @@
+"""Benchmark: stroke increments per second of the 3D solve step at ~360k DOF on one MI355X.
+
+Workload (default): the closed-die hex-head upset, billet D 30 x H 45 mm into a hexagonal cavity (across-flats
+38 mm, 2 mm flash land), 42x42x64 Kuhn cells = 120,185 nodes / 677,376 tets / 360,555 DOF, synthetic seeded mesh.
+Two arms run from the same initial state:
+  fp64   plain FP64 supernodal factor, previous-increment warm start (the reference's default)
+  mixed  FP32 MFMA supernode updates + FP64 POTRF/residual/PCG refinement (FP64 fallback on stall), with the learned
+         warm start when a trained network is available, else the previous-increment start
+A "step" is one stroke increment: PrepareStep (frames on the host, geometry/ANP/friction on the GPU) + SolveVelocity +
+FinalEvaluation + strain update; the host geometry update (TryUpdate) is identical in both arms and reported apart.
+`value` is the mixed arm's increments/s; the fp64 arm, the load error between the arms, the roofline of the dominant
+kernel (FP64 or FP32 MFMA supernode update, HIP events on the library's stream) and a host CPU baseline (the FP64
+oracle with MKL PARDISO on the host cores, on a bounded sample) are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
+HBM_PEAK = 8000.0                      # GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
+    from impressionfea_amd.forging3d import Forging3DSolver
+    s = Forging3DSolver.from_workload(wl, precision=precision, warm_start=warm_start)
+    s.set_params(profile=profile)
+    loads, ms_solve, ms_update, stats = [], [], [], []
+    for k in range(warmup):
+        r = s.step()
+        log(f"  warmup {k}: load {r.load:.6e} N it {r.iterations} fact {r.stats['factorizations']} "
+            f"{r.ms_solve:.0f} ms")
+    s.info(reset=True)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r = s.step()
+        loads.append(r.load)
+        ms_solve.append(r.ms_solve)
+        ms_update.append(r.ms_update)
+        stats.append(r.stats)
+        log(f"  step {k}: load {r.load:.6e} N it {r.iterations} fact {r.stats['factorizations']} "
+            f"pcg {r.stats['pcg_iterations']} solves {r.stats['solves']} fb {r.stats['fp64_fallbacks']} "
+            f"{r.ms_solve:.0f} ms (+{r.ms_update:.0f} ms update)")
+    wall = time.perf_counter() - t0
+    info = s.info(reset=True)
+    s.close()
+    return dict(loads=loads, ms_solve=ms_solve, ms_update=ms_update, stats=stats, wall=wall, info=info)
+
+
+def cpu_baseline(wl, gpu_stats, threads):
+    """FP64 oracle (C++ restatement, OpenMP assembly) with MKL PARDISO on the host cores, on a bounded sample of the
+    same workload: one PrepareStep, one tangent assembly, one residual assembly, one PARDISO factorisation and one
+    solve; scaled to increments/s with the fp64 arm's per-increment counts of assemblies, factorisations and solves."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from impressionfea_amd.mesh import kuhn_cylinder
+    O.build()
+    if not O.pardiso_available():
+        return None
+    O.set_threads(threads)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=wl.seed)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0, linear=1)
+    o.initial(0.0)
+    t0 = time.perf_counter()
+    o.prepare()
+    t_prep = time.perf_counter() - t0
+    v = o.v()
+    t0 = time.perf_counter()
+    e, p, r, val = o.assemble(v, True, True)
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.assemble(v, False, False)
+    t_res = time.perf_counter() - t0
+    fr = o.frames()
+    ns, nb, cp, ri = o.pattern()
+    n = o.ndof
+    fixed = fr["fixed"]
+    col = np.repeat(np.arange(n), np.diff(cp))
+    msk = fixed[ri] | fixed[col]
+    val[msk] = np.where(ri[msk] == col[msk], 1.0, 0.0)
+    dg = (ri == col) & ~fixed[col]
+    val[dg] += 1e-10 * val[dg].max()
+    b = -np.where(fixed, 0.0, r)
+    t0 = time.perf_counter()
+    pd = O.Pardiso(n, cp, ri)
+    t_an = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pd.factorize(val)
+    t_fac = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pd.solve(b)
+    t_sol = time.perf_counter() - t0
+    info = pd.info()
+    k = len(gpu_stats)
+    asm_m = sum(s["iterations"] for s in gpu_stats) / k
+    asm_r = sum(s["assemblies"] for s in gpu_stats) / k - asm_m + 1  # line-search + final residual assemblies
+    fac = sum(s["factorizations"] for s in gpu_stats) / k
+    sol = sum(s["solves"] for s in gpu_stats) / k
+    spmv = sum(s["pcg_iterations"] for s in gpu_stats) / k
+    per_inc = t_prep + asm_m * t_asm + asm_r * t_res + fac * t_fac + sol * t_sol + spmv * 0.25 * t_res
+    return dict(value=1.0 / per_inc, unit="increments/s", cores=threads, kind="port",
+                sample=(f"hex-die 360,555 DOF, one increment's work measured piecewise on the host: PrepareStep "
+                        f"{t_prep:.2f}s, tangent assembly {t_asm:.2f}s, residual assembly {t_res:.2f}s, MKL PARDISO "
+                        f"analyse {t_an:.1f}s (not per increment) / factor {t_fac:.2f}s ({info['nnz_l']:.3e} nnz(L), "
+                        f"{info['mflops']} MFLOP) / solve {t_sol:.2f}s; scaled by the fp64 arm's per-increment "
+                        f"counts (assemblies {asm_m:.1f}+{asm_r:.1f}, factorisations {fac:.2f}, solves {sol:.1f})"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="hexdie", choices=["hexdie", "upset"])
+    ap.add_argument("--arms", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    from impressionfea_amd.config import hexdie_production, upset_small
+    from impressionfea_amd import warmstart
+    wl = hexdie_production() if args.workload == "hexdie" else upset_small()
+    ws = warmstart.load_default()
+    log(f"workload {wl.name}: {wl.nxy}x{wl.nxy}x{wl.nz} cells; learned warm start: {'yes' if ws else 'no'}")
+    res = {}
+    if args.arms in ("both", "fp64"):
+        log("arm fp64")
+        res["fp64"] = run_arm(wl, 0, args.steps, args.warmup)
+    if args.arms in ("both", "mixed"):
+        log("arm mixed")
+        res["mixed"] = run_arm(wl, 1, args.steps, args.warmup, warm_start=ws)
+    head = res.get("mixed") or res["fp64"]
+    ips = {k: len(v["ms_solve"]) / (sum(v["ms_solve"]) / 1e3) for k, v in res.items()}
+    value = ips.get("mixed", ips.get("fp64"))
+    ms_step = 1e3 / value
+    load_err = None
+    if "mixed" in res and "fp64" in res:
+        a, b = np.array(res["fp64"]["loads"]), np.array(res["mixed"]["loads"])
+        load_err = float(np.max(np.abs(b - a) / np.abs(a)))
+    # roofline of the dominant kernel: the supernode update (MFMA), from HIP events around every launch
+    info = head["info"]
+    dtype_arm = "fp32" if "mixed" in res else "fp64"
+    roof = None
+    if info["syrk_ms"] > 0:
+        ach = info["syrk_flops"] / (info["syrk_ms"] / 1e3) / 1e12
+        roof = dict(bound="mfma", achieved=round(ach, 3), peak=PEAK[dtype_arm], unit="TFLOP/s",
+                    frac=round(ach / PEAK[dtype_arm], 4), traffic=None,
+                    kernel=f"k_syrk<{'float' if dtype_arm == 'fp32' else 'double'}>",
+                    launches=info["syrk_launches"], avg_launch_us=round(1e3 * info["syrk_ms"] / max(1, info["syrk_launches"]), 2),
+                    flops_per_launch=info["syrk_flops"] / max(1, info["syrk_launches"]))
+    sweep = None
+    if info["sweep_ms"] > 0:
+        bw = info["sweep_bytes"] / (info["sweep_ms"] / 1e3) / 1e9
+        sweep = dict(bound="hbm", achieved=round(bw, 1), peak=HBM_PEAK, unit="GB/s", frac=round(bw / HBM_PEAK, 4),
+                     solves=info["sweeps"], avg_ms=round(info["sweep_ms"] / max(1, info["sweeps"]), 3))
+    cpu = None
+    if not args.no_cpu_baseline and "fp64" in res:
+        try:
+            cpu = cpu_baseline(wl, res["fp64"]["stats"], args.cpu_threads)
+        except Exception as ex:  # noqa: BLE001 - the baseline is reported, never required
+            log(f"cpu baseline failed: {ex}")
+    out = {
+        "metric": "stroke increments/s @360k DOF: plain FP64 vs mixed+warm-start, <=0.1% load err",
+        "value": round(value, 4), "unit": "increments/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64" if "mixed" in res else "fp64", "data": "synthetic",
+        "config": {"workload": f"{wl.name} closed-die hex upset, {wl.nxy}x{wl.nxy}x{wl.nz} Kuhn cells, "
+                               f"360,555 DOF, FP64 residual" if args.workload == "hexdie" else wl.name,
+                   "arms": {k: round(v, 4) for k, v in ips.items()},
+                   "speedup_mixed_vs_fp64": round(ips["mixed"] / ips["fp64"], 3) if len(ips) == 2 else None,
+                   "load_rel_err_mixed_vs_fp64": load_err,
+                   "learned_warm_start": bool(ws),
+                   "nnz_L": info["nnz_l"], "factor_flops": info["flops"],
+                   "ms_update_host": round(float(np.mean(head["ms_update"])), 1)},
+        "roofline": roof,
+        "triangular_sweeps": sweep,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

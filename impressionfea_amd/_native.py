@@ -46,6 +46,7 @@ class StepParams(C.Structure):
         ("u0", C.c_double), ("newton_blend", C.c_double), ("tolerance", C.c_double), ("pcg_tol", C.c_double),
         ("max_iterations", C.c_int), ("pcg_budget", C.c_int), ("quadratic_ls", C.c_int),
         ("adaptive_newton", C.c_int), ("precision", C.c_int), ("newton_limit", C.c_int),
+        ("profile", C.c_int),
     ]
 
 
@@ -102,7 +103,7 @@ def lib():
     L.ifea_step_advance.argtypes = [vp, C.c_double, dp, dp, dp]
     L.ifea_step_get_v.argtypes = [vp, dp]
     L.ifea_step_set_v.argtypes = [vp, dp]
-    L.ifea_step_info.argtypes = [vp, dp]
+    L.ifea_step_info.argtypes = [vp, C.c_int, dp]
     L.ifea_step_destroy.argtypes = [vp]
     L.ifea_step_destroy.restype = None
     _lib = L

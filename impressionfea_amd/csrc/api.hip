@@ -134,6 +134,11 @@ int ifea_spd_factorize(ifea_spd_t h, const double* values) {
   try {
     IFEA_HIP(hipMemcpyAsync(h->vals.p, values, h->nnz * sizeof(double), hipMemcpyHostToDevice, h->stream));
     int rc = h->dev->factorize_device(h->vals.p, h->precision);
+    if (rc == IFEA_ERR_NOT_SPD && h->precision == 1) {
+      // not positive definite in FP32: factor in FP64 and keep an FP32 copy for the solves
+      h->fallbacks++;
+      rc = h->dev->factorize_device(h->vals.p, 2);
+    }
     if (rc) { set_last_error("matrix is not positive definite"); return rc; }
     return 0;
   } catch (const ifea::Error& e) {
@@ -148,7 +153,7 @@ static int spd_solve_impl(ifea_spd_t h) {
   SpdDevice& D = *h->dev;
   D.solve_device(h->b.p, h->x.p);
   h->last_refine = 0;
-  if (D.precision() == 1) {
+  if (D.precision() != 0) {
     h->vec->dot(h->b.p, h->b.p, n, S_DOT0);
     double bn2;
     h->vec->read(S_DOT0, 1, &bn2);
@@ -173,8 +178,12 @@ static int spd_solve_impl(ifea_spd_t h) {
     }
     if (!ok) {
       h->fallbacks++;
-      int rc = D.factorize_device(h->vals.p, 0);
-      if (rc) return rc;
+      if (D.precision() == 2) {
+        D.use_fp64_solves(true);
+      } else {
+        int rc = D.factorize_device(h->vals.p, 0);
+        if (rc) return rc;
+      }
       D.solve_device(h->b.p, h->x.p);
     }
   }
@@ -209,7 +218,7 @@ long long ifea_spd_factor_nnz(ifea_spd_t h) {
 int ifea_spd_set_option(ifea_spd_t h, int key, double value) {
   if (!h) { set_last_error("null handle"); return IFEA_ERR_ARG; }
   switch (key) {
-    case IFEA_OPT_PRECISION: h->precision = value != 0 ? 1 : 0; return 0;
+    case IFEA_OPT_PRECISION: h->precision = (int)value >= 0 && (int)value <= 2 ? (int)value : 0; return 0;
     case IFEA_OPT_REFINE_TOL: h->refine_tol = value; return 0;
     case IFEA_OPT_REFINE_MAXIT: h->refine_maxit = (int)value; return 0;
     case IFEA_OPT_LEAF_BLOCKS: h->opt.leaf_blocks = std::max(1, (int)value); return 0;
@@ -343,9 +352,9 @@ int ifea_step_set_v(ifea_step_t h, const double* v) {
   IFEA_GUARD(h->dev->set_v(v))
 }
 
-int ifea_step_info(ifea_step_t h, double* out8) {
-  if (!h || !out8) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
-  IFEA_GUARD(h->dev->info(out8))
+int ifea_step_info(ifea_step_t h, int reset, double* out16) {
+  if (!h || !out16) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->info(reset, out16))
 }
 
 void ifea_step_destroy(ifea_step_t h) { delete h; }

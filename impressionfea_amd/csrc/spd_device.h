@@ -40,22 +40,36 @@ class SpdDevice {
   bool analyzed() const { return analyzed_; }
   const Symbolic& symbolic() const { return S_; }
 
-  // Numeric factorization from full-CSC values already on the device. precision: 0 fp64, 1 fp32 storage.
+  // Numeric factorization from full-CSC values already on the device. precision: 0 FP64 factor; 1 FP32 factor
+  // (FP32 MFMA updates, FP64 diagonal-block POTRF and TRSM); 2 FP64 factor plus an FP32 copy of L for the solves.
   // Returns 0 or IFEA_ERR_NOT_SPD (-4).
   int factorize_device(const double* d_vals, int precision);
   bool factored() const { return factored_; }
   int precision() const { return prec_; }
+  // For precision 2: solve with the FP64 factor instead of its FP32 copy (the refinement fallback).
+  void use_fp64_solves(bool on) { solve64_ = on; }
 
   // x = A^{-1} b with the current factor; vectors in the original (unpermuted) order, on the device.
   // d_x may alias d_b.
   void solve_device(const double* d_b, double* d_x);
 
   double last_factor_ms() const { return last_factor_ms_; }
+
+  // Optional per-launch event timing of the dominant kernels (adds one event pair per launch).
+  struct Profile {
+    long long syrk_launches = 0, solve_launches = 0, factorizations = 0, solves = 0;
+    double syrk_ms = 0, syrk_flops = 0;     // algorithmic flops: sum over fronts of trailing^2 * width
+    double solve_ms = 0, solve_bytes = 0;   // one forward + backward sweep reads every panel once
+    double factor_ms = 0, factor_flops = 0;
+  };
+  void set_profiling(bool on) { profiling_ = on; }
+  const Profile& profile() const { return prof_; }
+  void reset_profile() { prof_ = Profile(); }
   size_t device_bytes() const;
 
  private:
   template <typename T> int factorize_t(const double* d_vals);
-  template <typename T> void solve_t(const double* d_b, double* d_x);
+  template <typename T> void solve_t(const T* L, const double* d_b, double* d_x);
 
   hipStream_t stream_;
   Symbolic S_;
@@ -76,14 +90,22 @@ class SpdDevice {
     LaunchList zero;
     std::vector<LaunchList> panel_steps, syrk_steps;
     long long sn_begin = 0, sn_end = 0;  // into level_sn_
+    std::vector<double> syrk_flops;      // algorithmic flops per syrk step
+    double solve_bytes = 0;              // panel bytes per element of storage (times sizeof(T) at run time)
     long long max_m = 0;
   };
   std::vector<Level> levels_;
   // numeric storage
   DBuf<unsigned char> L_, U_;
-  DBuf<double> V_, W_, y_;
+  DBuf<float> L32_;
+  bool solve64_ = false;
+  DBuf<double> V_, W_, y_, dscratch_;
   DBuf<int> err_;
   SymDev sd_{};
+  bool profiling_ = false;
+  Profile prof_;
+  std::vector<hipEvent_t> ev_pool_;
+  hipEvent_t event(size_t i);
 };
 
 }  // namespace ifea

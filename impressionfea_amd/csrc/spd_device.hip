@@ -96,10 +96,13 @@ __global__ __launch_bounds__(256) void k_extend(const int4* __restrict__ tasks, 
   }
 }
 
-// task: (supernode s, first row, end row, writes-diagonal flag). c0 = first column of the 64-wide block.
+// task: (supernode s, first row, end row, 1 + slot of the factored diagonal block in the scratch, or 0).
+// c0 = first column of the 64-wide block. Every task of a front re-factors the (unmodified) diagonal block in LDS;
+// the designated task stores the factor to the scratch, and k_commit copies it into the panel after the launch, so
+// no task can read a block that another task of the same launch has already overwritten.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
-                                               int* __restrict__ err) {
+                                               double* __restrict__ dscratch, int* __restrict__ err) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
@@ -134,8 +137,8 @@ __global__ __launch_bounds__(256) void k_panel(const int4* __restrict__ tasks, i
     }
     for (int q = i + 1; q < NB; q++) D[i][q] = 0.0;
     if (tk.w) {
-      for (int q = 0; q < w; q++)
-        if (i < w && i >= q) P[(long long)(c0 + q) * m + c0 + i] = (T)D[i][q];
+      double* dsc = dscratch + (long long)(tk.w - 1) * NB * NB;
+      for (int q = 0; q < NB; q++) dsc[q * NB + i] = D[i][q];
       if (bad && i == 0) atomicOr(err, 1);
     }
   }
@@ -151,6 +154,24 @@ __global__ __launch_bounds__(256) void k_panel(const int4* __restrict__ tasks, i
     for (int p = 0; p < q; p++) a -= x[p] * D[q][p];
     x[q] = a / D[q][q];
     if (q < w && valid) P[(long long)(c0 + q) * m + row] = (T)x[q];
+  }
+}
+
+// Copies the factored diagonal blocks of one panel launch into their panels (tasks as k_panel; non-designated exit).
+template <typename T>
+__global__ __launch_bounds__(256) void k_commit(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
+                                                const double* __restrict__ dscratch) {
+  const int4 tk = tasks[blockIdx.x];
+  if (!tk.w) return;
+  const int s = tk.x;
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int w = min(NB, k - c0);
+  T* P = L + sd.Loff[s];
+  const double* dsc = dscratch + (long long)(tk.w - 1) * NB * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+    int q = idx / NB, i = idx % NB;
+    if (i < w && q < w && i >= q) P[(long long)(c0 + q) * m + c0 + i] = (T)dsc[q * NB + i];
   }
 }
 
@@ -272,6 +293,11 @@ __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev
   }
 }
 
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
 __global__ void k_perm_in(const double* __restrict__ b, const int* __restrict__ perm, double* __restrict__ y, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) y[k] = b[perm[k]];
@@ -282,7 +308,18 @@ __global__ void k_perm_out(const double* __restrict__ y, const int* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------------ host
-SpdDevice::~SpdDevice() = default;
+SpdDevice::~SpdDevice() {
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t SpdDevice::event(size_t i) {
+  while (ev_pool_.size() <= i) {
+    hipEvent_t e;
+    IFEA_HIP(hipEventCreate(&e));
+    ev_pool_.push_back(e);
+  }
+  return ev_pool_[i];
+}
 
 size_t SpdDevice::device_bytes() const {
   return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
@@ -314,6 +351,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   for (int s = 0; s < ns; s++) bylevel[S.sn_level[s]].push_back(s);
   std::vector<int> lvl_sn;
   std::vector<int4> ext, pan, syr, zer;
+  int max_slots = 1;
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
     L.sn_begin = (long long)lvl_sn.size();
@@ -322,6 +360,8 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       L.max_m = std::max<long long>(L.max_m, S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
     }
     L.sn_end = (long long)lvl_sn.size();
+    for (int s : bylevel[l])
+      L.solve_bytes += (double)(S.sn_rowptr[s + 1] - S.sn_rowptr[s]) * (S.sn_first[s + 1] - S.sn_first[s]);
     L.zero.begin = (long long)zer.size();
     for (int s : bylevel[l]) {
       long long mu = (S.sn_rowptr[s + 1] - S.sn_rowptr[s]) - (S.sn_first[s + 1] - S.sn_first[s]);
@@ -360,6 +400,8 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       lp.begin = (long long)pan.size();
       ls.begin = (long long)syr.size();
       lp.c0 = ls.c0 = c0;
+      double fl = 0;
+      int nslot = 0;
       for (int s : bylevel[l]) {
         int k = S.sn_first[s + 1] - S.sn_first[s];
         if (k <= c0) continue;
@@ -370,8 +412,10 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
         long long chunks = std::max<long long>(1, (nrows + PANEL_ROWS - 1) / PANEL_ROWS);
         for (long long ci = 0; ci < chunks; ci++) {
           long long r0 = rb + ci * PANEL_ROWS, r1 = std::min<long long>(m, r0 + PANEL_ROWS);
-          pan.push_back(make_int4(s, (int)r0, (int)std::max(r0, r1), ci == 0 ? 1 : 0));
+          pan.push_back(make_int4(s, (int)r0, (int)std::max(r0, r1), ci == 0 ? 1 + nslot : 0));
         }
+        nslot++;
+        fl += (double)nrows * (double)(nrows + 1) * (double)w;
         long long tiles = (nrows + 63) / 64;
         for (long long I = 0; I < tiles; I++)
           for (long long J = 0; J <= I; J++) syr.push_back(make_int4(s, (int)I, (int)J, 0));
@@ -380,6 +424,8 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       ls.end = (long long)syr.size();
       L.panel_steps.push_back(lp);
       L.syrk_steps.push_back(ls);
+      L.syrk_flops.push_back(fl);
+      max_slots = std::max(max_slots, nslot);
     }
   }
   // upload
@@ -402,6 +448,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   if (syr.empty()) syrk_tasks_.alloc(1); else syrk_tasks_.upload(syr, stream_);
   if (zer.empty()) zero_tasks_.alloc(1); else zero_tasks_.upload(zer, stream_);
   level_sn_.upload(lvl_sn, stream_);
+  dscratch_.alloc((size_t)max_slots * NB * NB);
   V_.alloc(std::max<long long>(1, S.Vsize));
   W_.alloc(std::max<long long>(1, Woff[ns]));
   y_.alloc(n);
@@ -426,6 +473,7 @@ int SpdDevice::factorize_t(const double* d_vals) {
   T* U = (T*)U_.p;
   IFEA_HIP(hipMemsetAsync(L, 0, lbytes, stream_));
   IFEA_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), stream_));
+  size_t nev = 0;
   long long na = (long long)amap_src_.n;
   if (na) hipLaunchKernelGGL(k_scatter<T>, dim3(grid_for(na, 256)), dim3(256), 0, stream_, d_vals, amap_src_.p,
                              amap_dst_.p, na, L);
@@ -439,19 +487,39 @@ int SpdDevice::factorize_t(const double* d_vals) {
                          ext_tasks_.p + r.begin, sd_, L, U);
     for (size_t t = 0; t < lv.panel_steps.size(); t++) {
       const LaunchList& p = lv.panel_steps[t];
-      if (p.end > p.begin)
+      if (p.end > p.begin) {
         hipLaunchKernelGGL(k_panel<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
-                           panel_tasks_.p + p.begin, p.c0, sd_, L, err_.p);
+                           panel_tasks_.p + p.begin, p.c0, sd_, L, dscratch_.p, err_.p);
+        hipLaunchKernelGGL(k_commit<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
+                           panel_tasks_.p + p.begin, p.c0, sd_, L, dscratch_.p);
+      }
       const LaunchList& q = lv.syrk_steps[t];
-      if (q.end > q.begin)
+      if (q.end > q.begin) {
+        if (profiling_) IFEA_HIP(hipEventRecord(event(2 * nev), stream_));
         hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
                            syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+        if (profiling_) {
+          IFEA_HIP(hipEventRecord(event(2 * nev + 1), stream_));
+          nev++;
+          prof_.syrk_flops += lv.syrk_flops[t];
+        }
+      }
     }
   }
   IFEA_HIP(hipGetLastError());
   int herr = 0;
   IFEA_HIP(hipMemcpyAsync(&herr, err_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
   IFEA_HIP(hipStreamSynchronize(stream_));
+  if (profiling_) {
+    for (size_t e = 0; e < nev; e++) {
+      float ms = 0;
+      IFEA_HIP(hipEventElapsedTime(&ms, event(2 * e), event(2 * e + 1)));
+      prof_.syrk_ms += ms;
+    }
+    prof_.syrk_launches += (long long)nev;
+    prof_.factorizations++;
+    prof_.factor_flops += S.flops;
+  }
   return herr ? -4 : 0;
 }
 
@@ -462,12 +530,20 @@ int SpdDevice::factorize_device(const double* d_vals, int precision) {
   IFEA_HIP(hipEventCreate(&e1));
   IFEA_HIP(hipEventRecord(e0, stream_));
   prec_ = precision;
+  solve64_ = false;
   int rc = precision == 1 ? factorize_t<float>(d_vals) : factorize_t<double>(d_vals);
+  if (rc == 0 && precision == 2) {
+    long long n = std::max<long long>(1, S_.Lsize);
+    L32_.alloc(n);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, 256)), dim3(256), 0, stream_, (const double*)L_.p, L32_.p, n);
+    IFEA_HIP(hipGetLastError());
+  }
   IFEA_HIP(hipEventRecord(e1, stream_));
   IFEA_HIP(hipEventSynchronize(e1));
   float ms = 0;
   IFEA_HIP(hipEventElapsedTime(&ms, e0, e1));
   last_factor_ms_ = ms;
+  if (profiling_) prof_.factor_ms += ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   factored_ = rc == 0;
@@ -475,10 +551,10 @@ int SpdDevice::factorize_device(const double* d_vals, int precision) {
 }
 
 template <typename T>
-void SpdDevice::solve_t(const double* d_b, double* d_x) {
+void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
   const int n = S_.n;
-  const T* L = (const T*)L_.p;
   hipLaunchKernelGGL(k_perm_in, dim3(grid_for(n, 256)), dim3(256), 0, stream_, d_b, perm_.p, y_.p, n);
+  if (profiling_) IFEA_HIP(hipEventRecord(event(0), stream_));
   const int cap = 8192;  // doubles of LDS per forward/backward work vector
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
@@ -496,14 +572,27 @@ void SpdDevice::solve_t(const double* d_b, double* d_x) {
     hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, stream_, level_sn_.p + lv.sn_begin, sd_, L, y_.p, W_.p,
                        cap);
   }
+  if (profiling_) {
+    IFEA_HIP(hipEventRecord(event(1), stream_));
+    IFEA_HIP(hipEventSynchronize(event(1)));
+    float ms = 0;
+    IFEA_HIP(hipEventElapsedTime(&ms, event(0), event(1)));
+    prof_.solve_ms += ms;
+    prof_.solves++;
+    double b = 0;
+    for (const Level& lv : levels_) b += lv.solve_bytes;
+    prof_.solve_bytes += 2.0 * b * sizeof(T);
+    prof_.solve_launches += 2 * (long long)levels_.size();
+  }
   hipLaunchKernelGGL(k_perm_out, dim3(grid_for(n, 256)), dim3(256), 0, stream_, y_.p, perm_.p, d_x, n);
   IFEA_HIP(hipGetLastError());
 }
 
 void SpdDevice::solve_device(const double* d_b, double* d_x) {
   if (!factored_) throw Error(-5, "Factorize must be called first.");
-  if (prec_ == 1) solve_t<float>(d_b, d_x);
-  else solve_t<double>(d_b, d_x);
+  if (prec_ == 1) solve_t<float>((const float*)L_.p, d_b, d_x);
+  else if (prec_ == 2 && !solve64_) solve_t<float>(L32_.p, d_b, d_x);
+  else solve_t<double>((const double*)L_.p, d_b, d_x);
 }
 
 }  // namespace ifea

@@ -17,7 +17,7 @@ class StepDevice {
   StepDevice(int nn, int ne, const int* tets, const int* colPtr, const int* rowIdx, int nf, const int* faces,
              const int* face_owner);
   ~StepDevice();
-  void set_params(const ifea_step_params& p) { P_ = p; have_params_ = true; }
+  void set_params(const ifea_step_params& p) { P_ = p; have_params_ = true; spd_->set_profiling(p.profile != 0); }
   void set_perm(const int* perm);
   void prepare(const double* x, const double* ax, const int* nfix, const double* fixed_val, const int* contact,
                const double* vguess);
@@ -30,7 +30,7 @@ class StepDevice {
   void advance(double dt, double* eps, double* sigma, double* rate);
   void get_v(double* v);
   void set_v(const double* v);
-  void info(double* out);
+  void info(int reset, double* out);
 
  private:
   int solve_linear(double* b, double* x, ifea_step_stats* st);
@@ -51,6 +51,7 @@ class StepDevice {
   std::vector<double> h_coords_;  // 3 per DOF, for the geometric ordering
   bool need_analyze_ = true, have_factor_ = false;
   int pcg_failures_ = 0, pcg_skip_ = 0;
+  bool fp32_unstable_ = false;  // an FP32 factorisation was not positive definite: factor in FP64 from now on
   // topology (device)
   DBuf<int> tets_, nt_start_, nt_list_, colPtr_, rowIdx_, nbr_start_, nbr_, p_start_, p_node_, diagpos_;
   DBuf<int> nf_start_, nf_list_, faces_, face_owner_;

@@ -730,17 +730,28 @@ int StepDevice::solve_linear(double* b, double* x, ifea_step_stats* st) {
     if (its >= 0) { pcg_failures_ = 0; return 0; }
     if (++pcg_failures_ >= 2) pcg_skip_ = std::min(64, 1 << (pcg_failures_ - 1));
   }
-  int prec = P_.precision == 1 ? 1 : 0;
+  // precision 1: FP32 factor; after an FP32 factor that is not positive definite (penalty scale beyond FP32), FP64
+  // factors with an FP32 copy for the solves. Both are refined by PCG in FP64 against the FP64 matrix.
+  int prec = P_.precision == 1 ? (fp32_unstable_ ? 2 : 1) : (P_.precision == 2 ? 2 : 0);
   int rc = refactor(prec, st);
+  if (rc == -4 && prec == 1) {
+    fp32_unstable_ = true;
+    if (st) st->fp64_fallbacks++;
+    prec = 2;
+    rc = refactor(prec, st);
+  }
   if (rc) { have_factor_ = false; return rc; }
   have_factor_ = true;
-  if (prec == 1) {
-    // FP32 factor: refine in FP64 (PCG against the FP64 matrix); fall back to an FP64 factor when it stalls
+  if (prec != 0) {
     int its = pcg(b, x, P_.pcg_tol, std::max(P_.pcg_budget, 30), st);
     if (its >= 0) return 0;
     if (st) st->fp64_fallbacks++;
-    rc = refactor(0, st);
-    if (rc) { have_factor_ = false; return rc; }
+    if (prec == 2) {
+      spd_->use_fp64_solves(true);
+    } else {
+      rc = refactor(0, st);
+      if (rc) { have_factor_ = false; return rc; }
+    }
   }
   lin_solve(b, x, st);
   return 0;
@@ -866,8 +877,8 @@ void StepDevice::set_v(const double* v) {
   IFEA_HIP(hipStreamSynchronize(s_));
 }
 
-void StepDevice::info(double* out) {
-  for (int k = 0; k < 8; k++) out[k] = 0;
+void StepDevice::info(int reset, double* out) {
+  for (int k = 0; k < 16; k++) out[k] = 0;
   if (spd_->analyzed()) {
     const Symbolic& S = spd_->symbolic();
     out[0] = (double)S.nnzL;
@@ -879,6 +890,16 @@ void StepDevice::info(double* out) {
     out[6] = (double)S.Usize;
     out[7] = S.max_level + 1;
   }
+  const SpdDevice::Profile& p = spd_->profile();
+  out[8] = (double)p.syrk_launches;
+  out[9] = p.syrk_ms;
+  out[10] = p.syrk_flops;
+  out[11] = (double)p.solves;
+  out[12] = p.solve_ms;
+  out[13] = p.solve_bytes;
+  out[14] = (double)p.factorizations;
+  out[15] = p.factor_ms;
+  if (reset) spd_->reset_profile();
 }
 
 }  // namespace ifea

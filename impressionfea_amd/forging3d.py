@@ -184,9 +184,11 @@ class Forging3DSolver:
         if rc:
             raise_for(rc, what, N.last_error())
 
-    def set_params(self, precision: int | None = None):
+    def set_params(self, precision: int | None = None, profile: bool | None = None):
         if precision is not None:
             self.precision = precision
+        if profile is not None:
+            self.profile = bool(profile)
         c = self.cfg
         p = N.StepParams()
         p.C, p.n, p.m, p.eps0 = c.C, c.n, c.m, c.eps0
@@ -195,6 +197,7 @@ class Forging3DSolver:
         p.max_iterations, p.pcg_budget = c.max_iterations, c.pcg_budget
         p.quadratic_ls, p.adaptive_newton = int(c.quadratic_ls), int(c.adaptive_newton)
         p.precision, p.newton_limit = int(self.precision), 20
+        p.profile = int(getattr(self, "profile", False))
         self._check(N.lib().ifea_step_set_params(self._h, C.byref(p)), "set_params")
 
     @classmethod
@@ -401,11 +404,13 @@ class Forging3DSolver:
             self.el_sig = np.array(sigma, dtype=np.float64)
         self._state_dirty = True
 
-    def info(self):
-        out = np.zeros(8)
-        self._check(N.lib().ifea_step_info(self._h, N.ptr(out, C.c_double)), "info")
+    def info(self, reset: bool = False):
+        out = np.zeros(16)
+        self._check(N.lib().ifea_step_info(self._h, int(reset), N.ptr(out, C.c_double)), "info")
         return dict(nnz_l=int(out[0]), flops=out[1], supernodes=int(out[2]), device_bytes=int(out[3]),
-                    factor_ms=out[4], l_words=int(out[5]), u_words=int(out[6]), levels=int(out[7]))
+                    factor_ms=out[4], l_words=int(out[5]), u_words=int(out[6]), levels=int(out[7]),
+                    syrk_launches=int(out[8]), syrk_ms=out[9], syrk_flops=out[10], sweeps=int(out[11]),
+                    sweep_ms=out[12], sweep_bytes=out[13], factorizations=int(out[14]), factorization_ms=out[15])
 
     def close(self):
         if getattr(self, "_h", None):

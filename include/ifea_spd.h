@@ -35,7 +35,8 @@ typedef struct ifea_spd* ifea_spd_t;
 
 /* Option keys for ifea_spd_set_option. */
 #define IFEA_OPT_PRECISION 1      /* 0: FP64 factor (default). 1: mixed: FP32 supernode updates on MFMA with FP64
-                                     diagonal-block POTRF; ifea_spd_solve then refines in FP64 (see below). */
+                                     diagonal-block POTRF; 2: FP64 factor, FP32 copy of L for the solves. In 1 and 2
+                                     ifea_spd_solve refines in FP64 (see below). */
 #define IFEA_OPT_REFINE_TOL 2     /* relative residual target of the mixed-precision refinement (default 1e-10) */
 #define IFEA_OPT_REFINE_MAXIT 3   /* refinement iterations before the automatic FP64 refactorisation (default 30) */
 #define IFEA_OPT_LEAF_BLOCKS 4    /* nested-dissection leaf size in column blocks (default 32) */

@@ -36,9 +36,12 @@ typedef struct ifea_step_params {
   int pcg_budget;            /* Forging3DSolver.PcgBudget */
   int quadratic_ls;          /* Forging3DSolver.QuadraticLineSearch */
   int adaptive_newton;       /* Forging3DSolver.AdaptiveNewton */
-  int precision;             /* 0: FP64 factor. 1: FP32 supernode updates (MFMA) + FP64 POTRF/residual/PCG, with an
-                                FP64 refactorisation when PCG on the FP32 factor stalls */
+  int precision;             /* 0: FP64 factor. 1: FP32 supernode updates (MFMA) + FP64 POTRF/residual/PCG; an FP32
+                                factor that is not positive definite switches to 2 for the rest of the run, and PCG
+                                that stalls on a fresh FP32 factor falls back to FP64. 2: FP64 factor with an FP32
+                                copy of L for the triangular solves (half the solve bytes), FP64 PCG refinement */
   int newton_limit;          /* Newton iterations before Picard (20, Forging3DSolver.cs:953) */
+  int profile;               /* 1: time the factor's MFMA update launches and the triangular sweeps with HIP events */
 } ifea_step_params;
 
 /* Statistics of one solve_velocity call. */
@@ -89,8 +92,11 @@ int ifea_step_advance(ifea_step_t h, double dt, double* eps, double* sigma, doub
 int ifea_step_get_v(ifea_step_t h, double* v);
 int ifea_step_set_v(ifea_step_t h, const double* v);
 
-/* Device and factor statistics: [0] nnz(L), [1] factor flops, [2] supernodes, [3] device bytes, [4] last factor ms */
-int ifea_step_info(ifea_step_t h, double* out8);
+/* Device and factor statistics: [0] nnz(L), [1] factor flops, [2] supernodes, [3] device bytes, [4] last factor ms,
+ * [5] L words, [6] update-pool words, [7] levels; with params.profile set, accumulated since the last call with
+ * reset: [8] MFMA update launches, [9] their ms, [10] their algorithmic flops, [11] triangular sweeps (fwd+bwd),
+ * [12] their ms, [13] panel bytes they read, [14] factorizations, [15] factorization ms. */
+int ifea_step_info(ifea_step_t h, int reset, double* out16);
 
 void ifea_step_destroy(ifea_step_t h);
 

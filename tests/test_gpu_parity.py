@@ -1,0 +1,222 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle, through the C ABI.
+
+Bars (from the task statement of the solve step): ordering / elimination tree / pattern bit-exact; velocities within
+1e-9 relative and the ram load within 1e-6 relative in plain FP64; mixed precision within 0.1 % of the load.
+"""
+import threading
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from impressionfea_amd import spd
+from impressionfea_amd.config import upset_small
+from impressionfea_amd.errors import MatrixNotPositiveDefiniteError
+from impressionfea_amd.forging3d import Forging3DSolver
+from impressionfea_amd.mesh import kuhn_cylinder
+
+pytestmark = pytest.mark.gpu
+
+
+def _bc_system(o, fr, v):
+    """The linear system of one Newton iteration (SolveVelocity :891-917) from the oracle's assembly."""
+    e, p, r, val = o.assemble(v, True, True)
+    ns, nb, cp, ri = o.pattern()
+    n = o.ndof
+    fixed = fr["fixed"]
+    col = np.repeat(np.arange(n), np.diff(cp))
+    A = val.copy()
+    msk = fixed[ri] | fixed[col]
+    A[msk] = np.where(ri[msk] == col[msk], 1.0, 0.0)
+    dg = (ri == col) & ~fixed[col]
+    A[dg] += 1e-10 * A[dg].max()
+    b = -np.where(fixed, 0.0, r)
+    return cp, ri, A, b
+
+
+@pytest.fixture(scope="module")
+def c1_system(oracle_lib):
+    import oracle as O
+    wl = upset_small()
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=0)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    o.prepare()
+    fr = o.frames()
+    cp, ri, A, b = _bc_system(o, fr, o.v())
+    return o, cp, ri, A, b
+
+
+def test_self_test():
+    assert spd.self_test() is None
+
+
+def test_spd_fp64_matches_oracle_factor(c1_system):
+    import oracle as O
+    o, cp, ri, A, b = c1_system
+    n = o.ndof
+    with spd.HipSpdSolver() as h:
+        h.Analyze(n, cp, ri)
+        perm, parent, cc = h.symbolic()
+        ch = O.OracleCholesky(n, cp, ri, perm)
+        assert np.array_equal(ch.parent, parent) and np.array_equal(ch.colcount, cc)
+        assert h.FactorNonZeros == ch.nnz
+        h.Factorize(A)
+        x = h.Solve(b)
+        ch.factorize(A)
+        xr = ch.solve(b)
+        assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
+        As = sp.csc_matrix((A, ri, cp), shape=(n, n))
+        assert np.linalg.norm(As @ x - b) <= 1e-9 * np.linalg.norm(b)
+        # repeated Factorize / Solve on the same Analyze, as PCG and Newton do
+        h.Factorize(2.0 * A)
+        x2 = h.Solve(b)
+        assert np.linalg.norm(2 * x2 - xr) <= 1e-9 * np.linalg.norm(xr)
+
+
+def test_spd_caller_permutation_is_kept(c1_system):
+    import oracle as O
+    o, cp, ri, A, b = c1_system
+    n = o.ndof
+    ns, nb, _, _ = o.pattern()
+    perm = O.min_degree_nodes(o.nn, ns, nb)
+    with spd.HipSpdSolver() as h:
+        h.Analyze(n, cp, ri, perm=perm)
+        p2, parent, cc = h.symbolic()
+        assert np.array_equal(p2, perm)
+        bpar, bcc = O.brute_symbolic(n, cp, ri, perm)
+        assert np.array_equal(parent, bpar) and np.array_equal(cc, bcc)
+        h.Factorize(A)
+        x = h.Solve(b)
+        As = sp.csc_matrix((A, ri, cp), shape=(n, n))
+        assert np.linalg.norm(As @ x - b) <= 1e-9 * np.linalg.norm(b)
+
+
+def test_spd_mixed_precision_refines(c1_system):
+    o, cp, ri, A, b = c1_system
+    n = o.ndof
+    As = sp.csc_matrix((A, ri, cp), shape=(n, n))
+    with spd.HipSpdSolver(precision=1, refine_tol=1e-10) as h:
+        h.Analyze(n, cp, ri)
+        h.Factorize(A)
+        x = h.Solve(b)
+        st = h.stats()
+        assert np.linalg.norm(As @ x - b) <= 1e-10 * np.linalg.norm(b) * 10
+        assert st["refine_iterations"] >= 1 or st["fp64_fallbacks"] >= 1
+
+
+def test_spd_not_positive_definite(c1_system):
+    o, cp, ri, A, b = c1_system
+    bad = A.copy()
+    col = np.repeat(np.arange(o.ndof), np.diff(cp))
+    d = np.where((ri == col))[0]
+    bad[d[100]] = -abs(bad[d[100]])
+    with spd.HipSpdSolver() as h:
+        h.Analyze(o.ndof, cp, ri)
+        with pytest.raises(MatrixNotPositiveDefiniteError):
+            h.Factorize(bad)
+
+
+def test_spd_call_order():
+    with spd.HipSpdSolver() as h:
+        with pytest.raises(RuntimeError, match="Analyze must be called first"):
+            h.Factorize([1.0])
+
+
+def test_spd_concurrent_handles(c1_system):
+    """Several handles used from different threads at once (MeshConvergence.cs:117 runs three solvers)."""
+    o, cp, ri, A, b = c1_system
+    n = o.ndof
+    As = sp.csc_matrix((A, ri, cp), shape=(n, n))
+    out, errs = {}, []
+
+    def work(k):
+        try:
+            with spd.HipSpdSolver() as h:
+                h.Analyze(n, cp, ri)
+                h.Factorize(A * (k + 1))
+                out[k] = h.Solve(b)
+        except Exception as ex:  # noqa: BLE001
+            errs.append(ex)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(3)]
+    [t.start() for t in th]
+    [t.join() for t in th]
+    assert not errs, errs
+    for k in range(3):
+        assert np.linalg.norm(As @ (out[k] * (k + 1)) - b) <= 1e-9 * np.linalg.norm(b)
+
+
+def test_assembly_parity(oracle_lib):
+    """Assemble (:1115-1217) on the GPU equals the oracle's for residual, tangent, energy and power."""
+    import oracle as O
+    wl = upset_small(quarter=True, nxy=6, nz=8)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=4, quarter=True)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    o.prepare()
+    s = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0)
+    s.initial(0.0)
+    s.prepare()
+    v = o.v()
+    assert np.abs(s.get_v() - v).max() <= 1e-14 * np.abs(v).max()
+    rng = np.random.default_rng(1)
+    v = v * (1 + 0.1 * rng.standard_normal(v.size))
+    # K_vol v cancels (nearly isochoric flow under a large penalty): rounding is bounded by |K_vol| |v|, so the
+    # residual and energy bars are relative to that scale, not to the (much smaller) result.
+    ns, nb, cp, ri = o.pattern()
+    Kabs = sp.csc_matrix((np.abs(o.geometry()["kvol"]), ri, cp), shape=(o.ndof, o.ndof))
+    rscale = Kabs @ np.abs(v)
+    escale = 0.5 * np.abs(v) @ rscale
+    for matrix, newton in ((True, True), (True, False), (False, False)):
+        e1, p1, r1, val1 = o.assemble(v, matrix, newton)
+        e2, p2, r2, val2 = s.assemble(v, matrix, newton)
+        assert abs(e2 - e1) <= 1e-13 * (escale + abs(e1))
+        assert abs(p2 - p1) <= 1e-13 * (2 * escale + abs(p1))
+        assert np.abs(r2 - r1).max() <= 1e-13 * (rscale.max() + np.abs(r1).max())
+        if matrix:
+            assert np.abs(val2 - val1).max() <= 1e-13 * np.abs(val1).max()
+
+
+def test_increments_parity_fp64(oracle_lib):
+    """Three increments of the upset: GPU FP64 vs oracle, velocities 1e-9 and ram load 1e-6 relative."""
+    import oracle as O
+    wl = upset_small(nxy=8, nz=8, tolerance=1e-12)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=0)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    s = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0)
+    s.initial(0.0)
+    for k in range(3):
+        ro = o.step()
+        rs = s.step()
+        assert rs.converged
+        assert rs.load == pytest.approx(ro["load"], rel=1e-6)
+        vg = o.state()["vg"]
+        assert np.linalg.norm(s.vg - vg) <= 1e-9 * np.linalg.norm(vg)
+        assert np.abs(s.x - o.state()["x"]).max() <= 1e-9 * np.abs(s.x).max()
+
+
+def test_frictionless_upset_gpu():
+    """Solver3DTests.cs:30-49 on the GPU path: F = sigma(rate) A within 1 %."""
+    wl = upset_small(nxy=6, nz=6, friction=0.0, tolerance=1e-8)
+    s = Forging3DSolver.from_workload(wl)
+    r = s.step()
+    x, t = s.x, s.mesh.tets
+    cfg = wl.cfg
+    x0, *_ = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=wl.seed)
+    V = np.abs(np.einsum("ij,ij->i", x0[t[:, 1]] - x0[t[:, 0]],
+                         np.cross(x0[t[:, 2]] - x0[t[:, 0]], x0[t[:, 3]] - x0[t[:, 0]]))).sum() / 6
+    F = cfg.C * cfg.eps0 ** cfg.n * (cfg.ram / wl.height) ** cfg.m * V / wl.height
+    assert r.load / F == pytest.approx(1.0, abs=0.01)
+    del x
+
+
+def test_mixed_precision_load_within_tenth_percent():
+    """Mixed precision (FP32 MFMA factor + FP64 PCG refinement) keeps the ram load within 0.1 % of FP64."""
+    wl = upset_small(nxy=8, nz=8)
+    a = Forging3DSolver.from_workload(wl, precision=0)
+    b = Forging3DSolver.from_workload(wl, precision=1)
+    for k in range(3):
+        ra, rb = a.step(), b.step()
+        assert rb.load == pytest.approx(ra.load, rel=1e-3)

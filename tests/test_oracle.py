@@ -1,0 +1,359 @@
+"""Pins of the FP64 CPU oracle against closed forms, invariants, finite differences and brute force.
+
+The reference ships no golden vectors (its tests use closed forms and cross-backend agreement,
+tests/ImpressionFEA.Core.Tests/Solver3DTests.cs:13-72), so each oracle function is pinned here by mathematics that a
+plausible mistake (a dropped term, a sign, an index) would break.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from impressionfea_amd.config import DieSpec, StepConfig, upset_small
+from impressionfea_amd.mesh import TetMesh, build_pattern, kuhn_box, kuhn_cylinder
+
+pytestmark = []
+
+
+def _oracle(cfg, x, t, ox, oy, gap0, **kw):
+    import oracle as O
+    o = O.Oracle(cfg, x, t, ox, oy, gap0, **kw)
+    return o
+
+
+def _box_mesh(n=3, h=1.0, jitter=0.0, seed=0):
+    nx = ny = nz = n
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    x = np.stack([I.transpose(2, 1, 0).ravel(), J.transpose(2, 1, 0).ravel(), K.transpose(2, 1, 0).ravel()], 1) * h
+    x = x.astype(np.float64) + 5.0 * np.array([0, 0, 0.0])
+    if jitter:
+        rng = np.random.default_rng(seed)
+        interior = np.all((x > 0) & (x < n * h), axis=1)
+        x[interior] += rng.uniform(-jitter, jitter, size=(interior.sum(), 3)) * h
+    t = kuhn_box(nx, ny, nz).astype(np.int32)
+    z = np.zeros(x.shape[0], bool)
+    return x, t, z, z.copy()
+
+
+def test_constants_known_answers(oracle_lib, c1_mesh):
+    """Forging3DSolver.cs:146-150 at the upset workload (values computed from the formulas by hand)."""
+    wl, x, t, ox, oy = c1_mesh
+    c = _oracle(wl.cfg, x, t, ox, oy, wl.gap0).constants()
+    assert c["rateRef"] == pytest.approx(1 / 6, rel=1e-15)
+    assert c["rateCut"] == pytest.approx(1.666667e-4, rel=1e-6)
+    assert c["sigRef"] == pytest.approx(57.870676, rel=1e-7)
+    assert c["kPen"] == pytest.approx(3.472241e8, rel=1e-6)
+    assert c["u0"] == pytest.approx(0.05, rel=1e-15)
+    # the host's own constants agree with the oracle's
+    h = wl.cfg.constants()
+    for k in ("rateRef", "rateCut", "sigRef", "kPen", "u0"):
+        assert h[k] == pytest.approx(c[k], rel=1e-14)
+
+
+def test_pattern_known_answer_and_parity(oracle_lib, c1_mesh):
+    """BuildPattern (:469-521): 587,421 full-CSC entries on the 10^3-cell main-diagonal Kuhn box; host == oracle."""
+    wl, x, t, ox, oy = c1_mesh
+    o = _oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    ns, nb, cp, ri = o.pattern()
+    assert cp[-1] == 587421
+    m = TetMesh(x.shape[0], t, ox, oy)
+    ns2, nb2, cp2, ri2 = build_pattern(m)
+    assert np.array_equal(ns, ns2) and np.array_equal(nb, nb2)
+    assert np.array_equal(cp, cp2) and np.array_equal(ri, ri2)
+
+
+def test_pattern_brute_force_tiny():
+    """nbr[i] = union of patch(n) over n with i in patch(n), by Python sets on a 2x2x2 box."""
+    x, t, ox, oy = _box_mesh(2)
+    m = TetMesh(x.shape[0], t, ox, oy)
+    nn = m.nn
+    patch = [set() for _ in range(nn)]
+    for e in range(m.ne):
+        for a in t[e]:
+            patch[a].update(int(q) for q in t[e])
+    nbr = [set() for _ in range(nn)]
+    for n in range(nn):
+        for i in patch[n]:
+            nbr[i] |= patch[n]
+    ns, nb, cp, ri = build_pattern(m)
+    for i in range(nn):
+        assert sorted(nbr[i]) == list(nb[ns[i]:ns[i + 1]])
+    # full symmetric CSC of 3x3 blocks
+    A = sp.csc_matrix((np.ones(cp[-1]), ri, cp), shape=(3 * nn, 3 * nn))
+    assert (A != A.T).nnz == 0
+
+
+def test_topology_parity(oracle_lib):
+    """TetMesh (TetMesh.cs:9-64): faces in first-occurrence order, owners, symmetry flags (quarter model)."""
+    wl = upset_small(quarter=True, nxy=5, nz=6)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=3, quarter=True)
+    o = _oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    bf, ow, fs, su = o.topology()
+    m = TetMesh(x.shape[0], t, ox, oy)
+    assert np.array_equal(bf, m.boundary_faces) and np.array_equal(ow, m.boundary_face_owner)
+    assert np.array_equal(fs, m.face_is_symmetry) and np.array_equal(su, m.is_surface_node)
+    # outward orientation: the face normal points away from the owner's fourth node
+    for k in range(len(ow)):
+        a, b, c = x[bf[k]]
+        nrm = np.cross(b - a, c - a)
+        d = [q for q in t[ow[k]] if q not in bf[k]][0]
+        assert np.dot(nrm, x[d] - a) < 0
+    # closed surface: sum of outward area vectors vanishes
+    av = np.cross(x[bf[:, 1]] - x[bf[:, 0]], x[bf[:, 2]] - x[bf[:, 0]]).sum(0)
+    assert np.abs(av).max() < 1e-9
+
+
+def test_element_geometry_invariants(oracle_lib):
+    """PrepareStep geometry (:730-753): sum of gradients 0, grad N_a . (x_b - x_0) = delta, exact box volume."""
+    x, t, ox, oy = _box_mesh(3, h=2.0, jitter=0.1)
+    cfg = StepConfig(friction=0.0, size=6.0, h=2.0, bottom=DieSpec(0, 0, (-100.0,) + (0.0,) * 7),
+                     top=DieSpec(1, 0, (100.0,) + (0.0,) * 7))
+    o = _oracle(cfg, x, t, ox, oy, 0.0)
+    o.prepare()
+    g = o.geometry()
+    grad, vol = g["grad"], g["vol"]
+    assert np.abs(grad.sum(axis=1)).max() < 1e-12
+    e = 7
+    xe = x[t[e]]
+    for a in range(1, 4):
+        for b in range(1, 4):
+            assert np.dot(grad[e, a], xe[b] - xe[0]) == pytest.approx(1.0 if a == b else 0.0, abs=1e-12)
+    assert vol.sum() == pytest.approx(6.0 ** 3, rel=1e-13)
+    assert g["vNode"].sum() == pytest.approx(6.0 ** 3, rel=1e-13)
+
+
+def _local(frames, vg):
+    return np.einsum("nij,nj->ni", frames["ax"], vg).ravel()
+
+
+def test_anp_linear_field_closed_form(oracle_lib):
+    """BuildAnp (:761-823): for v = (a x, b y, c z), g_n . v = a + b + c at every node, so
+    1/2 v^T K_vol v = 1/2 kPen (a + b + c)^2 V and K_vol is PSD with rank <= nn."""
+    x, t, ox, oy = _box_mesh(3, h=1.5, jitter=0.15, seed=1)
+    cfg = StepConfig(friction=0.0, size=4.5, h=1.5, penalty_factor=1e3, bottom=DieSpec(0, 0, (-100.0,) + (0.0,) * 7),
+                     top=DieSpec(1, 0, (100.0,) + (0.0,) * 7))
+    o = _oracle(cfg, x, t, ox, oy, 0.0)
+    o.prepare()
+    fr, g = o.frames(), o.geometry()
+    ns, nb, cp, ri = o.pattern()
+    K = sp.csc_matrix((g["kvol"], ri, cp), shape=(o.ndof, o.ndof)).toarray()
+    a, b, c = 0.3, -0.7, 0.15
+    vg = x * np.array([a, b, c])
+    v = _local(fr, vg)
+    kpen = o.constants()["kPen"]
+    assert 0.5 * v @ K @ v == pytest.approx(0.5 * kpen * (a + b + c) ** 2 * g["vol"].sum(), rel=1e-11)
+    assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
+    w = np.linalg.eigvalsh(K)
+    assert w.min() > -1e-9 * w.max()
+    assert (w > 1e-9 * w.max()).sum() <= o.nn
+    # isochoric field: K_vol v = 0
+    v2 = _local(fr, x * np.array([0.5, 0.5, -1.0]))
+    assert np.abs(K @ v2).max() <= 1e-9 * np.abs(K).max() * np.abs(v2).max()
+
+
+def _homog(o, x, rate):
+    """Uniaxial compression at rate r: v = (r x/2, r y/2, -r z)."""
+    return x * np.array([0.5 * rate, 0.5 * rate, -rate])
+
+
+def test_kernel_homogeneous_closed_form(oracle_lib):
+    """Kernel (:1312-1400): under homogeneous compression every tet has rate r, sigma = C (eps0)^n r^m, and the
+    ram power is sigma r V; energy density 1/2 s_c r_c + (s r - s_c r_c)/(m + 1)."""
+    wl = upset_small(nxy=4, nz=4, friction=0.0)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=2)
+    o = _oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.prepare()
+    fr = o.frames()
+    c = o.constants()
+    r = 0.05
+    v = _local(fr, _homog(o, x, r))
+    e, pw, res, val = o.assemble(v, matrix=False, newton=False)
+    st = o.element_state()
+    cfg = wl.cfg
+    sig = cfg.C * cfg.eps0 ** cfg.n * r ** cfg.m
+    assert np.allclose(st["elRate"], r, rtol=1e-12)
+    assert np.allclose(st["elSig"], sig, rtol=1e-12)
+    V = o.geometry()["vol"].sum()
+    assert pw == pytest.approx(sig * r * V, rel=1e-10)
+    sc = cfg.C * cfg.eps0 ** cfg.n * c["rateCut"] ** cfg.m
+    wdens = 0.5 * sc * c["rateCut"] + (sig * r - sc * c["rateCut"]) / (cfg.m + 1)
+    assert e == pytest.approx(wdens * V, rel=1e-10)
+
+
+def _fd_setup(blend=1.0, friction=0.3):
+    wl = upset_small(nxy=3, nz=4, friction=friction, newton_blend=blend)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=5)
+    o = _oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    o.prepare()
+    rng = np.random.default_rng(7)
+    v = o.v() * (1 + 0.2 * rng.standard_normal(o.ndof)) + 0.01 * rng.standard_normal(o.ndof)
+    return o, v, rng
+
+
+def test_residual_is_energy_gradient(oracle_lib):
+    """Assemble (:1115-1217): r = dPi/dv (central differences), friction included."""
+    o, v, rng = _fd_setup()
+    e0, _, r, _ = o.assemble(v, matrix=False, newton=False)
+    for d in rng.choice(o.ndof, 12, replace=False):
+        h = 1e-6 * max(1.0, abs(v[d]))
+        vp, vm = v.copy(), v.copy()
+        vp[d] += h
+        vm[d] -= h
+        fd = (o.assemble(vp, False, False)[0] - o.assemble(vm, False, False)[0]) / (2 * h)
+        assert fd == pytest.approx(r[d], rel=2e-5, abs=1e-6 * np.abs(r).max())
+
+
+def test_newton_tangent_is_residual_jacobian(oracle_lib):
+    """With NewtonBlend = 1 the assembled tangent is dr/dv (Kernel :1390-1399 and friction :1202-1212)."""
+    o, v, rng = _fd_setup(blend=1.0)
+    _, _, r, val = o.assemble(v, matrix=True, newton=True)
+    ns, nb, cp, ri = o.pattern()
+    K = sp.csc_matrix((val, ri, cp), shape=(o.ndof, o.ndof))
+    d = rng.standard_normal(o.ndof)
+    h = 1e-7 * np.linalg.norm(v) / np.linalg.norm(d)
+    rp = o.assemble(v + h * d, False, False)[2]
+    rm = o.assemble(v - h * d, False, False)[2]
+    fd = (rp - rm) / (2 * h)
+    kd = K @ d
+    assert np.linalg.norm(fd - kd) <= 1e-5 * np.linalg.norm(kd)
+    # symmetric tangent
+    assert abs(K - K.T).max() <= 1e-10 * abs(K).max()
+
+
+def test_rigid_body_modes(oracle_lib):
+    """Rigid translation: zero strain rate and divergence, so r = 0 and K v = 0 (friction off)."""
+    o, v, rng = _fd_setup(friction=0.0)
+    fr = o.frames()
+    vg = np.tile(np.array([0.3, -0.2, 0.7]), (o.nn, 1))
+    vr = _local(fr, vg)
+    _, _, r, val = o.assemble(vr, matrix=True, newton=False)
+    ns, nb, cp, ri = o.pattern()
+    K = sp.csc_matrix((val, ri, cp), shape=(o.ndof, o.ndof))
+    assert np.abs(K @ vr).max() <= 1e-10 * abs(K).max()
+    assert np.abs(r).max() <= 1e-10 * abs(K).max()
+
+
+def test_frictionless_upset_load(oracle_lib):
+    """Solver3DTests.FrictionlessUpsetMatchesHomogeneousSolution (:13-49): first-increment ram load of a frictionless
+    upset within 1% of sigma(rate) * A, A = V / H."""
+    wl = upset_small(nxy=6, nz=6, friction=0.0, tolerance=1e-8)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=0)
+    o = _oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    vol = o.geometry
+    s = o.step()
+    m = TetMesh(x.shape[0], t, ox, oy)
+    V = np.abs(np.einsum("ij,ij->i", x[t[:, 1]] - x[t[:, 0]],
+                         np.cross(x[t[:, 2]] - x[t[:, 0]], x[t[:, 3]] - x[t[:, 0]]))).sum() / 6
+    H = wl.height
+    cfg = wl.cfg
+    rate = cfg.ram / H
+    F = cfg.C * cfg.eps0 ** cfg.n * rate ** cfg.m * V / H
+    assert s["load"] / F == pytest.approx(1.0, abs=0.01)
+    assert s["load"] == pytest.approx(s["reaction"], rel=1e-4)
+    del vol, m
+
+
+def test_frames_parity_and_invariants(oracle_lib):
+    """PrepareStep frames (:669-727), quarter model with symmetry planes and both dies: host numpy == oracle C++."""
+    from impressionfea_amd.forging3d import compute_frames
+    import oracle as O
+    wl = upset_small(quarter=True, nxy=5, nz=6)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=3, quarter=True)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    o.prepare()
+    fr = o.frames()
+    st = o.state()
+    ax, nfix, die_axis, fixed_val = compute_frames(st["x"], ox, oy, st["contact"], st["gap"], wl.cfg)
+    assert np.abs(ax - fr["ax"]).max() < 1e-14
+    assert np.array_equal(nfix, fr["nFix"]) and np.array_equal(die_axis, fr["dieAxis"])
+    assert np.abs(fixed_val - fr["fixedVal"]).max() < 1e-14
+    assert np.abs(np.einsum("nij,nkj->nik", ax, ax) - np.eye(3)).max() < 1e-14
+    # a top-die node on the axis of symmetry: x, y and z fixed; the die axis carries the ram speed
+    top = np.where((st["contact"] == 2) & ox & oy)[0]
+    assert top.size and (nfix[top] == 3).all()
+    assert fixed_val.reshape(-1, 3)[top, die_axis[top]] == pytest.approx(wl.cfg.ram)
+
+
+def test_oracle_cholesky_vs_dense(oracle_lib):
+    """FP64 left-looking sparse Cholesky: solution equals dense LAPACK; not-SPD is reported."""
+    import oracle as O
+    rng = np.random.default_rng(0)
+    x, t, ox, oy = _box_mesh(2)
+    m = TetMesh(x.shape[0], t, ox, oy)
+    ns, nb, cp, ri = build_pattern(m)
+    n = 3 * m.nn
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    col = np.repeat(np.arange(n), np.diff(cp))
+    vals = A[ri, col]
+    Asp = sp.csc_matrix((vals, ri, cp), shape=(n, n)).toarray()
+    b = rng.standard_normal(n)
+    perm = O.min_degree_nodes(m.nn, ns, nb)
+    ch = O.OracleCholesky(n, cp, ri, perm)
+    ch.factorize(vals)
+    xs = ch.solve(b)
+    assert np.linalg.norm(xs - np.linalg.solve(Asp, b)) <= 1e-12 * np.linalg.norm(xs)
+    bad = vals.copy()
+    bad[(ri == col) & (ri == 5)] = -1.0
+    with pytest.raises(ArithmeticError):
+        ch.factorize(bad)
+
+
+def test_oracle_symbolic_brute_force(oracle_lib):
+    """Sparse symbolic (etree, column counts) equals brute-force elimination on a dense boolean matrix."""
+    import oracle as O
+    x, t, ox, oy = _box_mesh(3, jitter=0.1)
+    m = TetMesh(x.shape[0], t, ox, oy)
+    ns, nb, cp, ri = build_pattern(m)
+    n = 3 * m.nn
+    perm = np.random.default_rng(4).permutation(n).astype(np.int32)
+    ch = O.OracleCholesky(n, cp, ri, perm)
+    par, cc = O.brute_symbolic(n, cp, ri, perm)
+    assert np.array_equal(ch.parent, par) and np.array_equal(ch.colcount, cc)
+    assert ch.nnz == cc.sum()
+
+
+def test_mkl_pardiso_agrees(oracle_lib, c1_mesh):
+    """MKL PARDISO with the reference's iparm (MklPardisoSolver.cs:20-31) solves the BC-applied upset system like
+    the oracle's Cholesky."""
+    import oracle as O
+    if not O.pardiso_available():
+        pytest.skip("MKL runtime not present")
+    wl, x, t, ox, oy = c1_mesh
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    o.prepare()
+    fr = o.frames()
+    e, p, r, val = o.assemble(o.v(), True, True)
+    ns, nb, cp, ri = o.pattern()
+    n = o.ndof
+    fixed = fr["fixed"]
+    col = np.repeat(np.arange(n), np.diff(cp))
+    A = val.copy()
+    msk = fixed[ri] | fixed[col]
+    A[msk] = np.where(ri[msk] == col[msk], 1.0, 0.0)
+    dg = (ri == col) & ~fixed[col]
+    A[dg] += 1e-10 * A[dg].max()
+    b = -np.where(fixed, 0.0, r)
+    ch = O.OracleCholesky(n, cp, ri, O.min_degree_nodes(o.nn, ns, nb))
+    ch.factorize(A)
+    x1 = ch.solve(b)
+    pd = O.Pardiso(n, cp, ri)
+    pd.factorize(A)
+    x2 = pd.solve(b)
+    assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x1)
+    assert pd.info()["nnz_l"] > 0
+
+
+def test_oracle_newton_converges(oracle_lib):
+    """SolveVelocity (:875-948): Picard then Newton converges to the tolerance on the upset."""
+    wl = upset_small(nxy=5, nz=5)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=0)
+    o = _oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    o.prepare()
+    s = o.solve_velocity()
+    assert s["converged"] and s["last_rel"] < wl.cfg.tolerance
+    assert s["factorizations"] >= 1
