@@ -20,6 +20,9 @@ LIB_PATH = os.path.join(HERE, "_lib", "libifea_amd.so")
 SPD_SYMBOLS = ["ifea_spd_create", "ifea_spd_name", "ifea_spd_analyze", "ifea_spd_analyze_perm", "ifea_spd_factorize",
                "ifea_spd_solve", "ifea_spd_factor_nnz", "ifea_spd_set_option", "ifea_spd_symbolic", "ifea_spd_stats",
                "ifea_spd_destroy", "ifea_spd_last_error", "ifea_symbolic_analyze", "ifea_spd_set_coordinates"]
+MLP_SYMBOLS = ["ifea_mlp_create", "ifea_mlp_num_params", "ifea_mlp_set_params", "ifea_mlp_get_params",
+               "ifea_mlp_forward", "ifea_mlp_loss_grad", "ifea_mlp_set_dataset", "ifea_mlp_train_epoch",
+               "ifea_mlp_destroy"]
 STEP_SYMBOLS = ["ifea_step_create", "ifea_step_set_params", "ifea_step_set_perm", "ifea_step_prepare",
                 "ifea_step_set_element_state", "ifea_step_assemble", "ifea_step_solve_velocity", "ifea_step_final",
                 "ifea_step_advance", "ifea_step_get_v", "ifea_step_set_v", "ifea_step_info", "ifea_step_destroy"]

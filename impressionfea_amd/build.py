@@ -17,7 +17,7 @@ LIB = os.path.join(OUT, "libifea_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["symbolic.cpp", "vec.hip", "spd_device.hip", "step_device.hip", "api.hip"]
+SOURCES = ["symbolic.cpp", "vec.hip", "spd_device.hip", "step_device.hip", "mlp.hip", "api.hip"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-Wno-unused-function",
           "-Wno-unused-variable"]
 HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -83,7 +83,8 @@ class SpdDevice {
   DBuf<long long> amap_dst_;
   // task arrays (int4: a, b, c, d)
   DBuf<int4> ext_tasks_, panel_tasks_, syrk_tasks_, zero_tasks_;
-  DBuf<int> level_sn_;
+  DBuf<int> level_sn_, sweep_sn_;
+  DBuf<int4> fwd_tasks_, bwd_tasks_;
   // launch schedule
   struct Level {
     std::vector<LaunchList> ext_rounds;
@@ -91,6 +92,11 @@ class SpdDevice {
     std::vector<LaunchList> panel_steps, syrk_steps;
     long long sn_begin = 0, sn_end = 0;  // into level_sn_
     std::vector<double> syrk_flops;      // algorithmic flops per syrk step
+    // triangular sweeps: narrow fronts one workgroup each; wide fronts blocked over several launches
+    long long small_begin = 0, small_end = 0, big_begin = 0, big_end = 0;  // into sweep_sn_
+    long long small_max_m = 0;
+    std::vector<LaunchList> fwd_steps, bwd_steps;  // bwd_steps in launch (descending) order
+    LaunchList bwd_gemv;
     double solve_bytes = 0;              // panel bytes per element of storage (times sizeof(T) at run time)
     long long max_m = 0;
   };

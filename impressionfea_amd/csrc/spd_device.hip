@@ -20,6 +20,10 @@ constexpr int NB = 64;          // panel width
 constexpr int PANEL_ROWS = 256; // rows per panel task (one per thread)
 constexpr int EXT_TARGET = 8192;
 constexpr long long ZERO_CHUNK = 1 << 16;
+constexpr int BIG_K = 256;      // supernodes wider than this use the blocked multi-workgroup triangular sweeps
+constexpr int SB = 128;         // diagonal block of the blocked sweeps
+constexpr int SWEEP_ROWS = 256; // rows (forward) per workgroup of a blocked sweep step
+constexpr int SWEEP_COLS = 64;  // columns (backward) per workgroup of a blocked sweep step
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -293,6 +297,172 @@ __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev
   }
 }
 
+
+// ---------------------------------------------------------------------------------------- blocked sweeps (wide fronts)
+// Forward: W_s = [y_own ; 0] + children's update vectors (fixed child order).
+__global__ __launch_bounds__(256) void k_fwdb_gather(const int* __restrict__ sns, SymDev sd, const double* __restrict__ y,
+                                                     const double* __restrict__ V, double* __restrict__ Wg) {
+  const int s = sns[blockIdx.x];
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  double* w = Wg + sd.Woff[s];
+  for (long long i = threadIdx.x; i < m; i += blockDim.x) w[i] = i < k ? y[f + i] : 0.0;
+  __syncthreads();
+  for (int cp = sd.child_ptr[s]; cp < sd.child_ptr[s + 1]; cp++) {
+    int c = sd.child_list[cp];
+    int mu = (int)(sn_m(sd, c) - sn_k(sd, c));
+    const int* rel = sd.rel + sd.rel_off[c];
+    const double* vc = V + sd.Voff[c];
+    for (int i = threadIdx.x; i < mu; i += blockDim.x) w[rel[i]] += vc[i];
+    __syncthreads();
+  }
+}
+
+// One 128-column step of the forward sweep. task: (s, first row, end row, writer). Every workgroup solves the diagonal
+// block in one wave (lane i: rows i and 64+i) from W_s (not modified by this launch), then updates its own rows >= c0+w.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwdb_step(const int4* __restrict__ tasks, int c0, SymDev sd,
+                                                   const T* __restrict__ L, double* __restrict__ y,
+                                                   double* __restrict__ Wg) {
+  __shared__ double xs[SB];
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int w = min(SB, k - c0);
+  const T* P = L + sd.Loff[s];
+  double* W = Wg + sd.Woff[s];
+  if (threadIdx.x < 64) {
+    const int i = threadIdx.x;
+    double r0 = i < w ? W[c0 + i] : 0.0, r1 = 64 + i < w ? W[c0 + 64 + i] : 0.0;
+    for (int q = 0; q < w; q++) {
+      const T* cq = P + (long long)(c0 + q) * m + c0;
+      double l0 = i < w ? (double)cq[i] : 0.0, l1 = 64 + i < w ? (double)cq[64 + i] : 0.0;
+      double xq, dq;
+      if (q < 64) {
+        xq = __shfl(r0, q, 64);
+        dq = __shfl(l0, q, 64);
+      } else {
+        xq = __shfl(r1, q - 64, 64);
+        dq = __shfl(l1, q - 64, 64);
+      }
+      xq /= dq;
+      if (q < 64) {
+        if (i == q) r0 = xq;
+        else if (i > q) r0 -= l0 * xq;
+        r1 -= l1 * xq;
+      } else {
+        if (64 + i == q) r1 = xq;
+        else if (64 + i > q) r1 -= l1 * xq;
+      }
+    }
+    xs[i] = r0;
+    xs[64 + i] = r1;
+    if (tk.w) {
+      if (i < w) y[f + c0 + i] = r0;
+      if (64 + i < w) y[f + c0 + 64 + i] = r1;
+    }
+  }
+  __syncthreads();
+  const long long row = (long long)tk.y + threadIdx.x;
+  if (row < tk.z) {
+    double acc = 0.0;
+    for (int q = 0; q < w; q++) acc += (double)P[(long long)(c0 + q) * m + row] * xs[q];
+    W[row] -= acc;
+  }
+}
+
+// Update vector of a wide front for its parent: V_s = W_s[k..m).
+__global__ __launch_bounds__(256) void k_fwdb_finish(const int* __restrict__ sns, SymDev sd, const double* __restrict__ Wg,
+                                                     double* __restrict__ V) {
+  const int s = sns[blockIdx.x];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const double* w = Wg + sd.Woff[s];
+  double* vs = V + sd.Voff[s];
+  for (long long i = k + threadIdx.x; i < m; i += blockDim.x) vs[i - k] = w[i];
+}
+
+// Backward, first part: z_q = y_own[q] - L21(:, q)^T x_rows for every own column q (wave per column).
+// task: (s, first column, end column, 0)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
+                                                   const double* __restrict__ y, double* __restrict__ Wg) {
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int* rows = sd.sn_rows + sd.sn_rowptr[s];
+  const T* P = L + sd.Loff[s];
+  double* z = Wg + sd.Woff[s];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = tk.y + wv; q < tk.z; q += 4) {
+    const T* cq = P + (long long)q * m;
+    double acc = 0.0;
+    for (long long i = k + lane; i < m; i += 64) acc += (double)cq[i] * y[rows[i]];
+    acc = wave_sum(acc);
+    if (lane == 0) z[q] = y[f + q] - acc;
+  }
+}
+
+// One 128-column step (descending) of the backward sweep. task: (s, first column, end column, writer); columns < c0.
+// Every workgroup solves L_tt^T x_t = z_t in one wave from an LDS copy of the block, then updates its own columns
+// j < c0: z_j -= L(c0.., j)^T x_t (wave per column).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwdb_step(const int4* __restrict__ tasks, int c0, SymDev sd,
+                                                   const T* __restrict__ L, double* __restrict__ y,
+                                                   double* __restrict__ Wg) {
+  __shared__ double Dt[SB][SB + 1];  // Dt[c][r] = L(c0 + r, c0 + c)
+  __shared__ double xs[SB];
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int w = min(SB, k - c0);
+  const T* P = L + sd.Loff[s];
+  double* z = Wg + sd.Woff[s];
+  for (int idx = threadIdx.x; idx < SB * SB; idx += blockDim.x) {
+    int c = idx / SB, r = idx % SB;
+    Dt[c][r] = (c < w && r < w && r >= c) ? (double)P[(long long)(c0 + c) * m + c0 + r] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int i = threadIdx.x;
+    double r0 = i < w ? z[c0 + i] : 0.0, r1 = 64 + i < w ? z[c0 + 64 + i] : 0.0;
+    for (int q = w - 1; q >= 0; q--) {
+      // x_q = z_q / L_qq; then z_p -= L(q, p) x_q for p < q, L(q, p) = Dt[p][q]
+      double xq = (q < 64 ? __shfl(r0, q, 64) : __shfl(r1, q - 64, 64)) / Dt[q][q];
+      if (q < 64) {
+        if (i == q) r0 = xq;
+        else if (i < q) r0 -= Dt[i][q] * xq;
+      } else {
+        if (64 + i == q) r1 = xq;
+        else if (64 + i < q) r1 -= Dt[64 + i][q] * xq;
+        r0 -= Dt[i][q] * xq;
+      }
+    }
+    xs[i] = r0;
+    xs[64 + i] = r1;
+    if (tk.w) {
+      if (i < w) y[f + c0 + i] = r0;
+      if (64 + i < w) y[f + c0 + 64 + i] = r1;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = tk.y + wv; j < tk.z; j += 4) {
+    const T* cj = P + (long long)j * m + c0;
+    double acc = 0.0;
+    for (int i = lane; i < w; i += 64) acc += (double)cj[i] * xs[i];
+    acc = wave_sum(acc);
+    if (lane == 0) z[j] -= acc;
+  }
+}
+
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
@@ -428,7 +598,69 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       max_slots = std::max(max_slots, nslot);
     }
   }
+  // sweep schedule: narrow fronts (k <= BIG_K) one workgroup each, wide fronts in 128-column steps
+  std::vector<int> swp;
+  std::vector<int4> fwt, bwt;
+  for (int l = 0; l < nl; l++) {
+    Level& L = levels_[l];
+    std::vector<int> small, big;
+    for (int s : bylevel[l]) ((S.sn_first[s + 1] - S.sn_first[s]) > BIG_K ? big : small).push_back(s);
+    L.small_begin = (long long)swp.size();
+    for (int s : small) {
+      swp.push_back(s);
+      L.small_max_m = std::max<long long>(L.small_max_m, S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+    }
+    L.small_end = (long long)swp.size();
+    L.big_begin = (long long)swp.size();
+    swp.insert(swp.end(), big.begin(), big.end());
+    L.big_end = (long long)swp.size();
+    int maxk = 0;
+    for (int s : big) maxk = std::max(maxk, S.sn_first[s + 1] - S.sn_first[s]);
+    for (int c0 = 0; c0 < maxk; c0 += SB) {
+      LaunchList ll;
+      ll.begin = (long long)fwt.size();
+      ll.c0 = c0;
+      for (int s : big) {
+        int k = S.sn_first[s + 1] - S.sn_first[s];
+        if (k <= c0) continue;
+        long long m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+        long long r0 = c0 + std::min(SB, k - c0);
+        long long chunks = std::max<long long>(1, (m - r0 + SWEEP_ROWS - 1) / SWEEP_ROWS);
+        for (long long c = 0; c < chunks; c++) {
+          long long a = r0 + c * SWEEP_ROWS, b = std::min<long long>(m, a + SWEEP_ROWS);
+          fwt.push_back(make_int4(s, (int)a, (int)std::max(a, b), c == 0 ? 1 : 0));
+        }
+      }
+      ll.end = (long long)fwt.size();
+      L.fwd_steps.push_back(ll);
+    }
+    L.bwd_gemv.begin = (long long)bwt.size();
+    for (int s : big) {
+      int k = S.sn_first[s + 1] - S.sn_first[s];
+      for (int q = 0; q < k; q += SWEEP_COLS) bwt.push_back(make_int4(s, q, std::min(k, q + SWEEP_COLS), 0));
+    }
+    L.bwd_gemv.end = (long long)bwt.size();
+    for (int c0 = ((maxk - 1) / SB) * SB; maxk > 0 && c0 >= 0; c0 -= SB) {
+      LaunchList ll;
+      ll.begin = (long long)bwt.size();
+      ll.c0 = c0;
+      for (int s : big) {
+        int k = S.sn_first[s + 1] - S.sn_first[s];
+        if (k <= c0) continue;
+        long long chunks = std::max<long long>(1, (c0 + SWEEP_COLS - 1) / SWEEP_COLS);
+        for (long long c = 0; c < chunks; c++) {
+          long long a = c * SWEEP_COLS, b = std::min<long long>(c0, a + SWEEP_COLS);
+          bwt.push_back(make_int4(s, (int)a, (int)std::max(a, b), c == 0 ? 1 : 0));
+        }
+      }
+      ll.end = (long long)bwt.size();
+      L.bwd_steps.push_back(ll);
+    }
+  }
   // upload
+  sweep_sn_.upload(swp, stream_);
+  if (fwt.empty()) fwd_tasks_.alloc(1); else fwd_tasks_.upload(fwt, stream_);
+  if (bwt.empty()) bwd_tasks_.alloc(1); else bwd_tasks_.upload(bwt, stream_);
   perm_.upload(S.perm, stream_);
   sn_first_.upload(S.sn_first, stream_);
   sn_rowptr_.upload(S.sn_rowptr, stream_);
@@ -558,19 +790,38 @@ void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
   const int cap = 8192;  // doubles of LDS per forward/backward work vector
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
-    unsigned cnt = (unsigned)(lv.sn_end - lv.sn_begin);
-    if (!cnt) continue;
-    size_t sh = (size_t)std::min<long long>(lv.max_m, cap) * sizeof(double);
-    hipLaunchKernelGGL(k_fwd<T>, dim3(cnt), dim3(256), sh, stream_, level_sn_.p + lv.sn_begin, sd_, L, y_.p, V_.p,
-                       W_.p, cap);
+    unsigned cnt = (unsigned)(lv.small_end - lv.small_begin);
+    if (cnt) {
+      size_t sh = (size_t)std::min<long long>(lv.small_max_m, cap) * sizeof(double);
+      hipLaunchKernelGGL(k_fwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L, y_.p,
+                         V_.p, W_.p, cap);
+    }
+    unsigned nbig = (unsigned)(lv.big_end - lv.big_begin);
+    if (nbig) {
+      hipLaunchKernelGGL(k_fwdb_gather, dim3(nbig), dim3(256), 0, stream_, sweep_sn_.p + lv.big_begin, sd_, y_.p,
+                         V_.p, W_.p);
+      for (const LaunchList& st : lv.fwd_steps)
+        hipLaunchKernelGGL(k_fwdb_step<T>, dim3((unsigned)(st.end - st.begin)), dim3(256), 0, stream_,
+                           fwd_tasks_.p + st.begin, st.c0, sd_, L, y_.p, W_.p);
+      hipLaunchKernelGGL(k_fwdb_finish, dim3(nbig), dim3(256), 0, stream_, sweep_sn_.p + lv.big_begin, sd_, W_.p,
+                         V_.p);
+    }
   }
   for (size_t l = levels_.size(); l-- > 0;) {
     const Level& lv = levels_[l];
-    unsigned cnt = (unsigned)(lv.sn_end - lv.sn_begin);
-    if (!cnt) continue;
-    size_t sh = (size_t)std::min<long long>(lv.max_m, cap) * sizeof(double);
-    hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, stream_, level_sn_.p + lv.sn_begin, sd_, L, y_.p, W_.p,
-                       cap);
+    unsigned cnt = (unsigned)(lv.small_end - lv.small_begin);
+    if (cnt) {
+      size_t sh = (size_t)std::min<long long>(lv.small_max_m, cap) * sizeof(double);
+      hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L, y_.p,
+                         W_.p, cap);
+    }
+    if (lv.big_end > lv.big_begin) {
+      hipLaunchKernelGGL(k_bwdb_gemv<T>, dim3((unsigned)(lv.bwd_gemv.end - lv.bwd_gemv.begin)), dim3(256), 0,
+                         stream_, bwd_tasks_.p + lv.bwd_gemv.begin, sd_, L, y_.p, W_.p);
+      for (const LaunchList& st : lv.bwd_steps)
+        hipLaunchKernelGGL(k_bwdb_step<T>, dim3((unsigned)(st.end - st.begin)), dim3(256), 0, stream_,
+                           bwd_tasks_.p + st.begin, st.c0, sd_, L, y_.p, W_.p);
+    }
   }
   if (profiling_) {
     IFEA_HIP(hipEventRecord(event(1), stream_));

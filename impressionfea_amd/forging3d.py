@@ -273,6 +273,25 @@ class Forging3DSolver:
                                  N.ptr(fv, C.c_double), N.ptr(ct, C.c_int), N.ptr(vgs, C.c_double))
         self._check(rc, "PrepareStep")
 
+    def _learned_start(self) -> str:
+        """PrepareStep with the learned guess, judged against the previous-increment start by the convex functional
+        Pi (the solver's own energy): the lower-energy start is kept, so a poor prediction costs one residual-only
+        assembly and never a worse start."""
+        guess = self.warm_start(self)
+        self.prepare(guess)
+        v_learned = self.get_v()
+        e_learned = self.assemble(None, matrix=False, newton=False)[0]
+        v_prev = np.einsum("nij,nj->ni", self.ax, self.vg).ravel()
+        nf = self.nfix
+        fixed = np.arange(3)[None, :] < nf[:, None]
+        v_prev[fixed.ravel()] = self.fixed_val[fixed.ravel()]
+        e_prev = self.assemble(v_prev, matrix=False, newton=False)[0]
+        if e_learned < e_prev:
+            self.set_v(v_learned)
+            return "learned"
+        self.set_v(v_prev)
+        return "previous"
+
     def solve_velocity(self):
         st = N.StepStats()
         self._check(N.lib().ifea_step_solve_velocity(self._h, int(self.warm), C.byref(st)), "SolveVelocity")
@@ -343,9 +362,13 @@ class Forging3DSolver:
         cfg = self.cfg
         self.step_index += 1
         t0 = time.perf_counter()
-        vguess = self.warm_start(self) if (self.warm_start is not None and self.warm) else None
-        self.prepare(vguess)
+        judged = None
+        if self.warm_start is not None and self.warm:
+            judged = self._learned_start()
+        else:
+            self.prepare()
         st = self.solve_velocity()
+        st["start"] = judged or "previous"
         if not st["converged"] and st["last_rel"] >= 1e-2:
             raise SolverDivergenceError(f"Velocity solution did not converge at step {self.step_index}: "
                                         f"|dv|/|v| = {st['last_rel']:.2E}.")
@@ -383,7 +406,7 @@ class Forging3DSolver:
         e, pw = C.c_double(0), C.c_double(0)
         r = np.zeros(self.ndof)
         val = np.zeros(self.nnz) if matrix else None
-        vv = N.f64(v)
+        vv = N.f64(v) if v is not None else None
         self._check(N.lib().ifea_step_assemble(self._h, int(matrix), int(newton), N.ptr(vv, C.c_double), C.byref(e),
                                                C.byref(pw), N.ptr(r, C.c_double), N.ptr(val, C.c_double)), "Assemble")
         return e.value, pw.value, r, val
