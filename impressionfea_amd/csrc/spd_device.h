@@ -70,6 +70,7 @@ class SpdDevice {
  private:
   template <typename T> int factorize_t(const double* d_vals);
   template <typename T> void solve_t(const T* L, const double* d_b, double* d_x);
+  template <typename T> void sweeps(const T* L);  // forward + backward sweeps on y_ (captured into a HIP graph)
 
   hipStream_t stream_;
   Symbolic S_;
@@ -82,16 +83,18 @@ class SpdDevice {
   DBuf<int> amap_src_;
   DBuf<long long> amap_dst_;
   // task arrays (int4: a, b, c, d)
-  DBuf<int4> ext_tasks_, panel_tasks_, syrk_tasks_, zero_tasks_;
+  DBuf<int4> ext_tasks_, panel_tasks_, syrk_tasks_, zero_tasks_, potrf_tasks_;
   DBuf<int> level_sn_, sweep_sn_;
   DBuf<int4> fwd_tasks_, bwd_tasks_;
   // launch schedule
   struct Level {
     std::vector<LaunchList> ext_rounds;
     LaunchList zero;
-    std::vector<LaunchList> panel_steps, syrk_steps;
+    std::vector<LaunchList> panel_steps, syrk_steps, potrf_steps;
     long long sn_begin = 0, sn_end = 0;  // into level_sn_
     std::vector<double> syrk_flops;      // algorithmic flops per syrk step
+    std::vector<LaunchList> outer_steps; // outer (256-column) updates, launched after the matching panel step
+    std::vector<double> outer_flops;
     // triangular sweeps: narrow fronts one workgroup each; wide fronts blocked over several launches
     long long small_begin = 0, small_end = 0, big_begin = 0, big_end = 0;  // into sweep_sn_
     long long small_max_m = 0;
@@ -105,6 +108,11 @@ class SpdDevice {
   DBuf<unsigned char> L_, U_;
   DBuf<float> L32_;
   bool solve64_ = false;
+  // the whole triangular-sweep sequence replayed as one HIP graph (rebuilt when the factor storage changes)
+  hipGraphExec_t sweep_exec_ = nullptr;
+  const void* sweep_key_ = nullptr;
+  size_t sweep_tsize_ = 0;
+  void drop_graph();
   DBuf<double> V_, W_, y_, dscratch_;
   DBuf<int> err_;
   SymDev sd_{};

@@ -17,10 +17,12 @@
 namespace ifea {
 
 constexpr int NB = 64;          // panel width
+constexpr int OB = 256;         // outer block: the trailing matrix beyond it is updated once per OB columns
 constexpr int PANEL_ROWS = 256; // rows per panel task (one per thread)
 constexpr int EXT_TARGET = 8192;
 constexpr long long ZERO_CHUNK = 1 << 16;
-constexpr int BIG_K = 256;      // supernodes wider than this use the blocked multi-workgroup triangular sweeps
+constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
+constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
 constexpr int SB = 128;         // diagonal block of the blocked sweeps
 constexpr int SWEEP_ROWS = 256; // rows (forward) per workgroup of a blocked sweep step
 constexpr int SWEEP_COLS = 64;  // columns (backward) per workgroup of a blocked sweep step
@@ -161,6 +163,113 @@ __global__ __launch_bounds__(256) void k_panel(const int4* __restrict__ tasks, i
   }
 }
 
+
+// Diagonal block of one 64-column step: FP64 Cholesky in one wave (lane i owns row i, columns broadcast through LDS),
+// then the inverse of the factor (lane c solves L x = e_c). The factor goes into the panel; L11^{-T} goes to the
+// scratch slot for the TRSM launch that follows. task: (supernode s, slot, 0, 0)
+template <typename T>
+__global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks, int c0, SymDev sd,
+                                                  T* __restrict__ L, double* __restrict__ dscratch,
+                                                  int* __restrict__ err) {
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int w = min(NB, k - c0);
+  T* P = L + sd.Loff[s];
+  __shared__ double D[NB][NB + 1];
+  const int i = threadIdx.x;
+  for (int j = 0; j < NB; j++) {
+    double v;
+    if (i < w && j < w) v = (i >= j) ? (double)P[(long long)(c0 + j) * m + c0 + i] : 0.0;
+    else v = (i == j) ? 1.0 : 0.0;
+    D[i][j] = v;
+  }
+  wave_sync_lds();
+  bool bad = false;
+  for (int j = 0; j < NB; j++) {
+    double piv = D[j][j];
+    bad |= !(piv > 0.0) || !isfinite(piv);
+    double d = sqrt(piv > 0.0 ? piv : 1.0);
+    double lij = (i > j) ? D[i][j] / d : 0.0;
+    wave_sync_lds();
+    if (i == j) D[i][j] = d;
+    else if (i > j) D[i][j] = lij;
+    wave_sync_lds();
+    for (int q = j + 1; q <= i; q++) D[i][q] -= lij * D[q][j];
+    wave_sync_lds();
+  }
+  for (int q = i + 1; q < NB; q++) D[i][q] = 0.0;
+  wave_sync_lds();
+  // factor into the panel (column q, row i)
+  for (int q = 0; q < w; q++)
+    if (i < w && i >= q) P[(long long)(c0 + q) * m + c0 + i] = (T)D[i][q];
+  if (bad && i == 0) atomicOr(err, 1);
+  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; store Lt[p][q] = Linv[q][p]
+  double* Lt = dscratch + (long long)tk.y * NB * NB;
+  const int c = i;
+  double x[NB];
+#pragma unroll
+  for (int r = 0; r < NB; r++) {
+    double a = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int p = 0; p < r; p++) a -= D[r][p] * x[p];
+    x[r] = a / D[r][r];
+  }
+#pragma unroll
+  for (int r = 0; r < NB; r++) Lt[c * NB + r] = x[r];  // Linv[r][c] -> Lt[c][r]
+}
+
+// Rows below the diagonal block: X = A L11^{-T} as a 64x64x64 FP64 MFMA product per task, written back into the
+// panel. task: (supernode s, first row, end row (<= first + 64), slot)
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ tasks, int c0, SymDev sd,
+                                                   T* __restrict__ L, const double* __restrict__ dscratch) {
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int w = min(NB, k - c0);
+  T* P = L + sd.Loff[s];
+  __shared__ double sA[NB][64 + 2];  // sA[p][r] = A(r0 + r, c0 + p)
+  __shared__ double sL[NB][64 + 2];  // sL[p][q] = Linv(q, p)
+  const long long r0 = tk.y;
+  const int nr = tk.z - tk.y;
+  const double* Lt = dscratch + (long long)tk.w * NB * NB;
+  for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
+    int p = idx >> 6, rr = idx & 63;
+    sA[p][rr] = (p < w && rr < nr) ? (double)P[(long long)(c0 + p) * m + r0 + rr] : 0.0;
+    // Lt is stored [c][r] = Linv[r][c]; sL[p][q] = Linv[q][p] = Lt[p][q]
+    sL[p][rr] = (p < w && rr < w) ? Lt[p * NB + rr] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int qb = (wv >> 1) * 32, rb = (wv & 1) * 32;
+  const int l16 = lane & 15;
+  typedef Mfma<double>::v4 v4;
+  v4 acc[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+  for (int kk = 0; kk < NB; kk += 4) {
+    const int p = kk + (lane >> 4);
+    double a0 = sL[p][qb + l16], a1 = sL[p][qb + 16 + l16];
+    double b0 = sA[p][rb + l16], b1 = sA[p][rb + 16 + l16];
+    acc[0][0] = Mfma<double>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<double>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<double>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<double>::op(a1, b1, acc[1][1]);
+  }
+  // D[q][r] = X(r0 + r, c0 + q)
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int q = qb + 16 * a + Mfma<double>::row(lane, r);
+        int rr = rb + 16 * b + l16;
+        if (q < w && rr < nr) P[(long long)(c0 + q) * m + r0 + rr] = (T)acc[a][b][r];
+      }
+}
+
 // Copies the factored diagonal blocks of one panel launch into their panels (tasks as k_panel; non-designated exit).
 template <typename T>
 __global__ __launch_bounds__(256) void k_commit(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
@@ -179,42 +288,52 @@ __global__ __launch_bounds__(256) void k_commit(const int4* __restrict__ tasks, 
   }
 }
 
-// task: (supernode s, row tile I, column tile J, 0), I >= J; tiles of 64 over the trailing part below the block.
+// Trailing update C -= X X^T of a front, 64x64 output tile per workgroup, MFMA (FP64 16x16x4 or FP32 16x16x4).
+// task: (supernode s, row tile I, column tile J, 0), I >= J.
+// mode 0 (inner): X = the 64 columns [kb, kb+w) just factored; C = columns [kb+w, min(k, outer_end)) of the same
+//                 outer block, all rows below (delayed right-looking update inside a 256-column outer block).
+// mode 1 (outer): X = the outer block [kb, kb+min(OB, k-kb)); C = every column to its right, including the update
+//                 block U, so the large trailing matrix is read and written once per 256 columns.
 template <typename T>
-__global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
-                                              T* __restrict__ U) {
+__global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, int kb, int mode, SymDev sd,
+                                              T* __restrict__ L, T* __restrict__ U) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  const int w = min(NB, k - c0);
-  const long long rb = c0 + w;
+  const int kw = mode == 0 ? min(NB, k - kb) : min(OB, k - kb);
+  const long long rb = kb + kw;
+  const long long col_end = mode == 0 ? min((long long)k, (long long)(kb / OB) * OB + OB) : m;
   const T* P = L + sd.Loff[s];
   __shared__ T Xi[NB][64 + 4];
   __shared__ T Xj[NB][64 + 4];
   const long long i0 = rb + 64LL * tk.y, j0 = rb + 64LL * tk.z;
-  for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
-    int q = idx >> 6, rr = idx & 63;
-    long long ri = i0 + rr, rj = j0 + rr;
-    const T* cp = P + (long long)(c0 + q) * m;
-    Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
-    Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ib = (wv >> 1) * 32, jb = (wv & 1) * 32;
+  const int l16 = lane & 15;
   typedef typename Mfma<T>::v4 v4;
   v4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
-  const int kq = (w + 3) & ~3;
-  const int l16 = lane & 15;
-  for (int kk = 0; kk < kq; kk += 4) {
-    const int q = kk + (lane >> 4);
-    T a0 = Xj[q][jb + l16], a1 = Xj[q][jb + 16 + l16];
-    T b0 = Xi[q][ib + l16], b1 = Xi[q][ib + 16 + l16];
-    acc00 = Mfma<T>::op(a0, b0, acc00);
-    acc01 = Mfma<T>::op(a0, b1, acc01);
-    acc10 = Mfma<T>::op(a1, b0, acc10);
-    acc11 = Mfma<T>::op(a1, b1, acc11);
+  for (int kc = 0; kc < kw; kc += NB) {
+    const int w = min(NB, kw - kc);
+    if (kc) __syncthreads();
+    for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
+      int q = idx >> 6, rr = idx & 63;
+      long long ri = i0 + rr, rj = j0 + rr;
+      const T* cp = P + (long long)(kb + kc + q) * m;
+      Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
+      Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
+    }
+    __syncthreads();
+    const int kq = (w + 3) & ~3;
+    for (int kk = 0; kk < kq; kk += 4) {
+      const int q = kk + (lane >> 4);
+      T a0 = Xj[q][jb + l16], a1 = Xj[q][jb + 16 + l16];
+      T b0 = Xi[q][ib + l16], b1 = Xi[q][ib + 16 + l16];
+      acc00 = Mfma<T>::op(a0, b0, acc00);
+      acc01 = Mfma<T>::op(a0, b1, acc01);
+      acc10 = Mfma<T>::op(a1, b0, acc10);
+      acc11 = Mfma<T>::op(a1, b1, acc11);
+    }
   }
   // D[a][b] = sum_q Xj[a][q] Xi[b][q] = contribution to C(i = i0+ib+16bi+b, j = j0+jb+16ai+a)
   v4 accs[2][2] = {{acc00, acc01}, {acc10, acc11}};
@@ -229,9 +348,78 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
       for (int r = 0; r < 4; r++) {
         long long j = j0 + jb + 16 * ai + Mfma<T>::row(lane, r);
         long long i = i0 + ib + 16 * bi + l16;
-        if (i < m && j < m && i >= j) {
+        if (i < m && j < col_end && i >= j) {
           T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
           *cp -= accs[ai][bi][r];
+        }
+      }
+}
+
+
+// Outer trailing update with 128x128 output tiles (mode 1 of k_syrk): 4 waves, each a 64x64 quadrant as 4x4 MFMA
+// blocks; X streamed through LDS in 32-column chunks. task: (supernode s, row tile I, column tile J, 0), I >= J.
+template <typename T>
+__global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
+                                                  T* __restrict__ L, T* __restrict__ U) {
+  constexpr int KC = 32;
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int kw = min(OB, k - kb);
+  const long long rb = kb + kw;
+  const T* P = L + sd.Loff[s];
+  __shared__ T Xi[KC][128 + 4];
+  __shared__ T Xj[KC][128 + 4];
+  const long long i0 = rb + 128LL * tk.y, j0 = rb + 128LL * tk.z;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ib = (wv >> 1) * 64, jb = (wv & 1) * 64;
+  const int l16 = lane & 15;
+  typedef typename Mfma<T>::v4 v4;
+  v4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4{0, 0, 0, 0};
+  for (int kc = 0; kc < kw; kc += KC) {
+    const int w = min(KC, kw - kc);
+    if (kc) __syncthreads();
+    for (int idx = threadIdx.x; idx < KC * 128; idx += blockDim.x) {
+      int q = idx >> 7, rr = idx & 127;
+      long long ri = i0 + rr, rj = j0 + rr;
+      const T* cp = P + (long long)(kb + kc + q) * m;
+      Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
+      Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
+    }
+    __syncthreads();
+    const int kq = (w + 3) & ~3;
+    for (int kk = 0; kk < kq; kk += 4) {
+      const int q = kk + (lane >> 4);
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) av[a] = Xj[q][jb + 16 * a + l16];
+#pragma unroll
+      for (int b = 0; b < 4; b++) bv[b] = Xi[q][ib + 16 * b + l16];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
+    }
+  }
+  T* Ls = L + sd.Loff[s];
+  T* Us = U + sd.Uoff[s];
+  const long long mu = m - k;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        long long j = j0 + jb + 16 * a + Mfma<T>::row(lane, r);
+        long long i = i0 + ib + 16 * b + l16;
+        if (i < m && j < m && i >= j) {
+          T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
+          *cp -= acc[a][b][r];
         }
       }
 }
@@ -479,6 +667,7 @@ __global__ void k_perm_out(const double* __restrict__ y, const int* __restrict__
 
 // ------------------------------------------------------------------------------------------------ host
 SpdDevice::~SpdDevice() {
+  drop_graph();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
 }
 
@@ -499,6 +688,7 @@ size_t SpdDevice::device_bytes() const {
 void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* perm, const SymbolicOptions& opt) {
   analyzed_ = false;
   factored_ = false;
+  drop_graph();
   int rc = symbolic_analyze(n, colPtr, rowIdx, perm, opt, S_);
   if (rc) throw Error(rc == -2 ? -2 : (rc == -1 ? -1 : -3), "symbolic analysis failed: " + S_.error);
   const Symbolic& S = S_;
@@ -520,7 +710,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   std::vector<std::vector<int>> bylevel(nl);
   for (int s = 0; s < ns; s++) bylevel[S.sn_level[s]].push_back(s);
   std::vector<int> lvl_sn;
-  std::vector<int4> ext, pan, syr, zer;
+  std::vector<int4> ext, pan, syr, zer, pot;
   int max_slots = 1;
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
@@ -562,16 +752,21 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       ll.end = (long long)ext.size();
       if (ll.end > ll.begin) L.ext_rounds.push_back(ll);
     }
-    // factor steps
+    // factor steps: 64-column panels; inner updates stay inside the 256-column outer block, the outer update
+    // of everything to its right follows the outer block's last panel
     int maxk = 0;
     for (int s : bylevel[l]) maxk = std::max(maxk, S.sn_first[s + 1] - S.sn_first[s]);
     for (int c0 = 0; c0 < maxk; c0 += NB) {
-      LaunchList lp, ls;
+      LaunchList lp, ls, lo, lpo;
       lp.begin = (long long)pan.size();
+      lpo.begin = (long long)pot.size();
+      lpo.c0 = c0;
       ls.begin = (long long)syr.size();
       lp.c0 = ls.c0 = c0;
-      double fl = 0;
+      double fl = 0, flo = 0;
       int nslot = 0;
+      const int ob0 = (c0 / OB) * OB;
+      const bool outer_now = (c0 + NB) % OB == 0 || c0 + NB >= maxk;
       for (int s : bylevel[l]) {
         int k = S.sn_first[s + 1] - S.sn_first[s];
         if (k <= c0) continue;
@@ -579,23 +774,50 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
         int w = std::min(NB, k - c0);
         long long rb = c0 + w;
         long long nrows = m - rb;
-        long long chunks = std::max<long long>(1, (nrows + PANEL_ROWS - 1) / PANEL_ROWS);
-        for (long long ci = 0; ci < chunks; ci++) {
-          long long r0 = rb + ci * PANEL_ROWS, r1 = std::min<long long>(m, r0 + PANEL_ROWS);
-          pan.push_back(make_int4(s, (int)r0, (int)std::max(r0, r1), ci == 0 ? 1 + nslot : 0));
-        }
+        pot.push_back(make_int4(s, nslot, 0, 0));
+        for (long long r0 = rb; r0 < m; r0 += 64)
+          pan.push_back(make_int4(s, (int)r0, (int)std::min<long long>(m, r0 + 64), nslot));
         nslot++;
-        fl += (double)nrows * (double)(nrows + 1) * (double)w;
-        long long tiles = (nrows + 63) / 64;
-        for (long long I = 0; I < tiles; I++)
-          for (long long J = 0; J <= I; J++) syr.push_back(make_int4(s, (int)I, (int)J, 0));
+        // inner update: columns [rb, min(k, ob0 + OB))
+        long long cend = std::min<long long>(k, ob0 + OB);
+        if (cend > rb) {
+          long long ti = (nrows + 63) / 64, tj = (cend - rb + 63) / 64;
+          for (long long I = 0; I < ti; I++)
+            for (long long J = 0; J <= std::min(I, tj - 1); J++) syr.push_back(make_int4(s, (int)I, (int)J, 0));
+          for (long long j = rb; j < cend; j++) fl += 2.0 * w * (double)(m - j);
+        }
       }
       lp.end = (long long)pan.size();
+      lpo.end = (long long)pot.size();
       ls.end = (long long)syr.size();
       L.panel_steps.push_back(lp);
+      L.potrf_steps.push_back(lpo);
       L.syrk_steps.push_back(ls);
       L.syrk_flops.push_back(fl);
       max_slots = std::max(max_slots, nslot);
+      lo.begin = (long long)syr.size();
+      lo.c0 = ob0;
+      if (outer_now) {
+        for (int s : bylevel[l]) {
+          int k = S.sn_first[s + 1] - S.sn_first[s];
+          if (k <= ob0) continue;
+          // the outer update of this front happens once, after its last panel inside [ob0, ob0 + OB)
+          int last_c0 = ((std::min(k, ob0 + OB) - 1) / NB) * NB;
+          if (last_c0 > c0) continue;
+          long long m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+          int kw = std::min(OB, k - ob0);
+          long long rb = ob0 + kw;
+          long long nrows = m - rb;
+          if (nrows <= 0) continue;
+          long long tiles = (nrows + 127) / 128;
+          for (long long I = 0; I < tiles; I++)
+            for (long long J = 0; J <= I; J++) syr.push_back(make_int4(s, (int)I, (int)J, 0));
+          flo += (double)kw * (double)nrows * (double)(nrows + 1);
+        }
+      }
+      lo.end = (long long)syr.size();
+      L.outer_steps.push_back(lo);
+      L.outer_flops.push_back(flo);
     }
   }
   // sweep schedule: narrow fronts (k <= BIG_K) one workgroup each, wide fronts in 128-column steps
@@ -604,7 +826,10 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
     std::vector<int> small, big;
-    for (int s : bylevel[l]) ((S.sn_first[s + 1] - S.sn_first[s]) > BIG_K ? big : small).push_back(s);
+    for (int s : bylevel[l]) {
+      long long k = S.sn_first[s + 1] - S.sn_first[s], m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+      (k > BIG_K || k * m > BIG_PANEL ? big : small).push_back(s);
+    }
     L.small_begin = (long long)swp.size();
     for (int s : small) {
       swp.push_back(s);
@@ -679,6 +904,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   if (pan.empty()) panel_tasks_.alloc(1); else panel_tasks_.upload(pan, stream_);
   if (syr.empty()) syrk_tasks_.alloc(1); else syrk_tasks_.upload(syr, stream_);
   if (zer.empty()) zero_tasks_.alloc(1); else zero_tasks_.upload(zer, stream_);
+  if (pot.empty()) potrf_tasks_.alloc(1); else potrf_tasks_.upload(pot, stream_);
   level_sn_.upload(lvl_sn, stream_);
   dscratch_.alloc((size_t)max_slots * NB * NB);
   V_.alloc(std::max<long long>(1, S.Vsize));
@@ -718,22 +944,28 @@ int SpdDevice::factorize_t(const double* d_vals) {
       hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, stream_,
                          ext_tasks_.p + r.begin, sd_, L, U);
     for (size_t t = 0; t < lv.panel_steps.size(); t++) {
+      const LaunchList& po = lv.potrf_steps[t];
+      if (po.end > po.begin)
+        hipLaunchKernelGGL(k_potrf_inv<T>, dim3((unsigned)(po.end - po.begin)), dim3(64), 0, stream_,
+                           potrf_tasks_.p + po.begin, po.c0, sd_, L, dscratch_.p, err_.p);
       const LaunchList& p = lv.panel_steps[t];
-      if (p.end > p.begin) {
-        hipLaunchKernelGGL(k_panel<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
-                           panel_tasks_.p + p.begin, p.c0, sd_, L, dscratch_.p, err_.p);
-        hipLaunchKernelGGL(k_commit<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
+      if (p.end > p.begin)
+        hipLaunchKernelGGL(k_trsm_mfma<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
                            panel_tasks_.p + p.begin, p.c0, sd_, L, dscratch_.p);
-      }
-      const LaunchList& q = lv.syrk_steps[t];
-      if (q.end > q.begin) {
+      for (int mode = 0; mode < 2; mode++) {
+        const LaunchList& q = mode == 0 ? lv.syrk_steps[t] : lv.outer_steps[t];
+        if (q.end <= q.begin) continue;
         if (profiling_) IFEA_HIP(hipEventRecord(event(2 * nev), stream_));
-        hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
-                           syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+        if (mode == 0)
+          hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
+                             syrk_tasks_.p + q.begin, q.c0, mode, sd_, L, U);
+        else
+          hipLaunchKernelGGL(k_syrk_big<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
+                             syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
         if (profiling_) {
           IFEA_HIP(hipEventRecord(event(2 * nev + 1), stream_));
           nev++;
-          prof_.syrk_flops += lv.syrk_flops[t];
+          prof_.syrk_flops += mode == 0 ? lv.syrk_flops[t] : lv.outer_flops[t];
         }
       }
     }
@@ -783,10 +1015,7 @@ int SpdDevice::factorize_device(const double* d_vals, int precision) {
 }
 
 template <typename T>
-void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
-  const int n = S_.n;
-  hipLaunchKernelGGL(k_perm_in, dim3(grid_for(n, 256)), dim3(256), 0, stream_, d_b, perm_.p, y_.p, n);
-  if (profiling_) IFEA_HIP(hipEventRecord(event(0), stream_));
+void SpdDevice::sweeps(const T* L) {
   const int cap = 8192;  // doubles of LDS per forward/backward work vector
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
@@ -823,6 +1052,31 @@ void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
                            bwd_tasks_.p + st.begin, st.c0, sd_, L, y_.p, W_.p);
     }
   }
+}
+
+void SpdDevice::drop_graph() {
+  if (sweep_exec_) (void)hipGraphExecDestroy(sweep_exec_);
+  sweep_exec_ = nullptr;
+  sweep_key_ = nullptr;
+}
+
+template <typename T>
+void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
+  const int n = S_.n;
+  hipLaunchKernelGGL(k_perm_in, dim3(grid_for(n, 256)), dim3(256), 0, stream_, d_b, perm_.p, y_.p, n);
+  if (profiling_) IFEA_HIP(hipEventRecord(event(0), stream_));
+  if (sweep_key_ != (const void*)L || sweep_tsize_ != sizeof(T) || !sweep_exec_) {
+    drop_graph();
+    hipGraph_t g = nullptr;
+    IFEA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    sweeps<T>(L);
+    IFEA_HIP(hipStreamEndCapture(stream_, &g));
+    IFEA_HIP(hipGraphInstantiate(&sweep_exec_, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    sweep_key_ = (const void*)L;
+    sweep_tsize_ = sizeof(T);
+  }
+  IFEA_HIP(hipGraphLaunch(sweep_exec_, stream_));
   if (profiling_) {
     IFEA_HIP(hipEventRecord(event(1), stream_));
     IFEA_HIP(hipEventSynchronize(event(1)));
