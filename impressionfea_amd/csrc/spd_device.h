@@ -23,6 +23,7 @@ struct SymDev {
   const int* rel;
   const int* child_ptr;
   const int* child_list;
+  const long long* Ioff;   // inverse diagonal blocks (64x64 per 64 columns) of each supernode
 };
 
 struct LaunchList {   // tasks of one launch: [begin, end) into a task array
@@ -79,7 +80,7 @@ class SpdDevice {
   double last_factor_ms_ = 0;
   // device symbolic
   DBuf<int> perm_, sn_first_, sn_rows_, rel_, child_ptr_, child_list_;
-  DBuf<long long> sn_rowptr_, Loff_, Uoff_, Voff_, Woff_, rel_off_;
+  DBuf<long long> sn_rowptr_, Loff_, Uoff_, Voff_, Woff_, rel_off_, Ioff_;
   DBuf<int> amap_src_;
   DBuf<long long> amap_dst_;
   // task arrays (int4: a, b, c, d)
@@ -113,7 +114,7 @@ class SpdDevice {
   const void* sweep_key_ = nullptr;
   size_t sweep_tsize_ = 0;
   void drop_graph();
-  DBuf<double> V_, W_, y_, dscratch_;
+  DBuf<double> V_, W_, y_, dscratch_, Linv_;
   DBuf<int> err_;
   SymDev sd_{};
   bool profiling_ = false;

@@ -10,6 +10,7 @@
 // backward sweep, in FP64 whatever the factor precision.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "spd_device.h"
@@ -23,7 +24,7 @@ constexpr int EXT_TARGET = 8192;
 constexpr long long ZERO_CHUNK = 1 << 16;
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
-constexpr int SB = 128;         // diagonal block of the blocked sweeps
+constexpr int SB = 64;          // diagonal block of the blocked sweeps (= the factor's panel width)
 constexpr int SWEEP_ROWS = 256; // rows (forward) per workgroup of a blocked sweep step
 constexpr int SWEEP_COLS = 64;  // columns (backward) per workgroup of a blocked sweep step
 
@@ -165,11 +166,11 @@ __global__ __launch_bounds__(256) void k_panel(const int4* __restrict__ tasks, i
 
 
 // Diagonal block of one 64-column step: FP64 Cholesky in one wave (lane i owns row i, columns broadcast through LDS),
-// then the inverse of the factor (lane c solves L x = e_c). The factor goes into the panel; L11^{-T} goes to the
-// scratch slot for the TRSM launch that follows. task: (supernode s, slot, 0, 0)
+// then the inverse of the factor (lane c solves L x = e_c). The factor goes into the panel; the inverse is kept
+// (Lt[c][r] = Linv[r][c]) for the TRSM launch that follows and for the triangular sweeps. task: (supernode s, 0, 0, 0)
 template <typename T>
 __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks, int c0, SymDev sd,
-                                                  T* __restrict__ L, double* __restrict__ dscratch,
+                                                  T* __restrict__ L, double* __restrict__ Linv,
                                                   int* __restrict__ err) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
     if (i < w && i >= q) P[(long long)(c0 + q) * m + c0 + i] = (T)D[i][q];
   if (bad && i == 0) atomicOr(err, 1);
   // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; store Lt[p][q] = Linv[q][p]
-  double* Lt = dscratch + (long long)tk.y * NB * NB;
+  double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
   const int c = i;
   double x[NB];
 #pragma unroll
@@ -221,10 +222,10 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
 }
 
 // Rows below the diagonal block: X = A L11^{-T} as a 64x64x64 FP64 MFMA product per task, written back into the
-// panel. task: (supernode s, first row, end row (<= first + 64), slot)
+// panel. task: (supernode s, first row, end row (<= first + 64), 0)
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ tasks, int c0, SymDev sd,
-                                                   T* __restrict__ L, const double* __restrict__ dscratch) {
+                                                   T* __restrict__ L, const double* __restrict__ Linv) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ task
   __shared__ double sL[NB][64 + 2];  // sL[p][q] = Linv(q, p)
   const long long r0 = tk.y;
   const int nr = tk.z - tk.y;
-  const double* Lt = dscratch + (long long)tk.w * NB * NB;
+  const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
   for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
     int p = idx >> 6, rr = idx & 63;
     sA[p][rr] = (p < w && rr < nr) ? (double)P[(long long)(c0 + p) * m + r0 + rr] : 0.0;
@@ -357,11 +358,14 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 
 
 // Outer trailing update with 128x128 output tiles (mode 1 of k_syrk): 4 waves, each a 64x64 quadrant as 4x4 MFMA
-// blocks; X streamed through LDS in 32-column chunks. task: (supernode s, row tile I, column tile J, 0), I >= J.
+// blocks. X is streamed in 16-column chunks through a double-buffered LDS tile; the next chunk's global loads are in
+// flight (in registers) while the current chunk feeds the MFMAs, one barrier per chunk.
+// task: (supernode s, row tile I, column tile J, 0), I >= J.
 template <typename T>
 __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
                                                   T* __restrict__ L, T* __restrict__ U) {
-  constexpr int KC = 32;
+  constexpr int KC = 16;
+  constexpr int PER = KC * 128 / 256;  // elements of each operand a thread stages per chunk
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
@@ -369,42 +373,62 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
   const int kw = min(OB, k - kb);
   const long long rb = kb + kw;
   const T* P = L + sd.Loff[s];
-  __shared__ T Xi[KC][128 + 4];
-  __shared__ T Xj[KC][128 + 4];
+  __shared__ T Xi[2][KC][128 + 4];
+  __shared__ T Xj[2][KC][128 + 4];
   const long long i0 = rb + 128LL * tk.y, j0 = rb + 128LL * tk.z;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ib = (wv >> 1) * 64, jb = (wv & 1) * 64;
   const int l16 = lane & 15;
+  const int nch = (kw + KC - 1) / KC;
+  T ri[PER], rj[PER];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+      int idx = threadIdx.x + 256 * t;
+      int q = idx >> 7, rr = idx & 127;
+      int col = c * KC + q;
+      const T* cp = P + (long long)(kb + col) * m;
+      long long ra = i0 + rr, rc = j0 + rr;
+      ri[t] = (col < kw && ra < m) ? cp[ra] : T(0);
+      rj[t] = (col < kw && rc < m) ? cp[rc] : T(0);
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+      int idx = threadIdx.x + 256 * t;
+      int q = idx >> 7, rr = idx & 127;
+      Xi[buf][q][rr] = ri[t];
+      Xj[buf][q][rr] = rj[t];
+    }
+  };
   typedef typename Mfma<T>::v4 v4;
   v4 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = v4{0, 0, 0, 0};
-  for (int kc = 0; kc < kw; kc += KC) {
-    const int w = min(KC, kw - kc);
-    if (kc) __syncthreads();
-    for (int idx = threadIdx.x; idx < KC * 128; idx += blockDim.x) {
-      int q = idx >> 7, rr = idx & 127;
-      long long ri = i0 + rr, rj = j0 + rr;
-      const T* cp = P + (long long)(kb + kc + q) * m;
-      Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
-      Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
-    }
-    __syncthreads();
-    const int kq = (w + 3) & ~3;
-    for (int kk = 0; kk < kq; kk += 4) {
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    const int buf = c & 1;
+    if (c + 1 < nch) fetch(c + 1);
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
       const int q = kk + (lane >> 4);
       T av[4], bv[4];
 #pragma unroll
-      for (int a = 0; a < 4; a++) av[a] = Xj[q][jb + 16 * a + l16];
+      for (int a = 0; a < 4; a++) av[a] = Xj[buf][q][jb + 16 * a + l16];
 #pragma unroll
-      for (int b = 0; b < 4; b++) bv[b] = Xi[q][ib + 16 * b + l16];
+      for (int b = 0; b < 4; b++) bv[b] = Xi[buf][q][ib + 16 * b + l16];
 #pragma unroll
       for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
     }
+    if (c + 1 < nch) stage(buf ^ 1);
+    __syncthreads();
   }
   T* Ls = L + sd.Loff[s];
   T* Us = U + sd.Uoff[s];
@@ -507,58 +531,46 @@ __global__ __launch_bounds__(256) void k_fwdb_gather(const int* __restrict__ sns
   }
 }
 
-// One 128-column step of the forward sweep. task: (s, first row, end row, writer). Every workgroup solves the diagonal
-// block in one wave (lane i: rows i and 64+i) from W_s (not modified by this launch), then updates its own rows >= c0+w.
+// One 64-column step of the forward sweep. task: (s, first row, end row, writer). x_t = Linv_t w_t from the kept
+// inverse diagonal block (a parallel 64x64 product, no sequential chain), then W(rows) -= L(rows, t) x_t for this
+// task's rows >= c0 + w. W(c0 .. c0+w) is read by every task and written by none.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwdb_step(const int4* __restrict__ tasks, int c0, SymDev sd,
-                                                   const T* __restrict__ L, double* __restrict__ y,
-                                                   double* __restrict__ Wg) {
-  __shared__ double xs[SB];
+                                                   const T* __restrict__ L, const double* __restrict__ Linv,
+                                                   double* __restrict__ y, double* __restrict__ Wg) {
+  __shared__ double ws[NB], xs[NB];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  const int w = min(SB, k - c0);
+  const int w = min(NB, k - c0);
   const T* P = L + sd.Loff[s];
+  const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
   double* W = Wg + sd.Woff[s];
-  if (threadIdx.x < 64) {
-    const int i = threadIdx.x;
-    double r0 = i < w ? W[c0 + i] : 0.0, r1 = 64 + i < w ? W[c0 + 64 + i] : 0.0;
-    for (int q = 0; q < w; q++) {
-      const T* cq = P + (long long)(c0 + q) * m + c0;
-      double l0 = i < w ? (double)cq[i] : 0.0, l1 = 64 + i < w ? (double)cq[64 + i] : 0.0;
-      double xq, dq;
-      if (q < 64) {
-        xq = __shfl(r0, q, 64);
-        dq = __shfl(l0, q, 64);
-      } else {
-        xq = __shfl(r1, q - 64, 64);
-        dq = __shfl(l1, q - 64, 64);
-      }
-      xq /= dq;
-      if (q < 64) {
-        if (i == q) r0 = xq;
-        else if (i > q) r0 -= l0 * xq;
-        r1 -= l1 * xq;
-      } else {
-        if (64 + i == q) r1 = xq;
-        else if (64 + i > q) r1 -= l1 * xq;
-      }
-    }
-    xs[i] = r0;
-    xs[64 + i] = r1;
-    if (tk.w) {
-      if (i < w) y[f + c0 + i] = r0;
-      if (64 + i < w) y[f + c0 + 64 + i] = r1;
-    }
+  if (threadIdx.x < NB) ws[threadIdx.x] = threadIdx.x < w ? W[c0 + threadIdx.x] : 0.0;
+  __syncthreads();
+  if (threadIdx.x < NB) {
+    const int r = threadIdx.x;
+    double x = 0.0;
+    for (int c = 0; c < w; c++) x += Lt[c * NB + r] * ws[c];
+    xs[r] = r < w ? x : 0.0;
+    if (tk.w && r < w) y[f + c0 + r] = x;
   }
   __syncthreads();
   const long long row = (long long)tk.y + threadIdx.x;
   if (row < tk.z) {
-    double acc = 0.0;
-    for (int q = 0; q < w; q++) acc += (double)P[(long long)(c0 + q) * m + row] * xs[q];
-    W[row] -= acc;
+    const T* pr = P + (long long)c0 * m + row;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int q = 0;
+    for (; q + 4 <= w; q += 4) {
+      a0 += (double)pr[(long long)q * m] * xs[q];
+      a1 += (double)pr[(long long)(q + 1) * m] * xs[q + 1];
+      a2 += (double)pr[(long long)(q + 2) * m] * xs[q + 2];
+      a3 += (double)pr[(long long)(q + 3) * m] * xs[q + 3];
+    }
+    for (; q < w; q++) a0 += (double)pr[(long long)q * m] * xs[q];
+    W[row] -= (a0 + a1) + (a2 + a3);
   }
 }
 
@@ -573,79 +585,83 @@ __global__ __launch_bounds__(256) void k_fwdb_finish(const int* __restrict__ sns
   for (long long i = k + threadIdx.x; i < m; i += blockDim.x) vs[i - k] = w[i];
 }
 
-// Backward, first part: z_q = y_own[q] - L21(:, q)^T x_rows for every own column q (wave per column).
+// Backward, first part: the solution at a wide front's lower rows, gathered into one contiguous vector (the front's
+// forward update-vector slot, free once the forward sweep is done).
+__global__ __launch_bounds__(256) void k_bwdb_xb(const int* __restrict__ sns, SymDev sd, const double* __restrict__ y,
+                                                 double* __restrict__ V) {
+  const int s = sns[blockIdx.x];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int* rows = sd.sn_rows + sd.sn_rowptr[s];
+  double* xb = V + sd.Voff[s];
+  for (long long i = k + threadIdx.x; i < m; i += blockDim.x) xb[i - k] = y[rows[i]];
+}
+
+// z_q = y_own[q] - L21(:, q)^T xb for every own column q: one wave per column, 4 independent partial sums per lane.
 // task: (s, first column, end column, 0)
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
-                                                   const double* __restrict__ y, double* __restrict__ Wg) {
+                                                   const double* __restrict__ y, const double* __restrict__ V,
+                                                   double* __restrict__ Wg) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  const int* rows = sd.sn_rows + sd.sn_rowptr[s];
+  const long long mb = m - k;
   const T* P = L + sd.Loff[s];
+  const double* xb = V + sd.Voff[s];
   double* z = Wg + sd.Woff[s];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int q = tk.y + wv; q < tk.z; q += 4) {
-    const T* cq = P + (long long)q * m;
-    double acc = 0.0;
-    for (long long i = k + lane; i < m; i += 64) acc += (double)cq[i] * y[rows[i]];
-    acc = wave_sum(acc);
+    const T* cq = P + (long long)q * m + k;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    long long i = lane;
+    for (; i + 192 < mb; i += 256) {
+      a0 += (double)cq[i] * xb[i];
+      a1 += (double)cq[i + 64] * xb[i + 64];
+      a2 += (double)cq[i + 128] * xb[i + 128];
+      a3 += (double)cq[i + 192] * xb[i + 192];
+    }
+    for (; i < mb; i += 64) a0 += (double)cq[i] * xb[i];
+    double acc = wave_sum((a0 + a1) + (a2 + a3));
     if (lane == 0) z[q] = y[f + q] - acc;
   }
 }
 
-// One 128-column step (descending) of the backward sweep. task: (s, first column, end column, writer); columns < c0.
-// Every workgroup solves L_tt^T x_t = z_t in one wave from an LDS copy of the block, then updates its own columns
-// j < c0: z_j -= L(c0.., j)^T x_t (wave per column).
+// One 64-column step (descending) of the backward sweep. task: (s, first column, end column, writer); columns < c0.
+// x_t = Linv_t^T z_t from the kept inverse block, then z_j -= L(c0.., j)^T x_t for this task's columns (wave per
+// column).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwdb_step(const int4* __restrict__ tasks, int c0, SymDev sd,
-                                                   const T* __restrict__ L, double* __restrict__ y,
-                                                   double* __restrict__ Wg) {
-  __shared__ double Dt[SB][SB + 1];  // Dt[c][r] = L(c0 + r, c0 + c)
-  __shared__ double xs[SB];
+                                                   const T* __restrict__ L, const double* __restrict__ Linv,
+                                                   double* __restrict__ y, double* __restrict__ Wg) {
+  __shared__ double zs[NB], xs[NB];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  const int w = min(SB, k - c0);
+  const int w = min(NB, k - c0);
   const T* P = L + sd.Loff[s];
+  const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
   double* z = Wg + sd.Woff[s];
-  for (int idx = threadIdx.x; idx < SB * SB; idx += blockDim.x) {
-    int c = idx / SB, r = idx % SB;
-    Dt[c][r] = (c < w && r < w && r >= c) ? (double)P[(long long)(c0 + c) * m + c0 + r] : (r == c ? 1.0 : 0.0);
-  }
+  if (threadIdx.x < NB) zs[threadIdx.x] = threadIdx.x < w ? z[c0 + threadIdx.x] : 0.0;
   __syncthreads();
-  if (threadIdx.x < 64) {
-    const int i = threadIdx.x;
-    double r0 = i < w ? z[c0 + i] : 0.0, r1 = 64 + i < w ? z[c0 + 64 + i] : 0.0;
-    for (int q = w - 1; q >= 0; q--) {
-      // x_q = z_q / L_qq; then z_p -= L(q, p) x_q for p < q, L(q, p) = Dt[p][q]
-      double xq = (q < 64 ? __shfl(r0, q, 64) : __shfl(r1, q - 64, 64)) / Dt[q][q];
-      if (q < 64) {
-        if (i == q) r0 = xq;
-        else if (i < q) r0 -= Dt[i][q] * xq;
-      } else {
-        if (64 + i == q) r1 = xq;
-        else if (64 + i < q) r1 -= Dt[64 + i][q] * xq;
-        r0 -= Dt[i][q] * xq;
-      }
-    }
-    xs[i] = r0;
-    xs[64 + i] = r1;
-    if (tk.w) {
-      if (i < w) y[f + c0 + i] = r0;
-      if (64 + i < w) y[f + c0 + 64 + i] = r1;
-    }
+  if (threadIdx.x < NB) {
+    // x_c = sum_r Linv[r][c] z_r = sum_r Lt[c][r] z_r
+    const int c = threadIdx.x;
+    const double* lc = Lt + c * NB;
+    double x = 0.0;
+    for (int r = 0; r < w; r++) x += lc[r] * zs[r];
+    xs[c] = c < w ? x : 0.0;
+    if (tk.w && c < w) y[f + c0 + c] = x;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int j = tk.y + wv; j < tk.z; j += 4) {
     const T* cj = P + (long long)j * m + c0;
-    double acc = 0.0;
-    for (int i = lane; i < w; i += 64) acc += (double)cj[i] * xs[i];
+    double acc = lane < w ? (double)cj[lane] * xs[lane] : 0.0;
     acc = wave_sum(acc);
     if (lane == 0) z[j] -= acc;
   }
@@ -702,8 +718,11 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
     std::vector<int> fill(cptr.begin(), cptr.end() - 1);
     for (int s = 0; s < ns; s++) if (S.sn_parent[s] >= 0) clist[fill[S.sn_parent[s]]++] = s;
   }
-  std::vector<long long> Woff(ns + 1, 0);
-  for (int s = 0; s < ns; s++) Woff[s + 1] = Woff[s] + (S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+  std::vector<long long> Woff(ns + 1, 0), Ioff(ns + 1, 0);
+  for (int s = 0; s < ns; s++) {
+    Woff[s + 1] = Woff[s] + (S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+    Ioff[s + 1] = Ioff[s] + (long long)((S.sn_first[s + 1] - S.sn_first[s] + NB - 1) / NB) * NB * NB;
+  }
   // level schedule and task lists
   int nl = S.max_level + 1;
   levels_.assign(nl, Level());
@@ -862,7 +881,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
     L.bwd_gemv.begin = (long long)bwt.size();
     for (int s : big) {
       int k = S.sn_first[s + 1] - S.sn_first[s];
-      for (int q = 0; q < k; q += SWEEP_COLS) bwt.push_back(make_int4(s, q, std::min(k, q + SWEEP_COLS), 0));
+      for (int q = 0; q < k; q += 16) bwt.push_back(make_int4(s, q, std::min(k, q + 16), 0));
     }
     L.bwd_gemv.end = (long long)bwt.size();
     for (int c0 = ((maxk - 1) / SB) * SB; maxk > 0 && c0 >= 0; c0 -= SB) {
@@ -894,6 +913,8 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   Uoff_.upload(S.Uoff, stream_);
   Voff_.upload(S.Voff, stream_);
   Woff_.upload(Woff, stream_);
+  Ioff_.upload(Ioff, stream_);
+  Linv_.alloc(std::max<long long>(1, Ioff[ns]));
   rel_off_.upload(S.rel_off, stream_);
   if (S.rel.empty()) rel_.alloc(1); else rel_.upload(S.rel, stream_);
   child_ptr_.upload(cptr, stream_);
@@ -912,7 +933,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   y_.alloc(n);
   err_.alloc(1);
   sd_ = SymDev{sn_first_.p, sn_rowptr_.p, sn_rows_.p, Loff_.p, Uoff_.p, Voff_.p, Woff_.p, rel_off_.p, rel_.p,
-               child_ptr_.p, child_list_.p};
+               child_ptr_.p, child_list_.p, Ioff_.p};
   IFEA_HIP(hipStreamSynchronize(stream_));
   // host copies not needed after upload
   std::vector<int>().swap(S_.amap_src);
@@ -947,11 +968,11 @@ int SpdDevice::factorize_t(const double* d_vals) {
       const LaunchList& po = lv.potrf_steps[t];
       if (po.end > po.begin)
         hipLaunchKernelGGL(k_potrf_inv<T>, dim3((unsigned)(po.end - po.begin)), dim3(64), 0, stream_,
-                           potrf_tasks_.p + po.begin, po.c0, sd_, L, dscratch_.p, err_.p);
+                           potrf_tasks_.p + po.begin, po.c0, sd_, L, Linv_.p, err_.p);
       const LaunchList& p = lv.panel_steps[t];
       if (p.end > p.begin)
         hipLaunchKernelGGL(k_trsm_mfma<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
-                           panel_tasks_.p + p.begin, p.c0, sd_, L, dscratch_.p);
+                           panel_tasks_.p + p.begin, p.c0, sd_, L, Linv_.p);
       for (int mode = 0; mode < 2; mode++) {
         const LaunchList& q = mode == 0 ? lv.syrk_steps[t] : lv.outer_steps[t];
         if (q.end <= q.begin) continue;
@@ -1031,7 +1052,7 @@ void SpdDevice::sweeps(const T* L) {
                          V_.p, W_.p);
       for (const LaunchList& st : lv.fwd_steps)
         hipLaunchKernelGGL(k_fwdb_step<T>, dim3((unsigned)(st.end - st.begin)), dim3(256), 0, stream_,
-                           fwd_tasks_.p + st.begin, st.c0, sd_, L, y_.p, W_.p);
+                           fwd_tasks_.p + st.begin, st.c0, sd_, L, Linv_.p, y_.p, W_.p);
       hipLaunchKernelGGL(k_fwdb_finish, dim3(nbig), dim3(256), 0, stream_, sweep_sn_.p + lv.big_begin, sd_, W_.p,
                          V_.p);
     }
@@ -1045,11 +1066,13 @@ void SpdDevice::sweeps(const T* L) {
                          W_.p, cap);
     }
     if (lv.big_end > lv.big_begin) {
+      hipLaunchKernelGGL(k_bwdb_xb, dim3((unsigned)(lv.big_end - lv.big_begin)), dim3(256), 0, stream_,
+                         sweep_sn_.p + lv.big_begin, sd_, y_.p, V_.p);
       hipLaunchKernelGGL(k_bwdb_gemv<T>, dim3((unsigned)(lv.bwd_gemv.end - lv.bwd_gemv.begin)), dim3(256), 0,
-                         stream_, bwd_tasks_.p + lv.bwd_gemv.begin, sd_, L, y_.p, W_.p);
+                         stream_, bwd_tasks_.p + lv.bwd_gemv.begin, sd_, L, y_.p, V_.p, W_.p);
       for (const LaunchList& st : lv.bwd_steps)
         hipLaunchKernelGGL(k_bwdb_step<T>, dim3((unsigned)(st.end - st.begin)), dim3(256), 0, stream_,
-                           bwd_tasks_.p + st.begin, st.c0, sd_, L, y_.p, W_.p);
+                           bwd_tasks_.p + st.begin, st.c0, sd_, L, Linv_.p, y_.p, W_.p);
     }
   }
 }
@@ -1065,7 +1088,13 @@ void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
   const int n = S_.n;
   hipLaunchKernelGGL(k_perm_in, dim3(grid_for(n, 256)), dim3(256), 0, stream_, d_b, perm_.p, y_.p, n);
   if (profiling_) IFEA_HIP(hipEventRecord(event(0), stream_));
-  if (sweep_key_ != (const void*)L || sweep_tsize_ != sizeof(T) || !sweep_exec_) {
+  static const bool use_graph = [] {
+    const char* e = std::getenv("IFEA_SWEEP_GRAPH");
+    return !(e && e[0] == '0');
+  }();
+  if (!use_graph) {
+    sweeps<T>(L);
+  } else if (sweep_key_ != (const void*)L || sweep_tsize_ != sizeof(T) || !sweep_exec_) {
     drop_graph();
     hipGraph_t g = nullptr;
     IFEA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
@@ -1076,7 +1105,7 @@ void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
     sweep_key_ = (const void*)L;
     sweep_tsize_ = sizeof(T);
   }
-  IFEA_HIP(hipGraphLaunch(sweep_exec_, stream_));
+  if (use_graph) IFEA_HIP(hipGraphLaunch(sweep_exec_, stream_));
   if (profiling_) {
     IFEA_HIP(hipEventRecord(event(1), stream_));
     IFEA_HIP(hipEventSynchronize(event(1)));
