@@ -352,9 +352,9 @@ int ifea_step_set_v(ifea_step_t h, const double* v) {
   IFEA_GUARD(h->dev->set_v(v))
 }
 
-int ifea_step_info(ifea_step_t h, int reset, double* out16) {
-  if (!h || !out16) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
-  IFEA_GUARD(h->dev->info(reset, out16))
+int ifea_step_info(ifea_step_t h, int reset, double* out24) {
+  if (!h || !out24) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->info(reset, out24))
 }
 
 void ifea_step_destroy(ifea_step_t h) { delete h; }

@@ -24,6 +24,10 @@ struct SymDev {
   const int* child_ptr;
   const int* child_list;
   const long long* Ioff;   // inverse diagonal blocks (64x64 per 64 columns) of each supernode
+  const long long* Xoff;   // wide fronts: inverse diagonal blocks of the sweeps (256x256 per 256 columns)
+  const long long* Goff;   // wide fronts: gptr + Goff[s] has m + 1 entries into gsrc
+  const long long* gptr;   // row-wise child contributions (indices into the update vectors V), in child order
+  const long long* gsrc;
 };
 
 struct LaunchList {   // tasks of one launch: [begin, end) into a task array
@@ -59,7 +63,11 @@ class SpdDevice {
   // Optional per-launch event timing of the dominant kernels (adds one event pair per launch).
   struct Profile {
     long long syrk_launches = 0, solve_launches = 0, factorizations = 0, solves = 0;
-    double syrk_ms = 0, syrk_flops = 0;     // algorithmic flops: sum over fronts of trailing^2 * width
+    long long outer_launches = 0, f32_factorizations = 0, f32_solves = 0;
+    double syrk_ms = 0, syrk_flops = 0;     // inner (64-column) updates; algorithmic flops 2 * width per entry
+    double outer_ms = 0, outer_flops = 0;   // outer (256-column) updates, k_syrk_big
+    long long outer_launches_f32 = 0;       // the part of the outer updates done on an FP32 factor
+    double outer_ms_f32 = 0, outer_flops_f32 = 0;
     double solve_ms = 0, solve_bytes = 0;   // one forward + backward sweep reads every panel once
     double factor_ms = 0, factor_flops = 0;
   };
@@ -86,7 +94,9 @@ class SpdDevice {
   // task arrays (int4: a, b, c, d)
   DBuf<int4> ext_tasks_, panel_tasks_, syrk_tasks_, zero_tasks_, potrf_tasks_;
   DBuf<int> level_sn_, sweep_sn_;
-  DBuf<int4> fwd_tasks_, bwd_tasks_;
+  DBuf<int4> fwd_tasks_, bwd_tasks_, inv_tasks_;
+  DBuf<long long> Xoff_, Goff_, gptr_, gsrc_;
+  LaunchList inv_launch_;
   // launch schedule
   struct Level {
     std::vector<LaunchList> ext_rounds;
@@ -99,8 +109,8 @@ class SpdDevice {
     // triangular sweeps: narrow fronts one workgroup each; wide fronts blocked over several launches
     long long small_begin = 0, small_end = 0, big_begin = 0, big_end = 0;  // into sweep_sn_
     long long small_max_m = 0;
-    std::vector<LaunchList> fwd_steps, bwd_steps;  // bwd_steps in launch (descending) order
-    LaunchList bwd_gemv;
+    LaunchList fwd_gather, bwd_xb, bwd_gemv;
+    std::vector<LaunchList> fwd_diag, fwd_panel, bwd_diag, bwd_panel;  // per SB-column step (backward descending)
     double solve_bytes = 0;              // panel bytes per element of storage (times sizeof(T) at run time)
     long long max_m = 0;
   };
@@ -114,12 +124,13 @@ class SpdDevice {
   const void* sweep_key_ = nullptr;
   size_t sweep_tsize_ = 0;
   void drop_graph();
-  DBuf<double> V_, W_, y_, dscratch_, Linv_;
+  DBuf<double> V_, W_, y_, Linv_, X_;
   DBuf<int> err_;
   SymDev sd_{};
   bool profiling_ = false;
   Profile prof_;
   std::vector<hipEvent_t> ev_pool_;
+  std::vector<char> ev_mode_;
   hipEvent_t event(size_t i);
 };
 

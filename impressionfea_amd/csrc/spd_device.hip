@@ -19,14 +19,12 @@ namespace ifea {
 
 constexpr int NB = 64;          // panel width
 constexpr int OB = 256;         // outer block: the trailing matrix beyond it is updated once per OB columns
-constexpr int PANEL_ROWS = 256; // rows per panel task (one per thread)
 constexpr int EXT_TARGET = 8192;
-constexpr long long ZERO_CHUNK = 1 << 16;
+constexpr long long ZERO_TARGET = 1 << 16;  // update-block entries zeroed per workgroup
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
-constexpr int SB = 64;          // diagonal block of the blocked sweeps (= the factor's panel width)
-constexpr int SWEEP_ROWS = 256; // rows (forward) per workgroup of a blocked sweep step
-constexpr int SWEEP_COLS = 64;  // columns (backward) per workgroup of a blocked sweep step
+constexpr int SB = 256;         // diagonal block of the blocked sweeps (explicit inverse per block)
+constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step / of the backward GEMV
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -68,16 +66,18 @@ __global__ void k_scatter(const double* __restrict__ vals, const int* __restrict
   if (k < cnt) L[dst[k]] = (T)vals[src[k]];
 }
 
-// task: (supernode s, chunk index): zero one chunk of the supernode's update block before its level starts
+// task: (supernode s, first column, end column, 0): zero the lower triangle (rows >= column) of those columns of the
+// supernode's update block before its level starts; nothing reads the strict upper triangle of an update block.
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero_u(const int4* __restrict__ tasks, SymDev sd, T* __restrict__ U) {
-  int4 tk = tasks[blockIdx.x];
-  int s = tk.x;
-  long long mu = sn_m(sd, s) - sn_k(sd, s);
-  long long size = mu * mu;
-  long long b = (long long)tk.y * ZERO_CHUNK, e = min(size, b + ZERO_CHUNK);
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const long long mu = sn_m(sd, s) - sn_k(sd, s);
   T* u = U + sd.Uoff[s];
-  for (long long i = b + threadIdx.x; i < e; i += blockDim.x) u[i] = T(0);
+  for (int j = tk.y; j < tk.z; j++) {
+    T* col = u + (long long)j * mu;
+    for (long long i = j + threadIdx.x; i < mu; i += blockDim.x) col[i] = T(0);
+  }
 }
 
 // task: (child c, first column, end column, parent p) of the child's update block
@@ -103,68 +103,6 @@ __global__ __launch_bounds__(256) void k_extend(const int4* __restrict__ tasks, 
   }
 }
 
-// task: (supernode s, first row, end row, 1 + slot of the factored diagonal block in the scratch, or 0).
-// c0 = first column of the 64-wide block. Every task of a front re-factors the (unmodified) diagonal block in LDS;
-// the designated task stores the factor to the scratch, and k_commit copies it into the panel after the launch, so
-// no task can read a block that another task of the same launch has already overwritten.
-template <typename T>
-__global__ __launch_bounds__(256) void k_panel(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
-                                               double* __restrict__ dscratch, int* __restrict__ err) {
-  const int4 tk = tasks[blockIdx.x];
-  const int s = tk.x;
-  const int k = sn_k(sd, s);
-  const long long m = sn_m(sd, s);
-  const int w = min(NB, k - c0);
-  T* P = L + sd.Loff[s];
-  __shared__ double D[NB][NB + 1];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
-    int j = idx / NB, i = idx % NB;
-    double v;
-    if (i < w && j < w) v = (i >= j) ? (double)P[(long long)(c0 + j) * m + c0 + i] : 0.0;
-    else v = (i == j) ? 1.0 : 0.0;
-    D[i][j] = v;
-  }
-  __syncthreads();
-  if (tid < 64) {
-    // one wave factors the block in LDS: lane i updates row i; column j is read back by broadcast
-    const int i = tid;
-    bool bad = false;
-    for (int j = 0; j < NB; j++) {
-      double piv = D[j][j];
-      bad |= !(piv > 0.0) || !isfinite(piv);
-      double d = sqrt(piv > 0.0 ? piv : 1.0);
-      double lij = (i > j) ? D[i][j] / d : 0.0;
-      wave_sync_lds();
-      if (i == j) D[i][j] = d;
-      else if (i > j) D[i][j] = lij;
-      wave_sync_lds();
-      for (int q = j + 1; q <= i; q++) D[i][q] -= lij * D[q][j];
-      wave_sync_lds();
-    }
-    for (int q = i + 1; q < NB; q++) D[i][q] = 0.0;
-    if (tk.w) {
-      double* dsc = dscratch + (long long)(tk.w - 1) * NB * NB;
-      for (int q = 0; q < NB; q++) dsc[q * NB + i] = D[i][q];
-      if (bad && i == 0) atomicOr(err, 1);
-    }
-  }
-  __syncthreads();
-  // rows below the block: X = A L11^{-T}, one row per thread
-  const long long row = (long long)tk.y + tid;
-  const bool valid = row < tk.z;
-  double x[NB];
-#pragma unroll
-  for (int q = 0; q < NB; q++) {
-    double a = (q < w && valid) ? (double)P[(long long)(c0 + q) * m + row] : 0.0;
-#pragma unroll
-    for (int p = 0; p < q; p++) a -= x[p] * D[q][p];
-    x[q] = a / D[q][q];
-    if (q < w && valid) P[(long long)(c0 + q) * m + row] = (T)x[q];
-  }
-}
-
-
 // Diagonal block of one 64-column step: FP64 Cholesky in one wave (lane i owns row i, columns broadcast through LDS),
 // then the inverse of the factor (lane c solves L x = e_c). The factor goes into the panel; the inverse is kept
 // (Lt[c][r] = Linv[r][c]) for the TRSM launch that follows and for the triangular sweeps. task: (supernode s, 0, 0, 0)
@@ -180,11 +118,19 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   T* P = L + sd.Loff[s];
   __shared__ double D[NB][NB + 1];
   const int i = threadIdx.x;
-  for (int j = 0; j < NB; j++) {
-    double v;
-    if (i < w && j < w) v = (i >= j) ? (double)P[(long long)(c0 + j) * m + c0 + i] : 0.0;
-    else v = (i == j) ? 1.0 : 0.0;
-    D[i][j] = v;
+  // the 64 column loads of the block are independent: issue them 16 at a time
+  for (int j0 = 0; j0 < NB; j0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      int j = j0 + t;
+      v[t] = (i < w && j < w && i >= j) ? (double)P[(long long)(c0 + j) * m + c0 + i] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      int j = j0 + t;
+      D[i][j] = (i < w && j < w) ? v[t] : ((i == j) ? 1.0 : 0.0);
+    }
   }
   wave_sync_lds();
   bool bad = false;
@@ -269,24 +215,6 @@ __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ task
         int rr = rb + 16 * b + l16;
         if (q < w && rr < nr) P[(long long)(c0 + q) * m + r0 + rr] = (T)acc[a][b][r];
       }
-}
-
-// Copies the factored diagonal blocks of one panel launch into their panels (tasks as k_panel; non-designated exit).
-template <typename T>
-__global__ __launch_bounds__(256) void k_commit(const int4* __restrict__ tasks, int c0, SymDev sd, T* __restrict__ L,
-                                                const double* __restrict__ dscratch) {
-  const int4 tk = tasks[blockIdx.x];
-  if (!tk.w) return;
-  const int s = tk.x;
-  const int k = sn_k(sd, s);
-  const long long m = sn_m(sd, s);
-  const int w = min(NB, k - c0);
-  T* P = L + sd.Loff[s];
-  const double* dsc = dscratch + (long long)(tk.w - 1) * NB * NB;
-  for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
-    int q = idx / NB, i = idx % NB;
-    if (i < w && q < w && i >= q) P[(long long)(c0 + q) * m + c0 + i] = (T)dsc[q * NB + i];
-  }
 }
 
 // Trailing update C -= X X^T of a front, 64x64 output tile per workgroup, MFMA (FP64 16x16x4 or FP32 16x16x4).
@@ -448,17 +376,21 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
       }
 }
 
+// Narrow fronts, one workgroup each, in 64-column steps from the kept inverse diagonal blocks: per step one parallel
+// 64x64 product x_t = Linv_t w_t (waves split the columns, partial sums combined in a fixed order) and one parallel
+// update of the rows below, so the dependent chain is ceil(k/64) steps long, not k columns.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd(const int* __restrict__ sns, SymDev sd, const T* __restrict__ L,
-                                             double* __restrict__ y, double* __restrict__ V, double* __restrict__ Wg,
-                                             int cap) {
+                                             const double* __restrict__ Linv, double* __restrict__ y,
+                                             double* __restrict__ V, double* __restrict__ Wg, int cap) {
   extern __shared__ double shw[];
+  __shared__ double part[4][NB], xs[NB];
   const int s = sns[blockIdx.x];
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   double* w = (m <= cap) ? shw : Wg + sd.Woff[s];
-  const int tid = threadIdx.x, bd = blockDim.x;
+  const int tid = threadIdx.x, bd = blockDim.x, lane = tid & 63, wv = tid >> 6;
   for (long long i = tid; i < m; i += bd) w[i] = i < k ? y[f + i] : 0.0;
   __syncthreads();
   for (int cp = sd.child_ptr[s]; cp < sd.child_ptr[s + 1]; cp++) {
@@ -470,21 +402,49 @@ __global__ __launch_bounds__(256) void k_fwd(const int* __restrict__ sns, SymDev
     __syncthreads();
   }
   const T* P = L + sd.Loff[s];
-  for (int q = 0; q < k; q++) {
-    const T* cq = P + (long long)q * m;
-    double xq = w[q] / (double)cq[q];
-    if (tid == 0) y[f + q] = xq;
-    for (long long i = q + 1 + tid; i < m; i += bd) w[i] -= (double)cq[i] * xq;
+  for (int c0 = 0; c0 < k; c0 += NB) {
+    const int wd = min(NB, k - c0);
+    const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
+    // x_r = sum_c Linv[r][c] w_c = sum_c Lt[c][r] w_c; wave wv takes c in [16 wv, 16 wv + 16)
+    double a = 0.0;
+#pragma unroll 4
+    for (int c = 16 * wv; c < 16 * wv + 16; c++)
+      if (c < wd) a += Lt[c * NB + lane] * w[c0 + c];
+    part[wv][lane] = a;
+    __syncthreads();
+    if (tid < NB) {
+      const double x = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+      xs[tid] = tid < wd ? x : 0.0;
+      if (tid < wd) y[f + c0 + tid] = x;
+    }
+    __syncthreads();
+    for (long long i = c0 + wd + tid; i < m; i += bd) {
+      const T* pr = P + (long long)c0 * m + i;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int q = 0;
+      for (; q + 4 <= wd; q += 4) {
+        a0 += (double)pr[(long long)q * m] * xs[q];
+        a1 += (double)pr[(long long)(q + 1) * m] * xs[q + 1];
+        a2 += (double)pr[(long long)(q + 2) * m] * xs[q + 2];
+        a3 += (double)pr[(long long)(q + 3) * m] * xs[q + 3];
+      }
+      for (; q < wd; q++) a0 += (double)pr[(long long)q * m] * xs[q];
+      w[i] -= (a0 + a1) + (a2 + a3);
+    }
     __syncthreads();
   }
   double* vs = V + sd.Voff[s];
   for (long long i = k + tid; i < m; i += bd) vs[i - k] = w[i];
 }
 
+// Backward for narrow fronts: z = y_own - L21^T x_below (wave per column), then descending 64-column steps
+// x_t = Linv_t^T z_t and z_j -= L(c0.., j)^T x_t for the columns j < c0 (wave per column, lanes over the step).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev sd, const T* __restrict__ L,
-                                             double* __restrict__ y, double* __restrict__ Wg, int cap) {
+                                             const double* __restrict__ Linv, double* __restrict__ y,
+                                             double* __restrict__ Wg, int cap) {
   extern __shared__ double shz[];
+  __shared__ double xs[NB];
   const int s = sns[blockIdx.x];
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
@@ -492,112 +452,212 @@ __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev
   const int* rows = sd.sn_rows + sd.sn_rowptr[s];
   double* z = (k <= cap) ? shz : Wg + sd.Woff[s];
   const T* P = L + sd.Loff[s];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int q = wv; q < k; q += nw) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = wv; q < k; q += 4) {
     const T* cq = P + (long long)q * m;
-    double acc = 0.0;
-    for (long long i = k + lane; i < m; i += 64) acc += (double)cq[i] * y[rows[i]];
-    acc = wave_sum(acc);
+    double a0 = 0.0, a1 = 0.0;
+    long long i = k + lane;
+    for (; i + 64 < m; i += 128) {
+      a0 += (double)cq[i] * y[rows[i]];
+      a1 += (double)cq[i + 64] * y[rows[i + 64]];
+    }
+    if (i < m) a0 += (double)cq[i] * y[rows[i]];
+    const double acc = wave_sum(a0 + a1);
     if (lane == 0) z[q] = y[f + q] - acc;
   }
   __syncthreads();
-  for (int q = k - 1; q >= 0; q--) {
-    double xq = z[q] / (double)P[(long long)q * m + q];
-    if (threadIdx.x == 0) y[f + q] = xq;
-    for (int p = threadIdx.x; p < q; p += blockDim.x) z[p] -= (double)P[(long long)p * m + q] * xq;
+  for (int c0 = ((k - 1) / NB) * NB; c0 >= 0; c0 -= NB) {
+    const int wd = min(NB, k - c0);
+    const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
+    const double zl = lane < wd ? z[c0 + lane] : 0.0;
+    for (int c = wv; c < wd; c += 4) {
+      // x_c = sum_r Linv[r][c] z_r = sum_r Lt[c][r] z_r
+      double a = lane < wd ? Lt[c * NB + lane] * zl : 0.0;
+      a = wave_sum(a);
+      if (lane == 0) xs[c] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < wd) y[f + c0 + threadIdx.x] = xs[threadIdx.x];
+    const double xl = lane < wd ? xs[lane] : 0.0;
+    for (int j = wv; j < c0; j += 4) {
+      double a = lane < wd ? (double)P[(long long)j * m + c0 + lane] * xl : 0.0;
+      a = wave_sum(a);
+      if (lane == 0) z[j] -= a;
+    }
     __syncthreads();
   }
 }
 
 
 // ---------------------------------------------------------------------------------------- blocked sweeps (wide fronts)
-// Forward: W_s = [y_own ; 0] + children's update vectors (fixed child order).
-__global__ __launch_bounds__(256) void k_fwdb_gather(const int* __restrict__ sns, SymDev sd, const double* __restrict__ y,
-                                                     const double* __restrict__ V, double* __restrict__ Wg) {
-  const int s = sns[blockIdx.x];
-  const int f = sd.sn_first[s];
+// Wide fronts are swept in steps of SB = 256 columns. Each step is two launches: a diagonal launch x_t = X_t w_t with
+// X_t the explicit inverse of the step's 256x256 diagonal block (built once per factorization from the kept 64x64
+// inverses, k_inv256), and a panel launch that applies the step to every other row (forward) or column (backward) of
+// the front with many workgroups. X_t is stored column by column: X[c * SB + r] = inv(L_tt)[r][c].
+
+// Inverse of one 256-column diagonal block, one 64-column strip j per workgroup: X_jj = Linv_j and, going down,
+// X_ij = -Linv_i * sum_{p=j}^{i-1} L_ip X_pj. Thread (r, q): row r of the 64-row tile, columns [16 q, 16 q + 16).
+// task: (s, c0, j, 0). Strips above the diagonal are stored as zeros.
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv256(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
+                                                const double* __restrict__ Linv, double* __restrict__ X) {
+  __shared__ double Ts[NB][NB + 1];
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x, c0 = tk.y, j = tk.z;
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  double* w = Wg + sd.Woff[s];
-  for (long long i = threadIdx.x; i < m; i += blockDim.x) w[i] = i < k ? y[f + i] : 0.0;
+  const int wd = min(SB, k - c0);
+  const int nsub = (wd + NB - 1) / NB;
+  const T* P = L + sd.Loff[s];
+  double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const double* Ltj = Linv + sd.Ioff[s] + (long long)((c0 + NB * j) / NB) * NB * NB;
+  for (int cc = 0; cc < 16; cc++) {
+    const int c = 16 * q + cc;
+    double* xc = Xb + (long long)(NB * j + c) * SB;
+    for (int i = 0; i < j; i++) xc[NB * i + r] = 0.0;
+    xc[NB * j + r] = Ltj[c * NB + r];
+  }
   __syncthreads();
-  for (int cp = sd.child_ptr[s]; cp < sd.child_ptr[s + 1]; cp++) {
-    int c = sd.child_list[cp];
-    int mu = (int)(sn_m(sd, c) - sn_k(sd, c));
-    const int* rel = sd.rel + sd.rel_off[c];
-    const double* vc = V + sd.Voff[c];
-    for (int i = threadIdx.x; i < mu; i += blockDim.x) w[rel[i]] += vc[i];
+  for (int i = j + 1; i < nsub; i++) {
+    double acc[16];
+#pragma unroll
+    for (int cc = 0; cc < 16; cc++) acc[cc] = 0.0;
+    const bool rok = NB * i + r < wd;
+    for (int p0 = j; p0 < i; p0++) {
+      for (int idx = threadIdx.x; idx < NB * NB; idx += 256) {
+        const int pp = idx & 63, c = idx >> 6;
+        Ts[pp][c] = Xb[(long long)(NB * j + c) * SB + NB * p0 + pp];
+      }
+      __syncthreads();
+      const T* lrow = P + (long long)(c0 + NB * p0) * m + c0 + NB * i + r;
+      for (int pp = 0; pp < NB; pp++) {
+        const double a = rok ? (double)lrow[(long long)pp * m] : 0.0;
+#pragma unroll
+        for (int cc = 0; cc < 16; cc++) acc[cc] += a * Ts[pp][16 * q + cc];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int cc = 0; cc < 16; cc++) Ts[r][16 * q + cc] = acc[cc];
+    __syncthreads();
+    const double* Lti = Linv + sd.Ioff[s] + (long long)((c0 + NB * i) / NB) * NB * NB;
+#pragma unroll
+    for (int cc = 0; cc < 16; cc++) acc[cc] = 0.0;
+    for (int pp = 0; pp < NB; pp++) {
+      const double a = Lti[pp * NB + r];  // Linv_i[r][pp]
+#pragma unroll
+      for (int cc = 0; cc < 16; cc++) acc[cc] += a * Ts[pp][16 * q + cc];
+    }
+#pragma unroll
+    for (int cc = 0; cc < 16; cc++) Xb[(long long)(NB * j + 16 * q + cc) * SB + NB * i + r] = -acc[cc];
     __syncthreads();
   }
 }
 
-// One 64-column step of the forward sweep. task: (s, first row, end row, writer). x_t = Linv_t w_t from the kept
-// inverse diagonal block (a parallel 64x64 product, no sequential chain), then W(rows) -= L(rows, t) x_t for this
-// task's rows >= c0 + w. W(c0 .. c0+w) is read by every task and written by none.
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwdb_step(const int4* __restrict__ tasks, int c0, SymDev sd,
-                                                   const T* __restrict__ L, const double* __restrict__ Linv,
-                                                   double* __restrict__ y, double* __restrict__ Wg) {
-  __shared__ double ws[NB], xs[NB];
+// Forward, first part: W_s = [y_own ; 0] plus the children's update vectors, one thread per row; the contributions
+// of a row are listed in child order (row-wise inverse of the children's relative maps), so the sum is fixed-order.
+// task: (s, first row, end row, 0)
+__global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ tasks, SymDev sd,
+                                                     const double* __restrict__ y, const double* __restrict__ V,
+                                                     double* __restrict__ Wg) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
+  const long long i = (long long)tk.y + threadIdx.x;
+  if (i >= tk.z) return;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long* gp = sd.gptr + sd.Goff[s];
+  double w = i < k ? y[f + i] : 0.0;
+  for (long long e = gp[i]; e < gp[i + 1]; e++) w += V[sd.gsrc[e]];
+  Wg[sd.Woff[s] + i] = w;
+}
+
+// Forward diagonal step: x_r = sum_c X_t[r][c] w_c for the 64 rows [c0 + 64 tk.z, ...) of the step; the waves split
+// the columns (only c <= r is nonzero) and the partial sums are combined in a fixed order. task: (s, c0, row block, 0)
+__global__ __launch_bounds__(256) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const double* __restrict__ Wg) {
+  __shared__ double part[4][NB];
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x, c0 = tk.y, rb = tk.z;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const int wd = min(SB, k - c0);
+  const double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  const double* w = Wg + sd.Woff[s] + c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = NB * rb + lane;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (wv <= rb) {
+    const int cend = min(wd, NB * wv + NB);
+    int c = NB * wv;
+    for (; c + 4 <= cend; c += 4) {
+      a0 += Xb[(long long)c * SB + r] * w[c];
+      a1 += Xb[(long long)(c + 1) * SB + r] * w[c + 1];
+      a2 += Xb[(long long)(c + 2) * SB + r] * w[c + 2];
+      a3 += Xb[(long long)(c + 3) * SB + r] * w[c + 3];
+    }
+    for (; c < cend; c++) a0 += Xb[(long long)c * SB + r] * w[c];
+  }
+  part[wv][lane] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (threadIdx.x < NB && r < wd) y[f + c0 + r] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+}
+
+// Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 rows per workgroup (waves split the step's columns);
+// on the front's last step the rows are its update-vector rows and go straight to V. task: (s, c0, first row, last)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tasks, SymDev sd,
+                                                    const T* __restrict__ L, const double* __restrict__ y,
+                                                    double* __restrict__ Wg, double* __restrict__ V) {
+  __shared__ double xs[SB], part[4][NB];
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  const int w = min(NB, k - c0);
-  const T* P = L + sd.Loff[s];
-  const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
-  double* W = Wg + sd.Woff[s];
-  if (threadIdx.x < NB) ws[threadIdx.x] = threadIdx.x < w ? W[c0 + threadIdx.x] : 0.0;
+  const int wd = min(SB, k - c0);
+  xs[threadIdx.x] = threadIdx.x < wd ? y[f + c0 + threadIdx.x] : 0.0;
   __syncthreads();
-  if (threadIdx.x < NB) {
-    const int r = threadIdx.x;
-    double x = 0.0;
-    for (int c = 0; c < w; c++) x += Lt[c * NB + r] * ws[c];
-    xs[r] = r < w ? x : 0.0;
-    if (tk.w && r < w) y[f + c0 + r] = x;
-  }
-  __syncthreads();
-  const long long row = (long long)tk.y + threadIdx.x;
-  if (row < tk.z) {
-    const T* pr = P + (long long)c0 * m + row;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int q = 0;
-    for (; q + 4 <= w; q += 4) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long row = (long long)tk.z + lane;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const int q0 = NB * wv, q1 = min(wd, q0 + NB);
+  if (row < m && q0 < q1) {
+    const T* pr = L + sd.Loff[s] + (long long)c0 * m + row;
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {
       a0 += (double)pr[(long long)q * m] * xs[q];
       a1 += (double)pr[(long long)(q + 1) * m] * xs[q + 1];
       a2 += (double)pr[(long long)(q + 2) * m] * xs[q + 2];
       a3 += (double)pr[(long long)(q + 3) * m] * xs[q + 3];
     }
-    for (; q < w; q++) a0 += (double)pr[(long long)q * m] * xs[q];
-    W[row] -= (a0 + a1) + (a2 + a3);
+    for (; q < q1; q++) a0 += (double)pr[(long long)q * m] * xs[q];
+  }
+  part[wv][lane] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (threadIdx.x < NB && row < m) {
+    double* W = Wg + sd.Woff[s];
+    const double v = W[row] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+    if (tk.w) V[sd.Voff[s] + row - k] = v;
+    else W[row] = v;
   }
 }
 
-// Update vector of a wide front for its parent: V_s = W_s[k..m).
-__global__ __launch_bounds__(256) void k_fwdb_finish(const int* __restrict__ sns, SymDev sd, const double* __restrict__ Wg,
-                                                     double* __restrict__ V) {
-  const int s = sns[blockIdx.x];
-  const int k = sn_k(sd, s);
-  const long long m = sn_m(sd, s);
-  const double* w = Wg + sd.Woff[s];
-  double* vs = V + sd.Voff[s];
-  for (long long i = k + threadIdx.x; i < m; i += blockDim.x) vs[i - k] = w[i];
-}
-
 // Backward, first part: the solution at a wide front's lower rows, gathered into one contiguous vector (the front's
-// forward update-vector slot, free once the forward sweep is done).
-__global__ __launch_bounds__(256) void k_bwdb_xb(const int* __restrict__ sns, SymDev sd, const double* __restrict__ y,
-                                                 double* __restrict__ V) {
-  const int s = sns[blockIdx.x];
+// forward update-vector slot, free once the forward sweep is done). task: (s, first row, end row, 0), rows >= k.
+__global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks, SymDev sd,
+                                                 const double* __restrict__ y, double* __restrict__ V) {
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x;
+  const long long i = (long long)tk.y + threadIdx.x;
+  if (i >= tk.z) return;
   const int k = sn_k(sd, s);
-  const long long m = sn_m(sd, s);
-  const int* rows = sd.sn_rows + sd.sn_rowptr[s];
-  double* xb = V + sd.Voff[s];
-  for (long long i = k + threadIdx.x; i < m; i += blockDim.x) xb[i - k] = y[rows[i]];
+  V[sd.Voff[s] + i - k] = y[sd.sn_rows[sd.sn_rowptr[s] + i]];
 }
 
-// z_q = y_own[q] - L21(:, q)^T xb for every own column q: one wave per column, 4 independent partial sums per lane.
+// z_q = y_own[q] - L21(:, q)^T xb for every own column q: one wave per column, 8 independent partial sums per lane.
 // task: (s, first column, end column, 0)
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
@@ -615,54 +675,67 @@ __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ task
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int q = tk.y + wv; q < tk.z; q += 4) {
     const T* cq = P + (long long)q * m + k;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     long long i = lane;
-    for (; i + 192 < mb; i += 256) {
-      a0 += (double)cq[i] * xb[i];
-      a1 += (double)cq[i + 64] * xb[i + 64];
-      a2 += (double)cq[i + 128] * xb[i + 128];
-      a3 += (double)cq[i + 192] * xb[i + 192];
+    for (; i + 448 < mb; i += 512) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) a[u] += (double)cq[i + 64 * u] * xb[i + 64 * u];
     }
-    for (; i < mb; i += 64) a0 += (double)cq[i] * xb[i];
-    double acc = wave_sum((a0 + a1) + (a2 + a3));
+    for (; i < mb; i += 64) a[0] += (double)cq[i] * xb[i];
+    double acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    acc = wave_sum(acc);
     if (lane == 0) z[q] = y[f + q] - acc;
   }
 }
 
-// One 64-column step (descending) of the backward sweep. task: (s, first column, end column, writer); columns < c0.
-// x_t = Linv_t^T z_t from the kept inverse block, then z_j -= L(c0.., j)^T x_t for this task's columns (wave per
-// column).
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwdb_step(const int4* __restrict__ tasks, int c0, SymDev sd,
-                                                   const T* __restrict__ L, const double* __restrict__ Linv,
-                                                   double* __restrict__ y, double* __restrict__ Wg) {
-  __shared__ double zs[NB], xs[NB];
+// Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...): one wave per
+// column, lanes over r >= c. task: (s, c0, column block, 0)
+__global__ __launch_bounds__(256) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const double* __restrict__ Wg) {
   const int4 tk = tasks[blockIdx.x];
-  const int s = tk.x;
+  const int s = tk.x, c0 = tk.y, cb = tk.z;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const int wd = min(SB, k - c0);
+  const double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  const double* z = Wg + sd.Woff[s] + c0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c = NB * cb + wv; c < min(wd, NB * cb + NB); c += 4) {
+    const double* xc = Xb + (long long)c * SB;
+    double a = 0.0;
+    for (int r = NB * cb + lane; r < wd; r += NB) a += xc[r] * z[r];
+    a = wave_sum(a);
+    if (lane == 0) y[f + c0 + c] = a;
+  }
+}
+
+// Backward panel step: z_j -= L(c0 .. c0+wd, j)^T x_t for the columns j of this task (< c0): one wave per column,
+// lanes over the step's rows. task: (s, c0, first column, end column)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwdb_panel(const int4* __restrict__ tasks, SymDev sd,
+                                                    const T* __restrict__ L, const double* __restrict__ y,
+                                                    double* __restrict__ Wg) {
+  __shared__ double xs[SB];
+  const int4 tk = tasks[blockIdx.x];
+  const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  const int w = min(NB, k - c0);
+  const int wd = min(SB, k - c0);
+  xs[threadIdx.x] = threadIdx.x < wd ? y[f + c0 + threadIdx.x] : 0.0;
+  __syncthreads();
   const T* P = L + sd.Loff[s];
-  const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
   double* z = Wg + sd.Woff[s];
-  if (threadIdx.x < NB) zs[threadIdx.x] = threadIdx.x < w ? z[c0 + threadIdx.x] : 0.0;
-  __syncthreads();
-  if (threadIdx.x < NB) {
-    // x_c = sum_r Linv[r][c] z_r = sum_r Lt[c][r] z_r
-    const int c = threadIdx.x;
-    const double* lc = Lt + c * NB;
-    double x = 0.0;
-    for (int r = 0; r < w; r++) x += lc[r] * zs[r];
-    xs[c] = c < w ? x : 0.0;
-    if (tk.w && c < w) y[f + c0 + c] = x;
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int j = tk.y + wv; j < tk.z; j += 4) {
+  for (int j = tk.z + wv; j < tk.w; j += 4) {
     const T* cj = P + (long long)j * m + c0;
-    double acc = lane < w ? (double)cj[lane] * xs[lane] : 0.0;
-    acc = wave_sum(acc);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (lane < wd) a0 = (double)cj[lane] * xs[lane];
+    if (lane + 64 < wd) a1 = (double)cj[lane + 64] * xs[lane + 64];
+    if (lane + 128 < wd) a2 = (double)cj[lane + 128] * xs[lane + 128];
+    if (lane + 192 < wd) a3 = (double)cj[lane + 192] * xs[lane + 192];
+    const double acc = wave_sum((a0 + a1) + (a2 + a3));
     if (lane == 0) z[j] -= acc;
   }
 }
@@ -698,7 +771,8 @@ hipEvent_t SpdDevice::event(size_t i) {
 
 size_t SpdDevice::device_bytes() const {
   return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
-         sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() + zero_tasks_.bytes();
+         sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() + zero_tasks_.bytes() +
+         Linv_.bytes() + X_.bytes() + gptr_.bytes() + gsrc_.bytes() + fwd_tasks_.bytes() + bwd_tasks_.bytes();
 }
 
 void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* perm, const SymbolicOptions& opt) {
@@ -730,7 +804,6 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   for (int s = 0; s < ns; s++) bylevel[S.sn_level[s]].push_back(s);
   std::vector<int> lvl_sn;
   std::vector<int4> ext, pan, syr, zer, pot;
-  int max_slots = 1;
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
     L.sn_begin = (long long)lvl_sn.size();
@@ -744,8 +817,13 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
     L.zero.begin = (long long)zer.size();
     for (int s : bylevel[l]) {
       long long mu = (S.sn_rowptr[s + 1] - S.sn_rowptr[s]) - (S.sn_first[s + 1] - S.sn_first[s]);
-      long long chunks = (mu * mu + ZERO_CHUNK - 1) / ZERO_CHUNK;
-      for (long long c = 0; c < chunks; c++) zer.push_back(make_int4(s, (int)c, 0, 0));
+      long long j = 0;
+      while (j < mu) {
+        long long j1 = j, work = 0;
+        while (j1 < mu && work < ZERO_TARGET) work += mu - j1++;
+        zer.push_back(make_int4(s, (int)j, (int)j1, 0));
+        j = j1;
+      }
     }
     L.zero.end = (long long)zer.size();
     // extend-add rounds: round r = r-th child of every parent at this level
@@ -813,7 +891,6 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       L.potrf_steps.push_back(lpo);
       L.syrk_steps.push_back(ls);
       L.syrk_flops.push_back(fl);
-      max_slots = std::max(max_slots, nslot);
       lo.begin = (long long)syr.size();
       lo.c0 = ob0;
       if (outer_now) {
@@ -839,68 +916,107 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       L.outer_flops.push_back(flo);
     }
   }
-  // sweep schedule: narrow fronts (k <= BIG_K) one workgroup each, wide fronts in 128-column steps
+  // sweep schedule: narrow fronts one workgroup each; wide fronts in SB-column steps (diagonal + panel launches)
   std::vector<int> swp;
-  std::vector<int4> fwt, bwt;
+  std::vector<int4> fwt, bwt, invt;
+  std::vector<long long> Xoff(ns + 1, 0), Goff(ns + 1, 0), gptr, gsrc;
+  auto sn_kk = [&](int s) { return S.sn_first[s + 1] - S.sn_first[s]; };
+  auto sn_mm = [&](int s) { return S.sn_rowptr[s + 1] - S.sn_rowptr[s]; };
+  std::vector<char> isbig(ns, 0);
+  for (int s = 0; s < ns; s++) {
+    long long k = sn_kk(s), m = sn_mm(s);
+    isbig[s] = k > BIG_K || k * m > BIG_PANEL;
+    Xoff[s + 1] = Xoff[s] + (isbig[s] ? ((k + SB - 1) / SB) * SB * SB : 0);
+    if (!isbig[s]) continue;
+    // row-wise inverse of the children's relative maps, contributions of a row in child order
+    Goff[s] = (long long)gptr.size();
+    std::vector<long long> cnt(m + 1, 0);
+    for (int cp = cptr[s]; cp < cptr[s + 1]; cp++) {
+      int c = clist[cp];
+      long long mu = sn_mm(c) - sn_kk(c);
+      const int* rel = S.rel.data() + S.rel_off[c];
+      for (long long i = 0; i < mu; i++) cnt[rel[i] + 1]++;
+    }
+    const long long base = (long long)gsrc.size();
+    for (long long i = 0; i < m; i++) cnt[i + 1] += cnt[i];
+    for (long long i = 0; i <= m; i++) gptr.push_back(base + cnt[i]);
+    gsrc.resize(base + cnt[m]);
+    for (int cp = cptr[s]; cp < cptr[s + 1]; cp++) {
+      int c = clist[cp];
+      long long mu = sn_mm(c) - sn_kk(c);
+      const int* rel = S.rel.data() + S.rel_off[c];
+      for (long long i = 0; i < mu; i++) gsrc[base + cnt[rel[i]]++] = S.Voff[c] + i;
+    }
+  }
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
     std::vector<int> small, big;
-    for (int s : bylevel[l]) {
-      long long k = S.sn_first[s + 1] - S.sn_first[s], m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
-      (k > BIG_K || k * m > BIG_PANEL ? big : small).push_back(s);
-    }
+    for (int s : bylevel[l]) (isbig[s] ? big : small).push_back(s);
     L.small_begin = (long long)swp.size();
     for (int s : small) {
       swp.push_back(s);
-      L.small_max_m = std::max<long long>(L.small_max_m, S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+      L.small_max_m = std::max<long long>(L.small_max_m, sn_mm(s));
     }
     L.small_end = (long long)swp.size();
-    L.big_begin = (long long)swp.size();
-    swp.insert(swp.end(), big.begin(), big.end());
-    L.big_end = (long long)swp.size();
     int maxk = 0;
-    for (int s : big) maxk = std::max(maxk, S.sn_first[s + 1] - S.sn_first[s]);
+    for (int s : big) maxk = std::max(maxk, sn_kk(s));
+    // forward
+    L.fwd_gather.begin = (long long)fwt.size();
+    for (int s : big)
+      for (long long r0 = 0; r0 < sn_mm(s); r0 += 256)
+        fwt.push_back(make_int4(s, (int)r0, (int)std::min<long long>(sn_mm(s), r0 + 256), 0));
+    L.fwd_gather.end = (long long)fwt.size();
     for (int c0 = 0; c0 < maxk; c0 += SB) {
-      LaunchList ll;
-      ll.begin = (long long)fwt.size();
-      ll.c0 = c0;
+      LaunchList d, pn;
+      d.begin = (long long)fwt.size();
       for (int s : big) {
-        int k = S.sn_first[s + 1] - S.sn_first[s];
+        int k = sn_kk(s);
         if (k <= c0) continue;
-        long long m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
-        long long r0 = c0 + std::min(SB, k - c0);
-        long long chunks = std::max<long long>(1, (m - r0 + SWEEP_ROWS - 1) / SWEEP_ROWS);
-        for (long long c = 0; c < chunks; c++) {
-          long long a = r0 + c * SWEEP_ROWS, b = std::min<long long>(m, a + SWEEP_ROWS);
-          fwt.push_back(make_int4(s, (int)a, (int)std::max(a, b), c == 0 ? 1 : 0));
-        }
+        int wd = std::min(SB, k - c0);
+        for (int rb = 0; rb * NB < wd; rb++) fwt.push_back(make_int4(s, c0, rb, 0));
+        for (int j = 0; j * NB < wd; j++) invt.push_back(make_int4(s, c0, j, 0));
       }
-      ll.end = (long long)fwt.size();
-      L.fwd_steps.push_back(ll);
+      d.end = pn.begin = (long long)fwt.size();
+      for (int s : big) {
+        int k = sn_kk(s);
+        if (k <= c0) continue;
+        int wd = std::min(SB, k - c0);
+        for (long long r0 = c0 + wd; r0 < sn_mm(s); r0 += NB)
+          fwt.push_back(make_int4(s, c0, (int)r0, c0 + wd == k ? 1 : 0));
+      }
+      pn.end = (long long)fwt.size();
+      L.fwd_diag.push_back(d);
+      L.fwd_panel.push_back(pn);
     }
-    L.bwd_gemv.begin = (long long)bwt.size();
-    for (int s : big) {
-      int k = S.sn_first[s + 1] - S.sn_first[s];
-      for (int q = 0; q < k; q += 16) bwt.push_back(make_int4(s, q, std::min(k, q + 16), 0));
-    }
+    // backward
+    L.bwd_xb.begin = (long long)bwt.size();
+    for (int s : big)
+      for (long long r0 = sn_kk(s); r0 < sn_mm(s); r0 += 256)
+        bwt.push_back(make_int4(s, (int)r0, (int)std::min<long long>(sn_mm(s), r0 + 256), 0));
+    L.bwd_xb.end = L.bwd_gemv.begin = (long long)bwt.size();
+    for (int s : big)
+      for (int q = 0; q < sn_kk(s); q += SWEEP_COLS) bwt.push_back(make_int4(s, q, std::min(sn_kk(s), q + SWEEP_COLS), 0));
     L.bwd_gemv.end = (long long)bwt.size();
     for (int c0 = ((maxk - 1) / SB) * SB; maxk > 0 && c0 >= 0; c0 -= SB) {
-      LaunchList ll;
-      ll.begin = (long long)bwt.size();
-      ll.c0 = c0;
+      LaunchList d, pn;
+      d.begin = (long long)bwt.size();
       for (int s : big) {
-        int k = S.sn_first[s + 1] - S.sn_first[s];
+        int k = sn_kk(s);
         if (k <= c0) continue;
-        long long chunks = std::max<long long>(1, (c0 + SWEEP_COLS - 1) / SWEEP_COLS);
-        for (long long c = 0; c < chunks; c++) {
-          long long a = c * SWEEP_COLS, b = std::min<long long>(c0, a + SWEEP_COLS);
-          bwt.push_back(make_int4(s, (int)a, (int)std::max(a, b), c == 0 ? 1 : 0));
-        }
+        int wd = std::min(SB, k - c0);
+        for (int cb = 0; cb * NB < wd; cb++) bwt.push_back(make_int4(s, c0, cb, 0));
       }
-      ll.end = (long long)bwt.size();
-      L.bwd_steps.push_back(ll);
+      d.end = pn.begin = (long long)bwt.size();
+      for (int s : big) {
+        if (sn_kk(s) <= c0) continue;
+        for (int j = 0; j < c0; j += SWEEP_COLS) bwt.push_back(make_int4(s, c0, j, std::min(c0, j + SWEEP_COLS)));
+      }
+      pn.end = (long long)bwt.size();
+      L.bwd_diag.push_back(d);
+      L.bwd_panel.push_back(pn);
     }
   }
+  inv_launch_ = LaunchList{0, (long long)invt.size(), 0};
   // upload
   sweep_sn_.upload(swp, stream_);
   if (fwt.empty()) fwd_tasks_.alloc(1); else fwd_tasks_.upload(fwt, stream_);
@@ -915,6 +1031,12 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   Woff_.upload(Woff, stream_);
   Ioff_.upload(Ioff, stream_);
   Linv_.alloc(std::max<long long>(1, Ioff[ns]));
+  Xoff_.upload(Xoff, stream_);
+  X_.alloc(std::max<long long>(1, Xoff[ns]));
+  Goff_.upload(Goff, stream_);
+  if (gptr.empty()) gptr_.alloc(1); else gptr_.upload(gptr, stream_);
+  if (gsrc.empty()) gsrc_.alloc(1); else gsrc_.upload(gsrc, stream_);
+  if (invt.empty()) inv_tasks_.alloc(1); else inv_tasks_.upload(invt, stream_);
   rel_off_.upload(S.rel_off, stream_);
   if (S.rel.empty()) rel_.alloc(1); else rel_.upload(S.rel, stream_);
   child_ptr_.upload(cptr, stream_);
@@ -927,13 +1049,12 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   if (zer.empty()) zero_tasks_.alloc(1); else zero_tasks_.upload(zer, stream_);
   if (pot.empty()) potrf_tasks_.alloc(1); else potrf_tasks_.upload(pot, stream_);
   level_sn_.upload(lvl_sn, stream_);
-  dscratch_.alloc((size_t)max_slots * NB * NB);
   V_.alloc(std::max<long long>(1, S.Vsize));
   W_.alloc(std::max<long long>(1, Woff[ns]));
   y_.alloc(n);
   err_.alloc(1);
   sd_ = SymDev{sn_first_.p, sn_rowptr_.p, sn_rows_.p, Loff_.p, Uoff_.p, Voff_.p, Woff_.p, rel_off_.p, rel_.p,
-               child_ptr_.p, child_list_.p, Ioff_.p};
+               child_ptr_.p, child_list_.p, Ioff_.p, Xoff_.p, Goff_.p, gptr_.p, gsrc_.p};
   IFEA_HIP(hipStreamSynchronize(stream_));
   // host copies not needed after upload
   std::vector<int>().swap(S_.amap_src);
@@ -986,11 +1107,17 @@ int SpdDevice::factorize_t(const double* d_vals) {
         if (profiling_) {
           IFEA_HIP(hipEventRecord(event(2 * nev + 1), stream_));
           nev++;
-          prof_.syrk_flops += mode == 0 ? lv.syrk_flops[t] : lv.outer_flops[t];
+          ev_mode_.push_back((char)mode);
+          if (mode == 0) prof_.syrk_flops += lv.syrk_flops[t];
+          else prof_.outer_flops += lv.outer_flops[t];
+          if (mode == 1 && sizeof(T) == 4) prof_.outer_flops_f32 += lv.outer_flops[t];
         }
       }
     }
   }
+  if (inv_launch_.end > inv_launch_.begin)
+    hipLaunchKernelGGL(k_inv256<T>, dim3((unsigned)(inv_launch_.end - inv_launch_.begin)), dim3(256), 0, stream_,
+                       inv_tasks_.p, sd_, L, Linv_.p, X_.p);
   IFEA_HIP(hipGetLastError());
   int herr = 0;
   IFEA_HIP(hipMemcpyAsync(&herr, err_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -999,10 +1126,21 @@ int SpdDevice::factorize_t(const double* d_vals) {
     for (size_t e = 0; e < nev; e++) {
       float ms = 0;
       IFEA_HIP(hipEventElapsedTime(&ms, event(2 * e), event(2 * e + 1)));
-      prof_.syrk_ms += ms;
+      if (ev_mode_[e] == 0) {
+        prof_.syrk_ms += ms;
+        prof_.syrk_launches++;
+      } else {
+        prof_.outer_ms += ms;
+        prof_.outer_launches++;
+        if (sizeof(T) == 4) {
+          prof_.outer_ms_f32 += ms;
+          prof_.outer_launches_f32++;
+        }
+      }
     }
-    prof_.syrk_launches += (long long)nev;
+    ev_mode_.clear();
     prof_.factorizations++;
+    if (sizeof(T) == 4) prof_.f32_factorizations++;
     prof_.factor_flops += S.flops;
   }
   return herr ? -4 : 0;
@@ -1037,24 +1175,24 @@ int SpdDevice::factorize_device(const double* d_vals, int precision) {
 
 template <typename T>
 void SpdDevice::sweeps(const T* L) {
-  const int cap = 8192;  // doubles of LDS per forward/backward work vector
+  const int cap = 7680;  // doubles of LDS per forward/backward work vector (plus 2.5 KB of static LDS)
+  auto launch = [&](auto kern, const LaunchList& ll, const int4* tasks, auto... args) {
+    if (ll.end > ll.begin)
+      hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(256), 0, stream_, tasks + ll.begin, sd_,
+                         args...);
+  };
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
     unsigned cnt = (unsigned)(lv.small_end - lv.small_begin);
     if (cnt) {
       size_t sh = (size_t)std::min<long long>(lv.small_max_m, cap) * sizeof(double);
-      hipLaunchKernelGGL(k_fwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L, y_.p,
-                         V_.p, W_.p, cap);
+      hipLaunchKernelGGL(k_fwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L,
+                         Linv_.p, y_.p, V_.p, W_.p, cap);
     }
-    unsigned nbig = (unsigned)(lv.big_end - lv.big_begin);
-    if (nbig) {
-      hipLaunchKernelGGL(k_fwdb_gather, dim3(nbig), dim3(256), 0, stream_, sweep_sn_.p + lv.big_begin, sd_, y_.p,
-                         V_.p, W_.p);
-      for (const LaunchList& st : lv.fwd_steps)
-        hipLaunchKernelGGL(k_fwdb_step<T>, dim3((unsigned)(st.end - st.begin)), dim3(256), 0, stream_,
-                           fwd_tasks_.p + st.begin, st.c0, sd_, L, Linv_.p, y_.p, W_.p);
-      hipLaunchKernelGGL(k_fwdb_finish, dim3(nbig), dim3(256), 0, stream_, sweep_sn_.p + lv.big_begin, sd_, W_.p,
-                         V_.p);
+    launch(k_fwdb_gather, lv.fwd_gather, fwd_tasks_.p, (const double*)y_.p, (const double*)V_.p, W_.p);
+    for (size_t t = 0; t < lv.fwd_diag.size(); t++) {
+      launch(k_fwdb_diag, lv.fwd_diag[t], fwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
+      launch(k_fwdb_panel<T>, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p, V_.p);
     }
   }
   for (size_t l = levels_.size(); l-- > 0;) {
@@ -1062,17 +1200,14 @@ void SpdDevice::sweeps(const T* L) {
     unsigned cnt = (unsigned)(lv.small_end - lv.small_begin);
     if (cnt) {
       size_t sh = (size_t)std::min<long long>(lv.small_max_m, cap) * sizeof(double);
-      hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L, y_.p,
-                         W_.p, cap);
+      hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L,
+                         Linv_.p, y_.p, W_.p, cap);
     }
-    if (lv.big_end > lv.big_begin) {
-      hipLaunchKernelGGL(k_bwdb_xb, dim3((unsigned)(lv.big_end - lv.big_begin)), dim3(256), 0, stream_,
-                         sweep_sn_.p + lv.big_begin, sd_, y_.p, V_.p);
-      hipLaunchKernelGGL(k_bwdb_gemv<T>, dim3((unsigned)(lv.bwd_gemv.end - lv.bwd_gemv.begin)), dim3(256), 0,
-                         stream_, bwd_tasks_.p + lv.bwd_gemv.begin, sd_, L, y_.p, V_.p, W_.p);
-      for (const LaunchList& st : lv.bwd_steps)
-        hipLaunchKernelGGL(k_bwdb_step<T>, dim3((unsigned)(st.end - st.begin)), dim3(256), 0, stream_,
-                           bwd_tasks_.p + st.begin, st.c0, sd_, L, Linv_.p, y_.p, W_.p);
+    launch(k_bwdb_xb, lv.bwd_xb, bwd_tasks_.p, (const double*)y_.p, V_.p);
+    launch(k_bwdb_gemv<T>, lv.bwd_gemv, bwd_tasks_.p, L, (const double*)y_.p, (const double*)V_.p, W_.p);
+    for (size_t t = 0; t < lv.bwd_diag.size(); t++) {
+      launch(k_bwdb_diag, lv.bwd_diag[t], bwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
+      launch(k_bwdb_panel<T>, lv.bwd_panel[t], bwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
   }
 }
@@ -1113,6 +1248,7 @@ void SpdDevice::solve_t(const T* L, const double* d_b, double* d_x) {
     IFEA_HIP(hipEventElapsedTime(&ms, event(0), event(1)));
     prof_.solve_ms += ms;
     prof_.solves++;
+    if (sizeof(T) == 4) prof_.f32_solves++;
     double b = 0;
     for (const Level& lv : levels_) b += lv.solve_bytes;
     prof_.solve_bytes += 2.0 * b * sizeof(T);

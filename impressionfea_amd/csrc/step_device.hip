@@ -878,7 +878,7 @@ void StepDevice::set_v(const double* v) {
 }
 
 void StepDevice::info(int reset, double* out) {
-  for (int k = 0; k < 16; k++) out[k] = 0;
+  for (int k = 0; k < 24; k++) out[k] = 0;
   if (spd_->analyzed()) {
     const Symbolic& S = spd_->symbolic();
     out[0] = (double)S.nnzL;
@@ -899,6 +899,14 @@ void StepDevice::info(int reset, double* out) {
   out[13] = p.solve_bytes;
   out[14] = (double)p.factorizations;
   out[15] = p.factor_ms;
+  out[16] = (double)p.outer_launches;
+  out[17] = p.outer_ms;
+  out[18] = p.outer_flops;
+  out[19] = (double)p.f32_factorizations;
+  out[20] = (double)p.f32_solves;
+  out[21] = (double)p.outer_launches_f32;
+  out[22] = p.outer_ms_f32;
+  out[23] = p.outer_flops_f32;
   if (reset) spd_->reset_profile();
 }
 

@@ -428,12 +428,15 @@ class Forging3DSolver:
         self._state_dirty = True
 
     def info(self, reset: bool = False):
-        out = np.zeros(16)
+        out = np.zeros(24)
         self._check(N.lib().ifea_step_info(self._h, int(reset), N.ptr(out, C.c_double)), "info")
         return dict(nnz_l=int(out[0]), flops=out[1], supernodes=int(out[2]), device_bytes=int(out[3]),
                     factor_ms=out[4], l_words=int(out[5]), u_words=int(out[6]), levels=int(out[7]),
                     syrk_launches=int(out[8]), syrk_ms=out[9], syrk_flops=out[10], sweeps=int(out[11]),
-                    sweep_ms=out[12], sweep_bytes=out[13], factorizations=int(out[14]), factorization_ms=out[15])
+                    sweep_ms=out[12], sweep_bytes=out[13], factorizations=int(out[14]), factorization_ms=out[15],
+                    outer_launches=int(out[16]), outer_ms=out[17], outer_flops=out[18],
+                    fp32_factorizations=int(out[19]), fp32_sweeps=int(out[20]), outer_launches_fp32=int(out[21]),
+                    outer_ms_fp32=out[22], outer_flops_fp32=out[23])
 
     def close(self):
         if getattr(self, "_h", None):

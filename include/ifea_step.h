@@ -92,11 +92,13 @@ int ifea_step_advance(ifea_step_t h, double dt, double* eps, double* sigma, doub
 int ifea_step_get_v(ifea_step_t h, double* v);
 int ifea_step_set_v(ifea_step_t h, const double* v);
 
-/* Device and factor statistics: [0] nnz(L), [1] factor flops, [2] supernodes, [3] device bytes, [4] last factor ms,
- * [5] L words, [6] update-pool words, [7] levels; with params.profile set, accumulated since the last call with
- * reset: [8] MFMA update launches, [9] their ms, [10] their algorithmic flops, [11] triangular sweeps (fwd+bwd),
- * [12] their ms, [13] panel bytes they read, [14] factorizations, [15] factorization ms. */
-int ifea_step_info(ifea_step_t h, int reset, double* out16);
+/* Device and factor statistics (24 doubles): [0] nnz(L), [1] factor flops, [2] supernodes, [3] device bytes,
+ * [4] last factor ms, [5] L words, [6] update-pool words, [7] levels; with params.profile set, accumulated since the
+ * last call with reset: [8] inner (64-column) MFMA update launches, [9] their ms, [10] their algorithmic flops,
+ * [11] triangular sweeps (fwd+bwd), [12] their ms, [13] panel bytes they read, [14] factorizations,
+ * [15] factorization ms, [16] outer (256-column) MFMA update launches, [17] their ms, [18] their algorithmic flops,
+ * [19] factorizations done in FP32, [20] sweeps done with an FP32 factor, [21..23] the FP32 part of [16..18]. */
+int ifea_step_info(ifea_step_t h, int reset, double* out24);
 
 void ifea_step_destroy(ifea_step_t h);
 
