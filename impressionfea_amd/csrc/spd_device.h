@@ -124,7 +124,7 @@ class SpdDevice {
   const void* sweep_key_ = nullptr;
   size_t sweep_tsize_ = 0;
   void drop_graph();
-  DBuf<double> V_, W_, y_, Linv_, X_;
+  DBuf<double> V_, W_, y_, Linv_, X_, XT_;
   DBuf<int> err_;
   SymDev sd_{};
   bool profiling_ = false;

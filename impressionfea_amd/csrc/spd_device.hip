@@ -24,7 +24,9 @@ constexpr long long ZERO_TARGET = 1 << 16;  // update-block entries zeroed per w
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
 constexpr int SB = 256;         // diagonal block of the blocked sweeps (explicit inverse per block)
-constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step / of the backward GEMV
+constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step
+constexpr int GEMV_COLS = 4;    // columns per workgroup of the backward GEMV (one wave each)
+constexpr int BT = 16;          // loads a lane issues before using them in the sweep kernels
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -95,17 +97,33 @@ __global__ __launch_bounds__(256) void k_extend(const int4* __restrict__ tasks, 
   const int* rel = sd.rel + sd.rel_off[c];
   T* Lp = L + sd.Loff[p];
   T* Up = U + sd.Uoff[p];
-  for (int j = tk.y; j < tk.z; j++) {
-    int pc = rel[j];
+  // one wave per child column (four columns in flight per workgroup), each lane with four rows in flight
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = tk.y + wv; j < tk.z; j += 4) {
+    const int pc = rel[j];
     T* dcol = pc < kp ? Lp + (long long)pc * mp : Up + (long long)(pc - kp) * mpu - kp;
     const T* scol = Uc + (long long)j * mu;
-    for (int i = j + threadIdx.x; i < mu; i += blockDim.x) dcol[rel[i]] += scol[i];
+    for (int i0 = j + lane; i0 < mu; i0 += 256) {
+      int d[4];
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int i = i0 + 64 * u;
+        d[u] = i < mu ? rel[i] : -1;
+        v[u] = i < mu ? scol[i] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (d[u] >= 0) dcol[d[u]] += v[u];
+    }
   }
 }
 
-// Diagonal block of one 64-column step: FP64 Cholesky in one wave (lane i owns row i, columns broadcast through LDS),
-// then the inverse of the factor (lane c solves L x = e_c). The factor goes into the panel; the inverse is kept
-// (Lt[c][r] = Linv[r][c]) for the TRSM launch that follows and for the triangular sweeps. task: (supernode s, 0, 0, 0)
+// Diagonal block of one 64-column step: FP64 Cholesky in one wave, lane i owning row i of the block in LDS; the
+// factored column goes to a separate (double-buffered) LDS vector, so the row updates of a step are independent
+// loads and stores that pipeline. Then the inverse of the factor (lane c solves L x = e_c with the factor broadcast
+// from LDS). The factor goes into the panel; the inverse is kept (Lt[c][r] = Linv[r][c]) for the TRSM launch that
+// follows and for the triangular sweeps. Columns beyond the block width are the identity. task: (supernode s, 0, 0, 0)
 template <typename T>
 __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks, int c0, SymDev sd,
                                                   T* __restrict__ L, double* __restrict__ Linv,
@@ -117,54 +135,64 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   const int w = min(NB, k - c0);
   T* P = L + sd.Loff[s];
   __shared__ double D[NB][NB + 1];
+  __shared__ double colb[2][NB];
   const int i = threadIdx.x;
+  const T* prow = P + (long long)c0 * m + c0 + i;
   // the 64 column loads of the block are independent: issue them 16 at a time
   for (int j0 = 0; j0 < NB; j0 += 16) {
     double v[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) {
-      int j = j0 + t;
-      v[t] = (i < w && j < w && i >= j) ? (double)P[(long long)(c0 + j) * m + c0 + i] : 0.0;
+      const int j = j0 + t;
+      v[t] = (i < w && j < w && i >= j) ? (double)prow[(long long)j * m] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < 16; t++) {
-      int j = j0 + t;
+      const int j = j0 + t;
       D[i][j] = (i < w && j < w) ? v[t] : ((i == j) ? 1.0 : 0.0);
     }
   }
   wave_sync_lds();
   bool bad = false;
+  double* row = &D[i][0];
   for (int j = 0; j < NB; j++) {
-    double piv = D[j][j];
+    const double piv = D[j][j];
     bad |= !(piv > 0.0) || !isfinite(piv);
-    double d = sqrt(piv > 0.0 ? piv : 1.0);
-    double lij = (i > j) ? D[i][j] / d : 0.0;
+    const double d = sqrt(piv > 0.0 ? piv : 1.0);
+    const double lij = (i > j) ? row[j] / d : ((i == j) ? d : 0.0);
+    row[j] = lij;
+    colb[j & 1][i] = lij;
     wave_sync_lds();
-    if (i == j) D[i][j] = d;
-    else if (i > j) D[i][j] = lij;
-    wave_sync_lds();
-    for (int q = j + 1; q <= i; q++) D[i][q] -= lij * D[q][j];
+    const double* cb = colb[j & 1];
+#pragma unroll 8
+    for (int q = j + 1; q < NB; q++) row[q] -= lij * cb[q];
     wave_sync_lds();
   }
-  for (int q = i + 1; q < NB; q++) D[i][q] = 0.0;
+  for (int q = i + 1; q < NB; q++) row[q] = 0.0;
   wave_sync_lds();
   // factor into the panel (column q, row i)
-  for (int q = 0; q < w; q++)
-    if (i < w && i >= q) P[(long long)(c0 + q) * m + c0 + i] = (T)D[i][q];
+#pragma unroll 16
+  for (int q = 0; q < NB; q++)
+    if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)row[q];
   if (bad && i == 0) atomicOr(err, 1);
-  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; store Lt[p][q] = Linv[q][p]
-  double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
+  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]
   const int c = i;
   double x[NB];
 #pragma unroll
   for (int r = 0; r < NB; r++) {
-    double a = (r == c) ? 1.0 : 0.0;
+    double acc = (r == c) ? 1.0 : 0.0;
 #pragma unroll
-    for (int p = 0; p < r; p++) a -= D[r][p] * x[p];
-    x[r] = a / D[r][r];
+    for (int p = 0; p < r; p++) acc -= D[r][p] * x[p];
+    x[r] = acc / D[r][r];
   }
+  wave_sync_lds();
+  // transpose through LDS so that the stores of Lt[c][r] = Linv[r][c] are contiguous over the lanes
 #pragma unroll
-  for (int r = 0; r < NB; r++) Lt[c * NB + r] = x[r];  // Linv[r][c] -> Lt[c][r]
+  for (int r = 0; r < NB; r++) D[c][r] = x[r];
+  wave_sync_lds();
+  double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
+#pragma unroll 16
+  for (int t = 0; t < NB; t++) Lt[t * NB + i] = D[t][i];
 }
 
 // Rows below the diagonal block: X = A L11^{-T} as a 64x64x64 FP64 MFMA product per task, written back into the
@@ -294,15 +322,20 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
                                                   T* __restrict__ L, T* __restrict__ U) {
   constexpr int KC = 16;
   constexpr int PER = KC * 128 / 256;  // elements of each operand a thread stages per chunk
-  const int4 tk = tasks[blockIdx.x];
+  constexpr int LD = 144;              // LDS row stride: consecutive k rows start 32 banks apart (no conflicts)
+  // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous range of the task list (tiles
+  // of the same tile row and neighbouring columns), so the panel rows they share stay in that XCD's L2
+  const int nt = gridDim.x, xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int per_x = nt >> 3, extra = nt & 7;
+  const int4 tk = tasks[xcd * per_x + min(xcd, extra) + slot];
   const int s = tk.x;
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   const int kw = min(OB, k - kb);
   const long long rb = kb + kw;
   const T* P = L + sd.Loff[s];
-  __shared__ T Xi[2][KC][128 + 4];
-  __shared__ T Xj[2][KC][128 + 4];
+  __shared__ T Xi[2][KC][LD];
+  __shared__ T Xj[2][KC][LD];
   const long long i0 = rb + 128LL * tk.y, j0 = rb + 128LL * tk.z;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ib = (wv >> 1) * 64, jb = (wv & 1) * 64;
@@ -500,7 +533,8 @@ __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev
 // task: (s, c0, j, 0). Strips above the diagonal are stored as zeros.
 template <typename T>
 __global__ __launch_bounds__(256) void k_inv256(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
-                                                const double* __restrict__ Linv, double* __restrict__ X) {
+                                                const double* __restrict__ Linv, double* __restrict__ X,
+                                                double* __restrict__ XT) {
   __shared__ double Ts[NB][NB + 1];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y, j = tk.z;
@@ -510,13 +544,19 @@ __global__ __launch_bounds__(256) void k_inv256(const int4* __restrict__ tasks, 
   const int nsub = (wd + NB - 1) / NB;
   const T* P = L + sd.Loff[s];
   double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  double* XTb = XT + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
   const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
   const double* Ltj = Linv + sd.Ioff[s] + (long long)((c0 + NB * j) / NB) * NB * NB;
   for (int cc = 0; cc < 16; cc++) {
     const int c = 16 * q + cc;
     double* xc = Xb + (long long)(NB * j + c) * SB;
-    for (int i = 0; i < j; i++) xc[NB * i + r] = 0.0;
-    xc[NB * j + r] = Ltj[c * NB + r];
+    for (int i = 0; i < j; i++) {
+      xc[NB * i + r] = 0.0;
+      XTb[(long long)(NB * i + r) * SB + NB * j + c] = 0.0;
+    }
+    const double d = Ltj[c * NB + r];
+    xc[NB * j + r] = d;
+    XTb[(long long)(NB * j + r) * SB + NB * j + c] = d;
   }
   __syncthreads();
   for (int i = j + 1; i < nsub; i++) {
@@ -550,7 +590,10 @@ __global__ __launch_bounds__(256) void k_inv256(const int4* __restrict__ tasks, 
       for (int cc = 0; cc < 16; cc++) acc[cc] += a * Ts[pp][16 * q + cc];
     }
 #pragma unroll
-    for (int cc = 0; cc < 16; cc++) Xb[(long long)(NB * j + 16 * q + cc) * SB + NB * i + r] = -acc[cc];
+    for (int cc = 0; cc < 16; cc++) {
+      Xb[(long long)(NB * j + 16 * q + cc) * SB + NB * i + r] = -acc[cc];
+      XTb[(long long)(NB * i + r) * SB + NB * j + 16 * q + cc] = -acc[cc];
+    }
     __syncthreads();
   }
 }
@@ -574,11 +617,12 @@ __global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ ta
 }
 
 // Forward diagonal step: x_r = sum_c X_t[r][c] w_c for the 64 rows [c0 + 64 tk.z, ...) of the step; the waves split
-// the columns (only c <= r is nonzero) and the partial sums are combined in a fixed order. task: (s, c0, row block, 0)
+// the columns c < 64 (tk.z + 1) (X_t is lower triangular) and combine their partial sums in a fixed order. Loads are
+// issued 16 at a time. task: (s, c0, row block, 0)
 __global__ __launch_bounds__(256) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
                                                    const double* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  __shared__ double part[4][NB];
+  __shared__ double ws[SB + BT], part[4][NB];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y, rb = tk.z;
   const int f = sd.sn_first[s];
@@ -586,32 +630,35 @@ __global__ __launch_bounds__(256) void k_fwdb_diag(const int4* __restrict__ task
   const int wd = min(SB, k - c0);
   const double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
   const double* w = Wg + sd.Woff[s] + c0;
+  ws[threadIdx.x] = threadIdx.x < wd ? w[threadIdx.x] : 0.0;
+  if (threadIdx.x < BT) ws[SB + threadIdx.x] = 0.0;
+  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = NB * rb + lane;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  if (wv <= rb) {
-    const int cend = min(wd, NB * wv + NB);
-    int c = NB * wv;
-    for (; c + 4 <= cend; c += 4) {
-      a0 += Xb[(long long)c * SB + r] * w[c];
-      a1 += Xb[(long long)(c + 1) * SB + r] * w[c + 1];
-      a2 += Xb[(long long)(c + 2) * SB + r] * w[c + 2];
-      a3 += Xb[(long long)(c + 3) * SB + r] * w[c + 3];
-    }
-    for (; c < cend; c++) a0 += Xb[(long long)c * SB + r] * w[c];
+  const int cend = min(wd, NB * rb + NB);
+  const int per = (((cend + 3) / 4 + BT - 1) / BT) * BT;
+  const int lo = wv * per, hi = min(cend, lo + per);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int base = lo; base < hi; base += BT) {
+    double v[BT];
+#pragma unroll
+    for (int u = 0; u < BT; u++) v[u] = base + u < hi ? Xb[(long long)(base + u) * SB + r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < BT; u++) acc[u & 3] += v[u] * ws[base + u];
   }
-  part[wv][lane] = (a0 + a1) + (a2 + a3);
+  part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (threadIdx.x < NB && r < wd) y[f + c0 + r] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
-// Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 rows per workgroup (waves split the step's columns);
-// on the front's last step the rows are its update-vector rows and go straight to V. task: (s, c0, first row, last)
+// Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 rows per workgroup (waves split the step's columns,
+// loads issued 16 at a time); on the front's last step the rows are its update-vector rows and go straight to V.
+// task: (s, c0, first row, last)
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tasks, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
                                                     double* __restrict__ Wg, double* __restrict__ V) {
-  __shared__ double xs[SB], part[4][NB];
+  __shared__ double xs[SB + BT], part[4][NB];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
@@ -619,23 +666,23 @@ __global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tas
   const long long m = sn_m(sd, s);
   const int wd = min(SB, k - c0);
   xs[threadIdx.x] = threadIdx.x < wd ? y[f + c0 + threadIdx.x] : 0.0;
+  if (threadIdx.x < BT) xs[SB + threadIdx.x] = 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long row = (long long)tk.z + lane;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int q0 = NB * wv, q1 = min(wd, q0 + NB);
-  if (row < m && q0 < q1) {
+  if (row < m) {
     const T* pr = L + sd.Loff[s] + (long long)c0 * m + row;
-    int q = q0;
-    for (; q + 4 <= q1; q += 4) {
-      a0 += (double)pr[(long long)q * m] * xs[q];
-      a1 += (double)pr[(long long)(q + 1) * m] * xs[q + 1];
-      a2 += (double)pr[(long long)(q + 2) * m] * xs[q + 2];
-      a3 += (double)pr[(long long)(q + 3) * m] * xs[q + 3];
+    for (int base = q0; base < q1; base += BT) {
+      T v[BT];
+#pragma unroll
+      for (int u = 0; u < BT; u++) v[u] = base + u < q1 ? pr[(long long)(base + u) * m] : T(0);
+#pragma unroll
+      for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * xs[base + u];
     }
-    for (; q < q1; q++) a0 += (double)pr[(long long)q * m] * xs[q];
   }
-  part[wv][lane] = (a0 + a1) + (a2 + a3);
+  part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (threadIdx.x < NB && row < m) {
     double* W = Wg + sd.Woff[s];
@@ -657,61 +704,78 @@ __global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks,
   V[sd.Voff[s] + i - k] = y[sd.sn_rows[sd.sn_rowptr[s] + i]];
 }
 
-// z_q = y_own[q] - L21(:, q)^T xb for every own column q: one wave per column, 8 independent partial sums per lane.
-// task: (s, first column, end column, 0)
+// z_q = y_own[q] - L21(:, q)^T xb for the own columns q of this task: one wave per column, 16 loads of each operand
+// in flight per lane. task: (s, first column, end column, 0), at most 4 columns.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
                                                    const double* __restrict__ y, const double* __restrict__ V,
                                                    double* __restrict__ Wg) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = tk.y + wv;
+  if (q >= tk.z) return;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   const long long mb = m - k;
-  const T* P = L + sd.Loff[s];
+  const T* cq = L + sd.Loff[s] + (long long)q * m + k;
   const double* xb = V + sd.Voff[s];
-  double* z = Wg + sd.Woff[s];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int q = tk.y + wv; q < tk.z; q += 4) {
-    const T* cq = P + (long long)q * m + k;
-    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    long long i = lane;
-    for (; i + 448 < mb; i += 512) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long long base = lane; base < mb; base += 64 * BT) {
+    T v[BT];
+    double xv[BT];
 #pragma unroll
-      for (int u = 0; u < 8; u++) a[u] += (double)cq[i + 64 * u] * xb[i + 64 * u];
+    for (int u = 0; u < BT; u++) {
+      const long long i = base + 64 * u;
+      v[u] = i < mb ? cq[i] : T(0);
+      xv[u] = i < mb ? xb[i] : 0.0;
     }
-    for (; i < mb; i += 64) a[0] += (double)cq[i] * xb[i];
-    double acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-    acc = wave_sum(acc);
-    if (lane == 0) z[q] = y[f + q] - acc;
+#pragma unroll
+    for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * xv[u];
   }
+  const double a = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  if (lane == 0) Wg[sd.Woff[s] + q] = y[f + q] - a;
 }
 
-// Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...): one wave per
-// column, lanes over r >= c. task: (s, c0, column block, 0)
+// Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...), lane per
+// column over the transposed copy XT[r * SB + c] = X_t[r][c]; the waves split the rows r >= 64 tk.z.
+// task: (s, c0, column block, 0)
 __global__ __launch_bounds__(256) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
-                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const double* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
+  __shared__ double zs[SB + BT], part[4][NB];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y, cb = tk.z;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
-  const double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  const double* Xb = XT + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
   const double* z = Wg + sd.Woff[s] + c0;
+  zs[threadIdx.x] = threadIdx.x < wd ? z[threadIdx.x] : 0.0;
+  if (threadIdx.x < BT) zs[SB + threadIdx.x] = 0.0;
+  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c = NB * cb + wv; c < min(wd, NB * cb + NB); c += 4) {
-    const double* xc = Xb + (long long)c * SB;
-    double a = 0.0;
-    for (int r = NB * cb + lane; r < wd; r += NB) a += xc[r] * z[r];
-    a = wave_sum(a);
-    if (lane == 0) y[f + c0 + c] = a;
+  const int c = NB * cb + lane;
+  const int span = wd - NB * cb;
+  const int per = (((span + 3) / 4 + BT - 1) / BT) * BT;
+  const int lo = NB * cb + wv * per, hi = min(wd, lo + per);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int base = lo; base < hi; base += BT) {
+    double v[BT];
+#pragma unroll
+    for (int u = 0; u < BT; u++) v[u] = base + u < hi ? Xb[(long long)(base + u) * SB + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < BT; u++) acc[u & 3] += v[u] * zs[base + u];
   }
+  part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  __syncthreads();
+  if (threadIdx.x < NB && c < wd) y[f + c0 + c] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
-// Backward panel step: z_j -= L(c0 .. c0+wd, j)^T x_t for the columns j of this task (< c0): one wave per column,
-// lanes over the step's rows. task: (s, c0, first column, end column)
+// Backward panel step: z_j -= L(c0 .. c0+wd, j)^T x_t for the columns j of this task (< c0): wave wv takes columns
+// j0 + wv + 4u, lanes over the step's rows, all 16 loads of a lane issued before the reductions.
+// task: (s, c0, first column, end column), at most 16 columns.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwdb_panel(const int4* __restrict__ tasks, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
@@ -728,15 +792,22 @@ __global__ __launch_bounds__(256) void k_bwdb_panel(const int4* __restrict__ tas
   const T* P = L + sd.Loff[s];
   double* z = Wg + sd.Woff[s];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int j = tk.z + wv; j < tk.w; j += 4) {
+  T v[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int j = tk.z + wv + 4 * u;
     const T* cj = P + (long long)j * m + c0;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    if (lane < wd) a0 = (double)cj[lane] * xs[lane];
-    if (lane + 64 < wd) a1 = (double)cj[lane + 64] * xs[lane + 64];
-    if (lane + 128 < wd) a2 = (double)cj[lane + 128] * xs[lane + 128];
-    if (lane + 192 < wd) a3 = (double)cj[lane + 192] * xs[lane + 192];
-    const double acc = wave_sum((a0 + a1) + (a2 + a3));
-    if (lane == 0) z[j] -= acc;
+#pragma unroll
+    for (int t = 0; t < 4; t++) v[u][t] = (j < tk.w && lane + 64 * t < wd) ? cj[lane + 64 * t] : T(0);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int j = tk.z + wv + 4 * u;
+    double a = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; t++) a += (double)v[u][t] * xs[(lane + 64 * t) & (SB - 1)];
+    a = wave_sum(a);
+    if (lane == 0 && j < tk.w) z[j] -= a;
   }
 }
 
@@ -772,7 +843,7 @@ hipEvent_t SpdDevice::event(size_t i) {
 size_t SpdDevice::device_bytes() const {
   return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
          sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() + zero_tasks_.bytes() +
-         Linv_.bytes() + X_.bytes() + gptr_.bytes() + gsrc_.bytes() + fwd_tasks_.bytes() + bwd_tasks_.bytes();
+         Linv_.bytes() + X_.bytes() + XT_.bytes() + gptr_.bytes() + gsrc_.bytes() + fwd_tasks_.bytes() + bwd_tasks_.bytes();
 }
 
 void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* perm, const SymbolicOptions& opt) {
@@ -995,7 +1066,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
         bwt.push_back(make_int4(s, (int)r0, (int)std::min<long long>(sn_mm(s), r0 + 256), 0));
     L.bwd_xb.end = L.bwd_gemv.begin = (long long)bwt.size();
     for (int s : big)
-      for (int q = 0; q < sn_kk(s); q += SWEEP_COLS) bwt.push_back(make_int4(s, q, std::min(sn_kk(s), q + SWEEP_COLS), 0));
+      for (int q = 0; q < sn_kk(s); q += GEMV_COLS) bwt.push_back(make_int4(s, q, std::min(sn_kk(s), q + GEMV_COLS), 0));
     L.bwd_gemv.end = (long long)bwt.size();
     for (int c0 = ((maxk - 1) / SB) * SB; maxk > 0 && c0 >= 0; c0 -= SB) {
       LaunchList d, pn;
@@ -1033,6 +1104,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   Linv_.alloc(std::max<long long>(1, Ioff[ns]));
   Xoff_.upload(Xoff, stream_);
   X_.alloc(std::max<long long>(1, Xoff[ns]));
+  XT_.alloc(std::max<long long>(1, Xoff[ns]));
   Goff_.upload(Goff, stream_);
   if (gptr.empty()) gptr_.alloc(1); else gptr_.upload(gptr, stream_);
   if (gsrc.empty()) gsrc_.alloc(1); else gsrc_.upload(gsrc, stream_);
@@ -1117,7 +1189,7 @@ int SpdDevice::factorize_t(const double* d_vals) {
   }
   if (inv_launch_.end > inv_launch_.begin)
     hipLaunchKernelGGL(k_inv256<T>, dim3((unsigned)(inv_launch_.end - inv_launch_.begin)), dim3(256), 0, stream_,
-                       inv_tasks_.p, sd_, L, Linv_.p, X_.p);
+                       inv_tasks_.p, sd_, L, Linv_.p, X_.p, XT_.p);
   IFEA_HIP(hipGetLastError());
   int herr = 0;
   IFEA_HIP(hipMemcpyAsync(&herr, err_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -1206,7 +1278,7 @@ void SpdDevice::sweeps(const T* L) {
     launch(k_bwdb_xb, lv.bwd_xb, bwd_tasks_.p, (const double*)y_.p, V_.p);
     launch(k_bwdb_gemv<T>, lv.bwd_gemv, bwd_tasks_.p, L, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.bwd_diag.size(); t++) {
-      launch(k_bwdb_diag, lv.bwd_diag[t], bwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
+      launch(k_bwdb_diag, lv.bwd_diag[t], bwd_tasks_.p, (const double*)XT_.p, y_.p, (const double*)W_.p);
       launch(k_bwdb_panel<T>, lv.bwd_panel[t], bwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
   }

@@ -155,8 +155,9 @@ struct ND {
   }
 };
 
-// Geometric nested dissection: cut the subset's bounding box at the median of its longest axis; the separator is the
-// layer of the upper half adjacent to the lower half (every edge between the halves ends in it).
+// Geometric nested dissection: cut the subset along its longest axis near the median (in a wide coordinate gap of the
+// middle tenth); the separator is the part of the upper half adjacent to the lower half (every edge between the halves
+// ends in it).
 struct GeoND {
   const Graph& g;
   const std::vector<double>& xyz;  // 3 per block
@@ -177,9 +178,22 @@ struct GeoND {
     for (int a = 1; a < 3; a++) if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
     std::vector<std::pair<double, int>> key(V.size());
     for (size_t q = 0; q < V.size(); q++) key[q] = {xyz[3 * V[q] + ax], V[q]};
-    size_t mid = V.size() / 2;
-    std::nth_element(key.begin(), key.begin() + mid, key.end());
-    // A = strictly below the median element (by (coord, id)), B = the rest
+    std::sort(key.begin(), key.end());
+    // cut in a gap between consecutive coordinates near the median: on a layered mesh this falls between two node
+    // layers, so the separator is the coupling depth (two layers for the two-ring pattern) rather than a split layer
+    // plus that depth. Among the gaps of the middle tenth at least half as wide as the widest, the one nearest the
+    // median is taken.
+    const size_t n = V.size();
+    size_t mid = n / 2;
+    const size_t q0 = std::max<size_t>(1, (9 * n) / 20), q1 = std::min(n - 1, (11 * n) / 20);
+    double widest = 0.0;
+    for (size_t q = q0; q <= q1; q++) widest = std::max(widest, key[q].first - key[q - 1].first);
+    size_t bestd = n;
+    for (size_t q = q0; q <= q1; q++) {
+      const size_t d = q > n / 2 ? q - n / 2 : n / 2 - q;
+      if (key[q].first - key[q - 1].first >= 0.5 * widest && d < bestd) { bestd = d; mid = q; }
+    }
+    // A = the elements below the cut, B = the rest
     for (size_t q = 0; q < V.size(); q++) mark[key[q].second] = q < mid ? 1 : 2;
     std::vector<int> A, B, S;
     for (size_t q = 0; q < V.size(); q++) {

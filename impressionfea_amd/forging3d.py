@@ -139,6 +139,7 @@ class Forging3DSolver:
         self.ds = cfg.stroke_increment
         self.warm = False
         self.step_index = 0
+        self.n_converged = 0     # converged velocity fields so far (FinalEvaluation calls)
         self.time = 0.0
         self.warm_start = warm_start
         self.vg_prev = None
@@ -306,6 +307,7 @@ class Forging3DSolver:
                                             N.ptr(vg, C.c_double), None, N.ptr(rel, C.c_ubyte)), "FinalEvaluation")
         self.vg_prev = self.vg
         self.vg = vg
+        self.n_converged += 1
         self.release = rel.astype(bool)
         return pw.value, rc.value
 
@@ -409,7 +411,24 @@ class Forging3DSolver:
         vv = N.f64(v) if v is not None else None
         self._check(N.lib().ifea_step_assemble(self._h, int(matrix), int(newton), N.ptr(vv, C.c_double), C.byref(e),
                                                C.byref(pw), N.ptr(r, C.c_double), N.ptr(val, C.c_double)), "Assemble")
+        # the assembly overwrote the device's element flow stresses; the next PrepareStep restores the increment's
+        # own (host copies) before it computes the friction coefficients from them
+        self._state_dirty = True
         return e.value, pw.value, r, val
+
+    def linear_system(self, newton: bool = True):
+        """The Newton/Picard system of the current velocity as SolveVelocity forms it (Forging3DSolver.cs:965-1007):
+        tangent and residual from Assemble, fixed local DOFs replaced by identity rows/columns, 1e-10 * max diagonal
+        added to the free diagonal; returns (values in the CSC pattern, right-hand side -r)."""
+        _, _, r, val = self.assemble(None, matrix=True, newton=newton)
+        fixed = (np.arange(3)[None, :] < self.nfix[:, None]).ravel()
+        col = np.repeat(np.arange(self.ndof), np.diff(self.colPtr))
+        row = self.rowIdx
+        msk = fixed[row] | fixed[col]
+        val[msk] = np.where(row[msk] == col[msk], 1.0, 0.0)
+        dg = (row == col) & ~fixed[col]
+        val[dg] += 1e-10 * val[dg].max()
+        return val, -np.where(fixed, 0.0, r)
 
     def get_v(self):
         v = np.zeros(self.ndof)

@@ -129,12 +129,14 @@ class MlpDevice:
 
 
 def node_features(solver):
-    """Per-node inputs at the start of an increment (before PrepareStep), from the solver's host state."""
+    """Per-node inputs at the start of an increment (before PrepareStep), from the solver's host state: coordinates,
+    the last two converged nodal velocities, the contact flag and the distance to the nearer die surface (11 values,
+    lengths scaled by the billet size and velocities by the ram speed)."""
     cfg = solver.cfg
     L = cfg.size
     x = solver.x
     vg = solver.vg
-    vp = solver.vg_prev if solver.vg_prev is not None else solver.vg
+    vp = solver.vg_prev if solver.n_converged >= 2 else solver.vg
     contact = (solver.contact != 0).astype(np.float64)
     cb = cfg.bottom.clearance(x, 0.0)
     ct = cfg.top.clearance(x, solver.gap)
@@ -143,22 +145,38 @@ def node_features(solver):
     return f.astype(np.float32)
 
 
-class LearnedWarmStart:
-    """Warm-start callback for Forging3DSolver: v_guess = v_n + ram * net(features)."""
+def trend_start(solver):
+    """The trend start 2 v_n - v_{n-1} (ExtrapolateStart, Forging3DSolver.cs:856-859) once two converged increments
+    exist (before that, the previous increment's field)."""
+    if solver.n_converged < 2:
+        return solver.vg.copy()
+    return 2.0 * solver.vg - solver.vg_prev
 
-    def __init__(self, params):
+
+def target(base, vg_after, ram, scale):
+    """Training target of one increment: the converged velocity's deviation from the start `base` (the trend start),
+    in units of scale * ram."""
+    return ((vg_after - base) / (ram * scale)).astype(np.float32)
+
+
+class LearnedWarmStart:
+    """Warm-start callback for Forging3DSolver: v_guess = (2 v_n - v_{n-1}) + scale * ram * net(features), i.e. the
+    network predicts the correction to the trend start."""
+
+    def __init__(self, params, scale=1.0):
         self.params = np.asarray(params, np.float32)
+        self.scale = float(scale)
         self.dev = MlpDevice(self.params)
 
     @classmethod
     def load(cls, path):
         with np.load(path, allow_pickle=False) as z:
-            return cls(z["params"])
+            return cls(z["params"], float(z["scale"]) if "scale" in z.files else 1.0)
 
     def save(self, path):
         os.makedirs(os.path.dirname(path), exist_ok=True)
-        np.savez(path, params=self.dev.get_params())
+        np.savez(path, params=self.dev.get_params(), scale=np.float64(self.scale))
 
     def __call__(self, solver):
         dv = self.dev.forward(node_features(solver)).astype(np.float64)
-        return solver.vg + solver.cfg.ram * dv
+        return trend_start(solver) + self.scale * solver.cfg.ram * dv

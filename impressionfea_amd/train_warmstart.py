@@ -3,7 +3,8 @@
 Trajectories: cylinder upsets between flat dies on the 10x10x10-cell Kuhn mesh with the upset flow law, one billet
 per seed with D ~ U[25, 35] mm, H/D ~ U[1.2, 1.8] and shear friction ~ U[0.2, 0.6]; 20 increments of 0.5 mm each.
 Every increment after the first gives one sample per node: the features at the start of the increment and the
-change of the converged nodal velocity over it. Seeds 0-15 train, seeds 100-103 are held out.
+converged nodal velocity's deviation from the trend start 2 v_n - v_{n-1} (the network learns the correction to the
+trend, in units of the targets' RMS). Seeds 0-15 train, seeds 100-103 are held out.
 
     python -m impressionfea_amd.train_warmstart --train --evaluate
 
@@ -22,7 +23,7 @@ import numpy as np
 
 from .config import upset_small
 from .forging3d import Forging3DSolver
-from .mlp import MlpDevice, LearnedWarmStart, init_params, node_features
+from .mlp import MlpDevice, LearnedWarmStart, init_params, node_features, target, trend_start
 from .warmstart import DEFAULT_WEIGHTS
 
 TRAIN_SEEDS = list(range(16))
@@ -42,18 +43,20 @@ def billet(seed: int):
 
 
 def trajectory(wl, increments=20, warm_start=None, precision=0):
+    """One billet stepped by the solver; every increment after the first yields (features at its start, converged
+    velocity deviation from the trend start in units of the ram speed)."""
     s = Forging3DSolver.from_workload(wl, precision=precision, warm_start=warm_start)
     feats, targets, its, loads, starts = [], [], [], [], []
     for k in range(increments):
         f = node_features(s) if k >= 1 else None
-        v0 = s.vg.copy()
+        base = trend_start(s)
         r = s.step()
         its.append(r.iterations)
         loads.append(r.load)
         starts.append(r.stats.get("start", "previous"))
         if f is not None:
             feats.append(f)
-            targets.append(((s.vg - v0) / wl.cfg.ram).astype(np.float32))
+            targets.append(target(base, s.vg, wl.cfg.ram, 1.0))
     s.close()
     return feats, targets, its, loads, starts
 
@@ -71,7 +74,10 @@ def build_dataset(seeds, increments=20, log=print):
 
 def train(seeds=TRAIN_SEEDS, epochs=60, batch=4096, lr=2e-3, seed=0, out=DEFAULT_WEIGHTS, log=print):
     X, Y = build_dataset(seeds, log=log)
-    log(f"dataset {X.shape[0]} rows")
+    # targets are O(1e-3) of the ram speed: train on them in units of their RMS so the BF16 network sees O(1) values
+    scale = float(np.sqrt(np.mean(Y.astype(np.float64) ** 2))) or 1.0
+    Y = (Y / scale).astype(np.float32)
+    log(f"dataset {X.shape[0]} rows, target scale {scale:.3e} x ram")
     net = MlpDevice(init_params(seed))
     net.set_dataset(X, Y)
     rng = np.random.default_rng(seed)
@@ -83,9 +89,9 @@ def train(seeds=TRAIN_SEEDS, epochs=60, batch=4096, lr=2e-3, seed=0, out=DEFAULT
         if ep % 10 == 0 or ep == epochs - 1:
             log(f"  epoch {ep}: loss {loss:.4e}")
     os.makedirs(os.path.dirname(out), exist_ok=True)
-    np.savez(out, params=net.get_params(), loss=np.array(hist, np.float32))
+    np.savez(out, params=net.get_params(), loss=np.array(hist, np.float32), scale=np.float64(scale))
     base = float(np.mean(Y.astype(np.float64) ** 2))
-    log(f"saved {out}; final loss {hist[-1]:.4e} vs predicting no change {base:.4e}")
+    log(f"saved {out}; final loss {hist[-1]:.4e} vs the plain trend start {base:.4e}")
     return hist, base
 
 

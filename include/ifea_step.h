@@ -73,7 +73,9 @@ int ifea_step_prepare(ifea_step_t h, const double* x, const double* ax, const in
 int ifea_step_set_element_state(ifea_step_t h, const double* eps, const double* sigma);
 
 /* One Assemble call at the given local-frame velocity v[3nn] (or the current one if NULL): returns Pi and power;
- * r[3nn] / val[nnz] are copied out when not NULL (matrix=1 assembles the tangent). Used by parity tests. */
+ * r[3nn] / val[nnz] are copied out when not NULL (matrix=1 assembles the tangent). Used by parity tests. Like the
+ * reference's Kernel (Forging3DSolver.cs:1351-1356) it overwrites the per-element flow stress and strain rate, which
+ * the next ifea_step_prepare reads for the friction coefficients: restore them with ifea_step_set_element_state. */
 int ifea_step_assemble(ifea_step_t h, int matrix, int newton, const double* v, double* energy, double* power,
                        double* r, double* val);
 
