@@ -23,10 +23,10 @@ constexpr int EXT_TARGET = 8192;
 constexpr long long ZERO_TARGET = 1 << 16;  // update-block entries zeroed per workgroup
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
-constexpr int SB = 256;         // diagonal block of the blocked sweeps (explicit inverse per block)
+constexpr int SB = 512;         // diagonal block of the blocked sweeps (explicit inverse per block)
 constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step
 constexpr int GEMV_COLS = 4;    // columns per workgroup of the backward GEMV (one wave each)
-constexpr int BT = 16;          // loads a lane issues before using them in the sweep kernels
+constexpr int BT = 32;          // loads a lane issues before using them in the sweep kernels
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -523,16 +523,16 @@ __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev
 
 
 // ---------------------------------------------------------------------------------------- blocked sweeps (wide fronts)
-// Wide fronts are swept in steps of SB = 256 columns. Each step is two launches: a diagonal launch x_t = X_t w_t with
-// X_t the explicit inverse of the step's 256x256 diagonal block (built once per factorization from the kept 64x64
-// inverses, k_inv256), and a panel launch that applies the step to every other row (forward) or column (backward) of
+// Wide fronts are swept in steps of SB = 512 columns. Each step is two launches: a diagonal launch x_t = X_t w_t with
+// X_t the explicit inverse of the step's SB x SB diagonal block (built once per factorization from the kept 64x64
+// inverses, k_inv_diag), and a panel launch that applies the step to every other row (forward) or column (backward) of
 // the front with many workgroups. X_t is stored column by column: X[c * SB + r] = inv(L_tt)[r][c].
 
-// Inverse of one 256-column diagonal block, one 64-column strip j per workgroup: X_jj = Linv_j and, going down,
+// Inverse of one SB-column diagonal block, one 64-column strip j per workgroup: X_jj = Linv_j and, going down,
 // X_ij = -Linv_i * sum_{p=j}^{i-1} L_ip X_pj. Thread (r, q): row r of the 64-row tile, columns [16 q, 16 q + 16).
 // task: (s, c0, j, 0). Strips above the diagonal are stored as zeros.
 template <typename T>
-__global__ __launch_bounds__(256) void k_inv256(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
+__global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
                                                 const double* __restrict__ Linv, double* __restrict__ X,
                                                 double* __restrict__ XT) {
   __shared__ double Ts[NB][NB + 1];
@@ -601,10 +601,9 @@ __global__ __launch_bounds__(256) void k_inv256(const int4* __restrict__ tasks, 
 // Forward, first part: W_s = [y_own ; 0] plus the children's update vectors, one thread per row; the contributions
 // of a row are listed in child order (row-wise inverse of the children's relative maps), so the sum is fixed-order.
 // task: (s, first row, end row, 0)
-__global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ tasks, SymDev sd,
+__device__ __forceinline__ void fwdb_gather_item(const int4 tk, SymDev sd,
                                                      const double* __restrict__ y, const double* __restrict__ V,
                                                      double* __restrict__ Wg) {
-  const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const long long i = (long long)tk.y + threadIdx.x;
   if (i >= tk.z) return;
@@ -618,20 +617,18 @@ __global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ ta
 
 // Forward diagonal step: x_r = sum_c X_t[r][c] w_c for the 64 rows [c0 + 64 tk.z, ...) of the step; the waves split
 // the columns c < 64 (tk.z + 1) (X_t is lower triangular) and combine their partial sums in a fixed order. Loads are
-// issued 16 at a time. task: (s, c0, row block, 0)
-__global__ __launch_bounds__(256) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+// issued BT at a time. task: (s, c0, row block, 0)
+__device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
                                                    const double* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
   __shared__ double ws[SB + BT], part[4][NB];
-  const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y, rb = tk.z;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
   const double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
   const double* w = Wg + sd.Woff[s] + c0;
-  ws[threadIdx.x] = threadIdx.x < wd ? w[threadIdx.x] : 0.0;
-  if (threadIdx.x < BT) ws[SB + threadIdx.x] = 0.0;
+  for (int q = threadIdx.x; q < SB + BT; q += 256) ws[q] = q < wd ? w[q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = NB * rb + lane;
@@ -652,26 +649,24 @@ __global__ __launch_bounds__(256) void k_fwdb_diag(const int4* __restrict__ task
 }
 
 // Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 rows per workgroup (waves split the step's columns,
-// loads issued 16 at a time); on the front's last step the rows are its update-vector rows and go straight to V.
+// loads issued BT at a time); on the front's last step the rows are its update-vector rows and go straight to V.
 // task: (s, c0, first row, last)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tasks, SymDev sd,
+__device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
                                                     double* __restrict__ Wg, double* __restrict__ V) {
   __shared__ double xs[SB + BT], part[4][NB];
-  const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   const int wd = min(SB, k - c0);
-  xs[threadIdx.x] = threadIdx.x < wd ? y[f + c0 + threadIdx.x] : 0.0;
-  if (threadIdx.x < BT) xs[SB + threadIdx.x] = 0.0;
+  for (int q = threadIdx.x; q < SB + BT; q += 256) xs[q] = q < wd ? y[f + c0 + q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long row = (long long)tk.z + lane;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int q0 = NB * wv, q1 = min(wd, q0 + NB);
+  const int q0 = (SB / 4) * wv, q1 = min(wd, q0 + SB / 4);
   if (row < m) {
     const T* pr = L + sd.Loff[s] + (long long)c0 * m + row;
     for (int base = q0; base < q1; base += BT) {
@@ -694,9 +689,8 @@ __global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tas
 
 // Backward, first part: the solution at a wide front's lower rows, gathered into one contiguous vector (the front's
 // forward update-vector slot, free once the forward sweep is done). task: (s, first row, end row, 0), rows >= k.
-__global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks, SymDev sd,
+__device__ __forceinline__ void bwdb_xb_item(const int4 tk, SymDev sd,
                                                  const double* __restrict__ y, double* __restrict__ V) {
-  const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const long long i = (long long)tk.y + threadIdx.x;
   if (i >= tk.z) return;
@@ -704,13 +698,12 @@ __global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks,
   V[sd.Voff[s] + i - k] = y[sd.sn_rows[sd.sn_rowptr[s] + i]];
 }
 
-// z_q = y_own[q] - L21(:, q)^T xb for the own columns q of this task: one wave per column, 16 loads of each operand
+// z_q = y_own[q] - L21(:, q)^T xb for the own columns q of this task: one wave per column, BT loads of each operand
 // in flight per lane. task: (s, first column, end column, 0), at most 4 columns.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
+__device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T* __restrict__ L,
                                                    const double* __restrict__ y, const double* __restrict__ V,
                                                    double* __restrict__ Wg) {
-  const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = tk.y + wv;
@@ -741,19 +734,17 @@ __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ task
 // Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...), lane per
 // column over the transposed copy XT[r * SB + c] = X_t[r][c]; the waves split the rows r >= 64 tk.z.
 // task: (s, c0, column block, 0)
-__global__ __launch_bounds__(256) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+__device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
                                                    const double* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
   __shared__ double zs[SB + BT], part[4][NB];
-  const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y, cb = tk.z;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
   const double* Xb = XT + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
   const double* z = Wg + sd.Woff[s] + c0;
-  zs[threadIdx.x] = threadIdx.x < wd ? z[threadIdx.x] : 0.0;
-  if (threadIdx.x < BT) zs[SB + threadIdx.x] = 0.0;
+  for (int q = threadIdx.x; q < SB + BT; q += 256) zs[q] = q < wd ? z[q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c = NB * cb + lane;
@@ -774,41 +765,87 @@ __global__ __launch_bounds__(256) void k_bwdb_diag(const int4* __restrict__ task
 }
 
 // Backward panel step: z_j -= L(c0 .. c0+wd, j)^T x_t for the columns j of this task (< c0): wave wv takes columns
-// j0 + wv + 4u, lanes over the step's rows, all 16 loads of a lane issued before the reductions.
+// j0 + wv + 4u, lanes over the step's rows, all loads of a lane (4 columns x SB/64 rows) issued before the
+// reductions.
 // task: (s, c0, first column, end column), at most 16 columns.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwdb_panel(const int4* __restrict__ tasks, SymDev sd,
+__device__ __forceinline__ void bwdb_panel_item(const int4 tk, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
                                                     double* __restrict__ Wg) {
   __shared__ double xs[SB];
-  const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   const int wd = min(SB, k - c0);
-  xs[threadIdx.x] = threadIdx.x < wd ? y[f + c0 + threadIdx.x] : 0.0;
+  for (int q = threadIdx.x; q < SB; q += 256) xs[q] = q < wd ? y[f + c0 + q] : 0.0;
   __syncthreads();
   const T* P = L + sd.Loff[s];
   double* z = Wg + sd.Woff[s];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  T v[4][4];
+  constexpr int NC = SB / 64;  // row chunks of a step
+  T v[4][NC];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int j = tk.z + wv + 4 * u;
     const T* cj = P + (long long)j * m + c0;
 #pragma unroll
-    for (int t = 0; t < 4; t++) v[u][t] = (j < tk.w && lane + 64 * t < wd) ? cj[lane + 64 * t] : T(0);
+    for (int t = 0; t < NC; t++) v[u][t] = (j < tk.w && lane + 64 * t < wd) ? cj[lane + 64 * t] : T(0);
   }
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int j = tk.z + wv + 4 * u;
     double a = 0.0;
 #pragma unroll
-    for (int t = 0; t < 4; t++) a += (double)v[u][t] * xs[(lane + 64 * t) & (SB - 1)];
+    for (int t = 0; t < NC; t++) a += (double)v[u][t] * xs[lane + 64 * t];
     a = wave_sum(a);
     if (lane == 0 && j < tk.w) z[j] -= a;
   }
+}
+
+// one launch per step and kind; the stream orders them
+__global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ tasks, SymDev sd,
+                                                     const double* __restrict__ y, const double* __restrict__ V,
+                                                     double* __restrict__ Wg) {
+  fwdb_gather_item(tasks[blockIdx.x], sd, y, V, Wg);
+}
+
+__global__ __launch_bounds__(256) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const double* __restrict__ Wg) {
+  fwdb_diag_item(tasks[blockIdx.x], sd, X, y, Wg);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tasks, SymDev sd,
+                                                    const T* __restrict__ L, const double* __restrict__ y,
+                                                    double* __restrict__ Wg, double* __restrict__ V) {
+  fwdb_panel_item<T>(tasks[blockIdx.x], sd, L, y, Wg, V);
+}
+
+__global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks, SymDev sd,
+                                                 const double* __restrict__ y, double* __restrict__ V) {
+  bwdb_xb_item(tasks[blockIdx.x], sd, y, V);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
+                                                   const double* __restrict__ y, const double* __restrict__ V,
+                                                   double* __restrict__ Wg) {
+  bwdb_gemv_item<T>(tasks[blockIdx.x], sd, L, y, V, Wg);
+}
+
+__global__ __launch_bounds__(256) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+                                                   const double* __restrict__ XT, double* __restrict__ y,
+                                                   const double* __restrict__ Wg) {
+  bwdb_diag_item(tasks[blockIdx.x], sd, XT, y, Wg);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwdb_panel(const int4* __restrict__ tasks, SymDev sd,
+                                                    const T* __restrict__ L, const double* __restrict__ y,
+                                                    double* __restrict__ Wg) {
+  bwdb_panel_item<T>(tasks[blockIdx.x], sd, L, y, Wg);
 }
 
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
@@ -1109,6 +1146,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   if (gptr.empty()) gptr_.alloc(1); else gptr_.upload(gptr, stream_);
   if (gsrc.empty()) gsrc_.alloc(1); else gsrc_.upload(gsrc, stream_);
   if (invt.empty()) inv_tasks_.alloc(1); else inv_tasks_.upload(invt, stream_);
+
   rel_off_.upload(S.rel_off, stream_);
   if (S.rel.empty()) rel_.alloc(1); else rel_.upload(S.rel, stream_);
   child_ptr_.upload(cptr, stream_);
@@ -1188,7 +1226,7 @@ int SpdDevice::factorize_t(const double* d_vals) {
     }
   }
   if (inv_launch_.end > inv_launch_.begin)
-    hipLaunchKernelGGL(k_inv256<T>, dim3((unsigned)(inv_launch_.end - inv_launch_.begin)), dim3(256), 0, stream_,
+    hipLaunchKernelGGL(k_inv_diag<T>, dim3((unsigned)(inv_launch_.end - inv_launch_.begin)), dim3(256), 0, stream_,
                        inv_tasks_.p, sd_, L, Linv_.p, X_.p, XT_.p);
   IFEA_HIP(hipGetLastError());
   int herr = 0;

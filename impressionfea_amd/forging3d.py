@@ -275,23 +275,26 @@ class Forging3DSolver:
         self._check(rc, "PrepareStep")
 
     def _learned_start(self) -> str:
-        """PrepareStep with the learned guess, judged against the previous-increment start by the convex functional
-        Pi (the solver's own energy): the lower-energy start is kept, so a poor prediction costs one residual-only
-        assembly and never a worse start."""
+        """PrepareStep with the learned guess, judged against the reference's own starts (the previous increment's
+        field and its trend 2 v_n - v_{n-1}, ExtrapolateStart) by the convex functional Pi, the solver's own energy:
+        the lowest-energy start is kept, so a poor prediction costs residual-only assemblies and never a worse start
+        than the best of the reference's."""
+        from .mlp import trend_start
         guess = self.warm_start(self)
         self.prepare(guess)
-        v_learned = self.get_v()
-        e_learned = self.assemble(None, matrix=False, newton=False)[0]
-        v_prev = np.einsum("nij,nj->ni", self.ax, self.vg).ravel()
-        nf = self.nfix
-        fixed = np.arange(3)[None, :] < nf[:, None]
-        v_prev[fixed.ravel()] = self.fixed_val[fixed.ravel()]
-        e_prev = self.assemble(v_prev, matrix=False, newton=False)[0]
-        if e_learned < e_prev:
-            self.set_v(v_learned)
-            return "learned"
-        self.set_v(v_prev)
-        return "previous"
+        cands = {"learned": self.get_v()}
+        fixed = (np.arange(3)[None, :] < self.nfix[:, None]).ravel()
+        for name, vg in (("previous", self.vg), ("trend", trend_start(self))):
+            v = np.einsum("nij,nj->ni", self.ax, vg).ravel()
+            v[fixed] = self.fixed_val[fixed]
+            cands[name] = v
+        best, best_e = None, np.inf
+        for name, v in cands.items():
+            e = self.assemble(v, matrix=False, newton=False)[0]
+            if e < best_e:
+                best, best_e = name, e
+        self.set_v(cands[best])
+        return best
 
     def solve_velocity(self):
         st = N.StepStats()

@@ -96,7 +96,8 @@ def train(seeds=TRAIN_SEEDS, epochs=60, batch=4096, lr=2e-3, seed=0, out=DEFAULT
 
 
 def evaluate(seeds=TEST_SEEDS, weights=DEFAULT_WEIGHTS, increments=20, log=print):
-    """Held-out comparison: Newton iterations per increment and ram load vs the previous-increment start."""
+    """Held-out comparison: Newton iterations per increment and ram load against the previous-increment start (the
+    reference's default), the trend start (its ExtrapolateStart option) and the energy judge without the network."""
     ws = LearnedWarmStart.load(weights)
     rows = []
     for sd in seeds:
@@ -105,8 +106,12 @@ def evaluate(seeds=TEST_SEEDS, weights=DEFAULT_WEIGHTS, increments=20, log=print
         wl_trend = replace(wl, cfg=replace(wl.cfg, extrapolate_start=True))
         _, _, it1, l1, _ = trajectory(wl_trend, increments)
         _, _, it2, l2, st2 = trajectory(wl, increments, warm_start=ws)
+        # the same energy judge with the network replaced by the trend start: what the judge alone gains
+        _, _, it3, l3, st3 = trajectory(wl, increments, warm_start=trend_start)
         row = dict(seed=sd, iterations_previous=int(np.sum(it0[1:])), iterations_trend=int(np.sum(it1[1:])),
+                   iterations_judged_without_network=int(np.sum(it3[1:])),
                    iterations_learned=int(np.sum(it2[1:])), learned_starts=int(sum(s == "learned" for s in st2)),
+                   trend_starts=int(sum(s == "trend" for s in st2)),
                    max_load_rel_diff_learned=float(np.max(np.abs(np.array(l2) - l0) / np.abs(l0))),
                    max_load_rel_diff_trend=float(np.max(np.abs(np.array(l1) - l0) / np.abs(l0))))
         log(f"  held-out seed {sd}: {json.dumps(row)}")
@@ -125,7 +130,9 @@ def main():
         train(epochs=a.epochs, out=a.out)
     if a.evaluate:
         rows = evaluate(weights=a.out)
-        tot = {k: sum(r[k] for r in rows) for k in ("iterations_previous", "iterations_trend", "iterations_learned")}
+        tot = {k: sum(r[k] for r in rows) for k in ("iterations_previous", "iterations_trend",
+                                                     "iterations_judged_without_network", "iterations_learned",
+                                                     "learned_starts", "trend_starts")}
         print(json.dumps({"held_out": rows, "totals": tot}))
 
 

@@ -219,6 +219,22 @@ def test_increments_parity_fp64(oracle_lib):
         assert np.abs(s.x - o.state()["x"]).max() <= 1e-6 * np.abs(s.x).max()
 
 
+def test_increments_parity_config1(oracle_lib):
+    """The small upset exactly as configured (10x10x10 cells, penalty 1e6, nonlinear tolerance 1e-6): three increments
+    of the GPU FP64 path against the oracle with MKL PARDISO (or its own Cholesky), ram load within 1e-6."""
+    import oracle as O
+    wl = upset_small()
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=0)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0, linear=1 if O.pardiso_available() else 0)
+    o.initial(0.0)
+    s = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0)
+    s.initial(0.0)
+    for k in range(3):
+        ro, rs = o.step(), s.step()
+        assert rs.converged
+        assert rs.load == pytest.approx(ro["load"], rel=1e-6)
+
+
 def test_frictionless_upset_gpu():
     """Solver3DTests.cs:30-49 on the GPU path: F = sigma(rate) A within 1 %."""
     wl = upset_small(nxy=6, nz=6, friction=0.0, tolerance=1e-8)
