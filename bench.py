@@ -7,10 +7,11 @@ Two arms run from the same initial state:
   mixed  FP32 MFMA supernode updates + FP64 POTRF/residual/PCG refinement (FP64 fallback on stall), with the learned
          warm start when a trained network is available, else the previous-increment start
 A "step" is one stroke increment: PrepareStep (frames on the host, geometry/ANP/friction on the GPU) + SolveVelocity +
-FinalEvaluation + strain update; the host geometry update (TryUpdate) is identical in both arms and reported apart.
-`value` is the mixed arm's increments/s; the fp64 arm, the load error between the arms, the roofline of the dominant
-kernel (FP64 or FP32 MFMA supernode update, HIP events on the library's stream) and a host CPU baseline (the FP64
-oracle with MKL PARDISO on the host cores, on a bounded sample) are reported beside it.
+FinalEvaluation + strain update, followed by the host geometry update (TryUpdate), which is identical in both arms.
+`value` is the mixed arm's increments per second of wall clock over the timed steps (TryUpdate included; the rate
+without it is reported too); the fp64 arm, the load error between the arms, the roofline of the dominant kernel (the
+outer MFMA supernode update, HIP events around every launch on the library's stream) and a host CPU baseline (the
+FP64 oracle with MKL PARDISO on the host cores, on a bounded sample) are reported beside it.
 """
 from __future__ import annotations
 
@@ -52,8 +53,8 @@ def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
         stats.append(r.stats)
         log(f"  step {k}: load {r.load:.6e} N it {r.iterations} fact {r.stats['factorizations']} "
             f"pcg {r.stats['pcg_iterations']} solves {r.stats['solves']} fb {r.stats['fp64_fallbacks']} "
-            f"{r.ms_solve:.0f} ms (+{r.ms_update:.0f} ms update)")
-    wall = time.perf_counter() - t0
+            f"start {r.stats.get('start')} {r.ms_solve:.0f} ms (+{r.ms_update:.0f} ms update)")
+    wall = time.perf_counter() - t0  # every step returns after its device work (results are read back)
     info = s.info(reset=True)
     s.close()
     return dict(loads=loads, ms_solve=ms_solve, ms_update=ms_update, stats=stats, wall=wall, info=info)
@@ -111,7 +112,12 @@ def cpu_baseline(wl, gpu_stats, threads):
     sol = sum(s["solves"] for s in gpu_stats) / k
     spmv = sum(s["pcg_iterations"] for s in gpu_stats) / k
     per_inc = t_prep + asm_m * t_asm + asm_r * t_res + fac * t_fac + sol * t_sol + spmv * 0.25 * t_res
-    return dict(value=1.0 / per_inc, unit="increments/s", cores=threads, kind="port",
+    model = ""
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    return dict(value=1.0 / per_inc, unit="increments/s", cores=threads, kind="port", cpu=model,
                 sample=(f"hex-die 360,555 DOF, one increment's work measured piecewise on the host: PrepareStep "
                         f"{t_prep:.2f}s, tangent assembly {t_asm:.2f}s, residual assembly {t_res:.2f}s, MKL PARDISO "
                         f"analyse {t_an:.1f}s (not per increment) / factor {t_fac:.2f}s ({info['nnz_l']:.3e} nnz(L), "
@@ -128,7 +134,12 @@ def main():
     ap.add_argument("--arms", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the triangular sweeps one by one instead of replaying them as a HIP graph "
+                         "(for kernel-trace profilers)")
     args = ap.parse_args()
+    if args.no_graph:
+        os.environ["IFEA_SWEEP_GRAPH"] = "0"
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
@@ -146,7 +157,8 @@ def main():
         log("arm mixed")
         res["mixed"] = run_arm(wl, 1, args.steps, args.warmup, warm_start=ws)
     head = res.get("mixed") or res["fp64"]
-    ips = {k: len(v["ms_solve"]) / (sum(v["ms_solve"]) / 1e3) for k, v in res.items()}
+    ips = {k: len(v["ms_solve"]) / v["wall"] for k, v in res.items()}                  # whole increments, wall clock
+    ips_solve = {k: len(v["ms_solve"]) / (sum(v["ms_solve"]) / 1e3) for k, v in res.items()}  # without TryUpdate
     value = ips.get("mixed", ips.get("fp64"))
     ms_step = 1e3 / value
     load_err = None
@@ -200,7 +212,10 @@ def main():
         "config": {"workload": f"{wl.name} closed-die hex upset, {wl.nxy}x{wl.nxy}x{wl.nz} Kuhn cells, "
                                f"360,555 DOF, FP64 residual" if args.workload == "hexdie" else wl.name,
                    "arms": {k: round(v, 4) for k, v in ips.items()},
+                   "arms_without_host_update": {k: round(v, 4) for k, v in ips_solve.items()},
                    "speedup_mixed_vs_fp64": round(ips["mixed"] / ips["fp64"], 3) if len(ips) == 2 else None,
+                   "newton_iterations": {k: [st["iterations"] for st in v["stats"]] for k, v in res.items()},
+                   "starts": {k: [st.get("start") for st in v["stats"]] for k, v in res.items()},
                    "load_rel_err_mixed_vs_fp64": load_err,
                    "learned_warm_start": bool(ws),
                    "nnz_L": info["nnz_l"], "factor_flops": info["flops"],

@@ -314,8 +314,8 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 
 
 // Outer trailing update with 128x128 output tiles (mode 1 of k_syrk): 4 waves, each a 64x64 quadrant as 4x4 MFMA
-// blocks. X is streamed in 16-column chunks through a double-buffered LDS tile; the next chunk's global loads are in
-// flight (in registers) while the current chunk feeds the MFMAs, one barrier per chunk.
+// blocks. X is streamed in 16-column chunks through a double-buffered LDS tile; the global loads of the next two
+// chunks are in flight (in two register sets) while the current chunk feeds the MFMAs, one barrier per chunk.
 // task: (supernode s, row tile I, column tile J, 0), I >= J.
 template <typename T>
 __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
@@ -341,8 +341,8 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
   const int ib = (wv >> 1) * 64, jb = (wv & 1) * 64;
   const int l16 = lane & 15;
   const int nch = (kw + KC - 1) / KC;
-  T ri[PER], rj[PER];
-  auto fetch = [&](int c) {
+  T ai[PER], aj[PER], bi[PER], bj[PER];  // two chunks in flight: loads have two chunks of MFMAs to arrive
+  auto fetch = [&](int c, T* ri, T* rj) {
 #pragma unroll
     for (int t = 0; t < PER; t++) {
       int idx = threadIdx.x + 256 * t;
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
       rj[t] = (col < kw && rc < m) ? cp[rc] : T(0);
     }
   };
-  auto stage = [&](int buf) {
+  auto stage = [&](int buf, const T* ri, const T* rj) {
 #pragma unroll
     for (int t = 0; t < PER; t++) {
       int idx = threadIdx.x + 256 * t;
@@ -369,12 +369,7 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = v4{0, 0, 0, 0};
-  fetch(0);
-  stage(0);
-  __syncthreads();
-  for (int c = 0; c < nch; c++) {
-    const int buf = c & 1;
-    if (c + 1 < nch) fetch(c + 1);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int q = kk + (lane >> 4);
@@ -388,7 +383,21 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
     }
-    if (c + 1 < nch) stage(buf ^ 1);
+  };
+  fetch(0, ai, aj);
+  stage(0, ai, aj);
+  if (1 < nch) fetch(1, ai, aj);
+  if (2 < nch) fetch(2, bi, bj);
+  __syncthreads();
+  for (int c = 0; c < nch; c += 2) {
+    compute(0);                                  // chunk c
+    if (c + 1 < nch) stage(1, ai, aj);           // chunk c + 1
+    if (c + 3 < nch) fetch(c + 3, ai, aj);
+    __syncthreads();
+    if (c + 1 >= nch) break;
+    compute(1);                                  // chunk c + 1
+    if (c + 2 < nch) stage(0, bi, bj);           // chunk c + 2
+    if (c + 4 < nch) fetch(c + 4, bi, bj);
     __syncthreads();
   }
   T* Ls = L + sd.Loff[s];
@@ -639,9 +648,10 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
   for (int base = lo; base < hi; base += BT) {
     double v[BT];
 #pragma unroll
-    for (int u = 0; u < BT; u++) v[u] = base + u < hi ? Xb[(long long)(base + u) * SB + r] : 0.0;
+    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * SB + r];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < BT; u++) acc[u & 3] += v[u] * ws[base + u];
+    for (int u = 0; u < BT; u++) acc[u & 3] += base + u < hi ? v[u] * ws[base + u] : 0.0;
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
@@ -668,13 +678,15 @@ __device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int q0 = (SB / 4) * wv, q1 = min(wd, q0 + SB / 4);
   if (row < m) {
+    // loads past the wave's columns are clamped in bounds and weighted by zero, so all BT are issued unconditionally
     const T* pr = L + sd.Loff[s] + (long long)c0 * m + row;
     for (int base = q0; base < q1; base += BT) {
       T v[BT];
 #pragma unroll
-      for (int u = 0; u < BT; u++) v[u] = base + u < q1 ? pr[(long long)(base + u) * m] : T(0);
+      for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * m];
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * xs[base + u];
+      for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * (base + u < q1 ? xs[base + u] : 0.0);
     }
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
@@ -715,18 +727,20 @@ __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T
   const T* cq = L + sd.Loff[s] + (long long)q * m + k;
   const double* xb = V + sd.Voff[s];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (long long base = lane; base < mb; base += 64 * BT) {
+  long long base = lane;
+  for (; base + 64 * (BT - 1) < mb; base += 64 * BT) {  // full batches: BT independent loads of each operand
     T v[BT];
     double xv[BT];
 #pragma unroll
     for (int u = 0; u < BT; u++) {
-      const long long i = base + 64 * u;
-      v[u] = i < mb ? cq[i] : T(0);
-      xv[u] = i < mb ? xb[i] : 0.0;
+      v[u] = cq[base + 64 * u];
+      xv[u] = xb[base + 64 * u];
     }
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of their uses (one wait, not one per pair)
 #pragma unroll
     for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * xv[u];
   }
+  for (; base < mb; base += 64) acc[0] += (double)cq[base] * xb[base];
   const double a = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
   if (lane == 0) Wg[sd.Woff[s] + q] = y[f + q] - a;
 }
@@ -755,9 +769,10 @@ __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
   for (int base = lo; base < hi; base += BT) {
     double v[BT];
 #pragma unroll
-    for (int u = 0; u < BT; u++) v[u] = base + u < hi ? Xb[(long long)(base + u) * SB + c] : 0.0;
+    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * SB + c];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < BT; u++) acc[u & 3] += v[u] * zs[base + u];
+    for (int u = 0; u < BT; u++) acc[u & 3] += base + u < hi ? v[u] * zs[base + u] : 0.0;
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
@@ -787,17 +802,19 @@ __device__ __forceinline__ void bwdb_panel_item(const int4 tk, SymDev sd,
   T v[4][NC];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const int j = tk.z + wv + 4 * u;
+    // columns past the task and rows past the step are clamped in bounds (weighted by zero or not stored)
+    const int j = min(tk.z + wv + 4 * u, tk.w - 1);
     const T* cj = P + (long long)j * m + c0;
 #pragma unroll
-    for (int t = 0; t < NC; t++) v[u][t] = (j < tk.w && lane + 64 * t < wd) ? cj[lane + 64 * t] : T(0);
+    for (int t = 0; t < NC; t++) v[u][t] = cj[min(lane + 64 * t, wd - 1)];
   }
+  __builtin_amdgcn_sched_barrier(0);  // all loads of the lane in flight before the first reduction
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int j = tk.z + wv + 4 * u;
     double a = 0.0;
 #pragma unroll
-    for (int t = 0; t < NC; t++) a += (double)v[u][t] * xs[lane + 64 * t];
+    for (int t = 0; t < NC; t++) a += lane + 64 * t < wd ? (double)v[u][t] * xs[lane + 64 * t] : 0.0;
     a = wave_sum(a);
     if (lane == 0 && j < tk.w) z[j] -= a;
   }
