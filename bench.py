@@ -125,6 +125,21 @@ def cpu_baseline(wl, gpu_stats, threads):
                         f"counts (assemblies {asm_m:.1f}+{asm_r:.1f}, factorisations {fac:.2f}, solves {sol:.1f})"))
 
 
+def copy_probe(nbytes: int, reps: int = 3):
+    """Device-to-device copies of a known size (the calibration kernel for HBM byte counters)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,7 +152,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the triangular sweeps one by one instead of replaying them as a HIP graph "
                          "(for kernel-trace profilers)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of this size (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.no_graph:
         os.environ["IFEA_SWEEP_GRAPH"] = "0"
     if args.gpus != 1:
@@ -181,6 +201,16 @@ def main():
                            kernel=f"k_syrk_big<{'float' if prec == 'fp32' else 'double'}>", launches=nl,
                            avg_launch_us=round(1e3 * ms / nl, 2), flops_per_launch=round(fl / nl),
                            total_ms=round(ms, 1))
+    # HBM bytes per launch of the same kernel from the two PMC passes (FETCH_SIZE, WRITE_SIZE) summarised in profiles/
+    traffic = {}
+    pmc = os.path.join(ROOT, "profiles", "k_syrk_big_hexdie_pmc.md")
+    if args.workload == "hexdie" and os.path.exists(pmc):
+        for line in open(pmc):
+            for prec in ("fp64", "fp32"):
+                if line.startswith(f"traffic_bytes_per_launch_{prec}:"):
+                    traffic[prec] = float(line.split(":", 1)[1])
+    for prec, r in roofs.items():
+        r["traffic"] = traffic.get(prec)
     roof = None
     if roofs:
         dom = max(roofs, key=lambda k: roofs[k]["total_ms"])

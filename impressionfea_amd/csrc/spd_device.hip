@@ -27,6 +27,7 @@ constexpr int SB = 512;         // diagonal block of the blocked sweeps (explici
 constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step
 constexpr int GEMV_COLS = 4;    // columns per workgroup of the backward GEMV (one wave each)
 constexpr int BT = 32;          // loads a lane issues before using them in the sweep kernels
+constexpr int DIAG_WAVES = 16;  // waves of a diagonal-step workgroup (they split the step's columns)
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -624,25 +625,25 @@ __device__ __forceinline__ void fwdb_gather_item(const int4 tk, SymDev sd,
   Wg[sd.Woff[s] + i] = w;
 }
 
-// Forward diagonal step: x_r = sum_c X_t[r][c] w_c for the 64 rows [c0 + 64 tk.z, ...) of the step; the waves split
-// the columns c < 64 (tk.z + 1) (X_t is lower triangular) and combine their partial sums in a fixed order. Loads are
-// issued BT at a time. task: (s, c0, row block, 0)
+// Forward diagonal step: x_r = sum_c X_t[r][c] w_c for the 64 rows [c0 + 64 tk.z, ...) of the step; the 16 waves
+// split the columns c < 64 (tk.z + 1) (X_t is lower triangular) and combine their partial sums in a fixed order.
+// Loads are issued BT at a time. task: (s, c0, row block, 0)
 __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
                                                    const double* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  __shared__ double ws[SB + BT], part[4][NB];
+  __shared__ double ws[SB + BT], part[DIAG_WAVES][NB];
   const int s = tk.x, c0 = tk.y, rb = tk.z;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
   const double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
   const double* w = Wg + sd.Woff[s] + c0;
-  for (int q = threadIdx.x; q < SB + BT; q += 256) ws[q] = q < wd ? w[q] : 0.0;
+  for (int q = threadIdx.x; q < SB + BT; q += DIAG_WAVES * 64) ws[q] = q < wd ? w[q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = NB * rb + lane;
   const int cend = min(wd, NB * rb + NB);
-  const int per = (((cend + 3) / 4 + BT - 1) / BT) * BT;
+  const int per = (((cend + DIAG_WAVES - 1) / DIAG_WAVES + BT - 1) / BT) * BT;
   const int lo = wv * per, hi = min(cend, lo + per);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int base = lo; base < hi; base += BT) {
@@ -655,7 +656,12 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
-  if (threadIdx.x < NB && r < wd) y[f + c0 + r] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  if (threadIdx.x < NB && r < wd) {
+    double x = 0.0;
+#pragma unroll
+    for (int q = 0; q < DIAG_WAVES; q++) x += part[q][lane];
+    y[f + c0 + r] = x;
+  }
 }
 
 // Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 rows per workgroup (waves split the step's columns,
@@ -746,24 +752,24 @@ __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T
 }
 
 // Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...), lane per
-// column over the transposed copy XT[r * SB + c] = X_t[r][c]; the waves split the rows r >= 64 tk.z.
+// column over the transposed copy XT[r * SB + c] = X_t[r][c]; the 16 waves split the rows r >= 64 tk.z.
 // task: (s, c0, column block, 0)
 __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
                                                    const double* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  __shared__ double zs[SB + BT], part[4][NB];
+  __shared__ double zs[SB + BT], part[DIAG_WAVES][NB];
   const int s = tk.x, c0 = tk.y, cb = tk.z;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
   const double* Xb = XT + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
   const double* z = Wg + sd.Woff[s] + c0;
-  for (int q = threadIdx.x; q < SB + BT; q += 256) zs[q] = q < wd ? z[q] : 0.0;
+  for (int q = threadIdx.x; q < SB + BT; q += DIAG_WAVES * 64) zs[q] = q < wd ? z[q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c = NB * cb + lane;
   const int span = wd - NB * cb;
-  const int per = (((span + 3) / 4 + BT - 1) / BT) * BT;
+  const int per = (((span + DIAG_WAVES - 1) / DIAG_WAVES + BT - 1) / BT) * BT;
   const int lo = NB * cb + wv * per, hi = min(wd, lo + per);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int base = lo; base < hi; base += BT) {
@@ -776,7 +782,12 @@ __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
-  if (threadIdx.x < NB && c < wd) y[f + c0 + c] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  if (threadIdx.x < NB && c < wd) {
+    double x = 0.0;
+#pragma unroll
+    for (int q = 0; q < DIAG_WAVES; q++) x += part[q][lane];
+    y[f + c0 + c] = x;
+  }
 }
 
 // Backward panel step: z_j -= L(c0 .. c0+wd, j)^T x_t for the columns j of this task (< c0): wave wv takes columns
@@ -827,7 +838,7 @@ __global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ ta
   fwdb_gather_item(tasks[blockIdx.x], sd, y, V, Wg);
 }
 
-__global__ __launch_bounds__(256) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+__global__ __launch_bounds__(DIAG_WAVES * 64) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
                                                    const double* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
   fwdb_diag_item(tasks[blockIdx.x], sd, X, y, Wg);
@@ -852,7 +863,7 @@ __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ task
   bwdb_gemv_item<T>(tasks[blockIdx.x], sd, L, y, V, Wg);
 }
 
-__global__ __launch_bounds__(256) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
+__global__ __launch_bounds__(DIAG_WAVES * 64) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
                                                    const double* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
   bwdb_diag_item(tasks[blockIdx.x], sd, XT, y, Wg);
@@ -1308,6 +1319,11 @@ void SpdDevice::sweeps(const T* L) {
       hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(256), 0, stream_, tasks + ll.begin, sd_,
                          args...);
   };
+  auto launch_diag = [&](auto kern, const LaunchList& ll, const int4* tasks, auto... args) {
+    if (ll.end > ll.begin)
+      hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(DIAG_WAVES * 64), 0, stream_,
+                         tasks + ll.begin, sd_, args...);
+  };
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
     unsigned cnt = (unsigned)(lv.small_end - lv.small_begin);
@@ -1318,7 +1334,7 @@ void SpdDevice::sweeps(const T* L) {
     }
     launch(k_fwdb_gather, lv.fwd_gather, fwd_tasks_.p, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.fwd_diag.size(); t++) {
-      launch(k_fwdb_diag, lv.fwd_diag[t], fwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
+      launch_diag(k_fwdb_diag, lv.fwd_diag[t], fwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
       launch(k_fwdb_panel<T>, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p, V_.p);
     }
   }
@@ -1333,7 +1349,7 @@ void SpdDevice::sweeps(const T* L) {
     launch(k_bwdb_xb, lv.bwd_xb, bwd_tasks_.p, (const double*)y_.p, V_.p);
     launch(k_bwdb_gemv<T>, lv.bwd_gemv, bwd_tasks_.p, L, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.bwd_diag.size(); t++) {
-      launch(k_bwdb_diag, lv.bwd_diag[t], bwd_tasks_.p, (const double*)XT_.p, y_.p, (const double*)W_.p);
+      launch_diag(k_bwdb_diag, lv.bwd_diag[t], bwd_tasks_.p, (const double*)XT_.p, y_.p, (const double*)W_.p);
       launch(k_bwdb_panel<T>, lv.bwd_panel[t], bwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
   }
