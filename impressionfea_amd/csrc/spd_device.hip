@@ -165,8 +165,20 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
     colb[j & 1][i] = lij;
     wave_sync_lds();
     const double* cb = colb[j & 1];
-#pragma unroll 8
-    for (int q = j + 1; q < NB; q++) row[q] -= lij * cb[q];
+    // eight columns at a time, all loads before the stores: the row and the column vector are separate arrays, but
+    // stated this way no store has to wait for the loads of the next columns
+    for (int q0 = j + 1; q0 < NB; q0 += 8) {
+      double cv[8], rv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int q = min(q0 + u, NB - 1);
+        cv[u] = cb[q];
+        rv[u] = row[q];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (q0 + u < NB) row[q0 + u] = rv[u] - lij * cv[u];
+    }
     wave_sync_lds();
   }
   for (int q = i + 1; q < NB; q++) row[q] = 0.0;
@@ -176,15 +188,23 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   for (int q = 0; q < NB; q++)
     if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)row[q];
   if (bad && i == 0) atomicOr(err, 1);
-  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]
+  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; four partial sums per row keep
+  // the dependent chain of a row a quarter as long
   const int c = i;
   double x[NB];
 #pragma unroll
   for (int r = 0; r < NB; r++) {
-    double acc = (r == c) ? 1.0 : 0.0;
+    double a0 = (r == c) ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
-    for (int p = 0; p < r; p++) acc -= D[r][p] * x[p];
-    x[r] = acc / D[r][r];
+    for (int p = 0; p + 3 < r; p += 4) {
+      a0 -= D[r][p] * x[p];
+      a1 -= D[r][p + 1] * x[p + 1];
+      a2 -= D[r][p + 2] * x[p + 2];
+      a3 -= D[r][p + 3] * x[p + 3];
+    }
+#pragma unroll
+    for (int p = r & ~3; p < r; p++) a0 -= D[r][p] * x[p];
+    x[r] = ((a0 + a1) + (a2 + a3)) / D[r][r];
   }
   wave_sync_lds();
   // transpose through LDS so that the stores of Lt[c][r] = Linv[r][c] are contiguous over the lanes
@@ -314,15 +334,18 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 }
 
 
-// Outer trailing update with 128x128 output tiles (mode 1 of k_syrk): 4 waves, each a 64x64 quadrant as 4x4 MFMA
-// blocks. X is streamed in 16-column chunks through a double-buffered LDS tile; the global loads of the next two
-// chunks are in flight (in two register sets) while the current chunk feeds the MFMAs, one barrier per chunk.
-// task: (supernode s, row tile I, column tile J, 0), I >= J.
-template <typename T>
-__global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
-                                                  T* __restrict__ L, T* __restrict__ U) {
+// Outer trailing update with 128x128 output tiles (mode 1 of k_syrk): NT/64 waves, two along the rows (64 each) and
+// NT/128 along the columns, each wave a block of 16x16 MFMA tiles (64x64 with 4 waves, 64x32 with 8 waves: the FP64
+// accumulators of 8 waves fit two waves per SIMD). X is streamed in 16-column chunks through a double-buffered LDS
+// tile; the global loads of the next two chunks are in flight (in two register sets) while the current chunk feeds
+// the MFMAs, one barrier per chunk. task: (supernode s, row tile I, column tile J, 0), I >= J.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
+                                                 T* __restrict__ L, T* __restrict__ U) {
   constexpr int KC = 16;
-  constexpr int PER = KC * 128 / 256;  // elements of each operand a thread stages per chunk
+  constexpr int PER = KC * 128 / NT;   // elements of each operand a thread stages per chunk
+  constexpr int WJ = NT / 128;         // waves along the columns
+  constexpr int BJ = 128 / WJ / 16;    // 16x16 blocks of a wave along the columns (rows: 4)
   constexpr int LD = 144;              // LDS row stride: consecutive k rows start 32 banks apart (no conflicts)
   // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous range of the task list (tiles
   // of the same tile row and neighbouring columns), so the panel rows they share stay in that XCD's L2
@@ -339,14 +362,14 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
   __shared__ T Xj[2][KC][LD];
   const long long i0 = rb + 128LL * tk.y, j0 = rb + 128LL * tk.z;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int ib = (wv >> 1) * 64, jb = (wv & 1) * 64;
+  const int ib = (wv & 1) * 64, jb = (wv >> 1) * (128 / WJ);
   const int l16 = lane & 15;
   const int nch = (kw + KC - 1) / KC;
   T ai[PER], aj[PER], bi[PER], bj[PER];  // two chunks in flight: loads have two chunks of MFMAs to arrive
   auto fetch = [&](int c, T* ri, T* rj) {
 #pragma unroll
     for (int t = 0; t < PER; t++) {
-      int idx = threadIdx.x + 256 * t;
+      int idx = threadIdx.x + NT * t;
       int q = idx >> 7, rr = idx & 127;
       int col = c * KC + q;
       const T* cp = P + (long long)(kb + col) * m;
@@ -358,29 +381,29 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
   auto stage = [&](int buf, const T* ri, const T* rj) {
 #pragma unroll
     for (int t = 0; t < PER; t++) {
-      int idx = threadIdx.x + 256 * t;
+      int idx = threadIdx.x + NT * t;
       int q = idx >> 7, rr = idx & 127;
       Xi[buf][q][rr] = ri[t];
       Xj[buf][q][rr] = rj[t];
     }
   };
   typedef typename Mfma<T>::v4 v4;
-  v4 acc[4][4];
+  v4 acc[BJ][4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < BJ; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = v4{0, 0, 0, 0};
   auto compute = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int q = kk + (lane >> 4);
-      T av[4], bv[4];
+      T av[BJ], bv[4];
 #pragma unroll
-      for (int a = 0; a < 4; a++) av[a] = Xj[buf][q][jb + 16 * a + l16];
+      for (int a = 0; a < BJ; a++) av[a] = Xj[buf][q][jb + 16 * a + l16];
 #pragma unroll
       for (int b = 0; b < 4; b++) bv[b] = Xi[buf][q][ib + 16 * b + l16];
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int a = 0; a < BJ; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
     }
@@ -405,7 +428,7 @@ __global__ __launch_bounds__(256) void k_syrk_big(const int4* __restrict__ tasks
   T* Us = U + sd.Uoff[s];
   const long long mu = m - k;
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < BJ; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++)
 #pragma unroll
@@ -1200,6 +1223,14 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   analyzed_ = true;
 }
 
+static int outer_threads() {
+  static const int nt = [] {
+    const char* e = std::getenv("IFEA_OUTER_THREADS");
+    return (e && std::atoi(e) == 256) ? 256 : 512;
+  }();
+  return nt;
+}
+
 template <typename T>
 int SpdDevice::factorize_t(const double* d_vals) {
   const Symbolic& S = S_;
@@ -1239,8 +1270,11 @@ int SpdDevice::factorize_t(const double* d_vals) {
         if (mode == 0)
           hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, mode, sd_, L, U);
+        else if (outer_threads() == 256)
+          hipLaunchKernelGGL((k_syrk_big<T, 256>), dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
+                             syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
         else
-          hipLaunchKernelGGL(k_syrk_big<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
+          hipLaunchKernelGGL((k_syrk_big<T, 512>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
         if (profiling_) {
           IFEA_HIP(hipEventRecord(event(2 * nev + 1), stream_));
