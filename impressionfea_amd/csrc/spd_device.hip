@@ -339,10 +339,9 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 // accumulators of 8 waves fit two waves per SIMD). X is streamed in 16-column chunks through a double-buffered LDS
 // tile; the global loads of the next two chunks are in flight (in two register sets) while the current chunk feeds
 // the MFMAs, one barrier per chunk. task: (supernode s, row tile I, column tile J, 0), I >= J.
-template <typename T, int NT>
+template <typename T, int NT, int KC>
 __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
                                                  T* __restrict__ L, T* __restrict__ U) {
-  constexpr int KC = 16;
   constexpr int PER = KC * 128 / NT;   // elements of each operand a thread stages per chunk
   constexpr int WJ = NT / 128;         // waves along the columns
   constexpr int BJ = 128 / WJ / 16;    // 16x16 blocks of a wave along the columns (rows: 4)
@@ -1223,12 +1222,12 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   analyzed_ = true;
 }
 
-static int outer_threads() {
-  static const int nt = [] {
-    const char* e = std::getenv("IFEA_OUTER_THREADS");
-    return (e && std::atoi(e) == 256) ? 256 : 512;
+static int outer_kc() {
+  static const int kc = [] {
+    const char* e = std::getenv("IFEA_OUTER_KC");
+    return (e && std::atoi(e) == 32) ? 32 : 16;
   }();
-  return nt;
+  return kc;
 }
 
 template <typename T>
@@ -1270,11 +1269,11 @@ int SpdDevice::factorize_t(const double* d_vals) {
         if (mode == 0)
           hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, mode, sd_, L, U);
-        else if (outer_threads() == 256)
-          hipLaunchKernelGGL((k_syrk_big<T, 256>), dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
+        else if (outer_kc() == 32)
+          hipLaunchKernelGGL((k_syrk_big<T, 512, 32>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
         else
-          hipLaunchKernelGGL((k_syrk_big<T, 512>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, stream_,
+          hipLaunchKernelGGL((k_syrk_big<T, 512, 16>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
         if (profiling_) {
           IFEA_HIP(hipEventRecord(event(2 * nev + 1), stream_));
