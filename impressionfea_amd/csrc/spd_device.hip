@@ -1222,14 +1222,6 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   analyzed_ = true;
 }
 
-static int outer_kc() {
-  static const int kc = [] {
-    const char* e = std::getenv("IFEA_OUTER_KC");
-    return (e && std::atoi(e) == 32) ? 32 : 16;
-  }();
-  return kc;
-}
-
 template <typename T>
 int SpdDevice::factorize_t(const double* d_vals) {
   const Symbolic& S = S_;
@@ -1269,9 +1261,6 @@ int SpdDevice::factorize_t(const double* d_vals) {
         if (mode == 0)
           hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, mode, sd_, L, U);
-        else if (outer_kc() == 32)
-          hipLaunchKernelGGL((k_syrk_big<T, 512, 32>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, stream_,
-                             syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
         else
           hipLaunchKernelGGL((k_syrk_big<T, 512, 16>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
