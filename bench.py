@@ -56,8 +56,23 @@ def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
             f"start {r.stats.get('start')} {r.ms_solve:.0f} ms (+{r.ms_update:.0f} ms update)")
     wall = time.perf_counter() - t0  # every step returns after its device work (results are read back)
     info = s.info(reset=True)
+    used = device_memory_used()
     s.close()
-    return dict(loads=loads, ms_solve=ms_solve, ms_update=ms_update, stats=stats, wall=wall, info=info)
+    return dict(loads=loads, ms_solve=ms_solve, ms_update=ms_update, stats=stats, wall=wall, info=info,
+                device_used=used)
+
+
+def device_memory_used():
+    """Bytes in use on the current device (total - free, hipMemGetInfo), or None when it cannot be read."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        free, total = C.c_size_t(), C.c_size_t()
+        if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+            return None
+        return dict(used=total.value - free.value, total=total.value)
+    except OSError:
+        return None
 
 
 def cpu_baseline(wl, gpu_stats, threads):
@@ -66,15 +81,14 @@ def cpu_baseline(wl, gpu_stats, threads):
     solve; scaled to increments/s with the fp64 arm's per-increment counts of assemblies, factorisations and solves."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from impressionfea_amd.mesh import kuhn_cylinder
     O.build()
     if not O.pardiso_available():
         return None
     O.set_threads(threads)
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=wl.seed)
+    x, t, ox, oy = wl.mesh()
     o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0, linear=1)
-    o.initial(0.0)
+    o.initial(float(x[:, 2].min()))
     t0 = time.perf_counter()
     o.prepare()
     t_prep = time.perf_counter() - t0
@@ -118,7 +132,7 @@ def cpu_baseline(wl, gpu_stats, threads):
     except (OSError, StopIteration):
         pass
     return dict(value=1.0 / per_inc, unit="increments/s", cores=threads, kind="port", cpu=model,
-                sample=(f"hex-die 360,555 DOF, one increment's work measured piecewise on the host: PrepareStep "
+                sample=(f"{wl.name} {3 * x.shape[0]:,} DOF, one increment's work measured piecewise on the host: PrepareStep "
                         f"{t_prep:.2f}s, tangent assembly {t_asm:.2f}s, residual assembly {t_res:.2f}s, MKL PARDISO "
                         f"analyse {t_an:.1f}s (not per increment) / factor {t_fac:.2f}s ({info['nnz_l']:.3e} nnz(L), "
                         f"{info['mflops']} MFLOP) / solve {t_sol:.2f}s; scaled by the fp64 arm's per-increment "
@@ -145,7 +159,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="hexdie", choices=["hexdie", "upset"])
+    ap.add_argument("--workload", default="hexdie", choices=["hexdie", "upset", "conrod"],
+                    help="hexdie: the 360k-DOF hex-head closed die (the headline line); conrod: the ~1M-DOF "
+                         "connecting-rod closed die (peak device memory against the 288 GB of one MI355X); upset: "
+                         "the small upset")
     ap.add_argument("--arms", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -164,9 +181,9 @@ def main():
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
-    from impressionfea_amd.config import hexdie_production, upset_small
+    from impressionfea_amd.config import conrod_production, hexdie_production, upset_small
     from impressionfea_amd import warmstart
-    wl = hexdie_production() if args.workload == "hexdie" else upset_small()
+    wl = {"hexdie": hexdie_production, "conrod": conrod_production, "upset": upset_small}[args.workload]()
     ws = warmstart.load_default()
     log(f"workload {wl.name}: {wl.nxy}x{wl.nxy}x{wl.nz} cells; learned warm start: {'yes' if ws else 'no'}")
     res = {}
@@ -234,13 +251,26 @@ def main():
             cpu = cpu_baseline(wl, res["fp64"]["stats"], args.cpu_threads)
         except Exception as ex:  # noqa: BLE001 - the baseline is reported, never required
             log(f"cpu baseline failed: {ex}")
+    ndof = 3 * ((wl.nxy + 1) ** 2 * (wl.nz + 1))
+    memory = {k: dict(factor_bytes=v["info"]["device_bytes"],
+                      process_bytes=v["device_used"]["used"] if v["device_used"] else None,
+                      device_total=v["device_used"]["total"] if v["device_used"] else None)
+              for k, v in res.items()}
+    if args.workload == "hexdie":
+        metric = "stroke increments/s @360k DOF: plain FP64 vs mixed+warm-start, <=0.1% load err"
+        desc = f"{wl.name} closed-die hex upset, {wl.nxy}x{wl.nxy}x{wl.nz} Kuhn cells, 360,555 DOF, FP64 residual"
+    elif args.workload == "conrod":
+        metric = "stroke increments/s @1M DOF con-rod: plain FP64 vs mixed, peak factor memory vs 288 GB"
+        desc = (f"{wl.name} connecting-rod closed die, billet lying along x, {wl.nxy}x{wl.nxy}x{wl.nz} Kuhn cells, "
+                f"{ndof:,} DOF, FP64 residual")
+    else:
+        metric, desc = "stroke increments/s (small upset)", wl.name
     out = {
-        "metric": "stroke increments/s @360k DOF: plain FP64 vs mixed+warm-start, <=0.1% load err",
+        "metric": metric,
         "value": round(value, 4), "unit": "increments/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_step, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64" if "mixed" in res else "fp64", "data": "synthetic",
-        "config": {"workload": f"{wl.name} closed-die hex upset, {wl.nxy}x{wl.nxy}x{wl.nz} Kuhn cells, "
-                               f"360,555 DOF, FP64 residual" if args.workload == "hexdie" else wl.name,
+        "config": {"workload": desc, "dof": ndof,
                    "arms": {k: round(v, 4) for k, v in ips.items()},
                    "arms_without_host_update": {k: round(v, 4) for k, v in ips_solve.items()},
                    "speedup_mixed_vs_fp64": round(ips["mixed"] / ips["fp64"], 3) if len(ips) == 2 else None,
@@ -249,7 +279,8 @@ def main():
                    "load_rel_err_mixed_vs_fp64": load_err,
                    "learned_warm_start": bool(ws),
                    "nnz_L": info["nnz_l"], "factor_flops": info["flops"],
-                   "ms_update_host": round(float(np.mean(head["ms_update"])), 1)},
+                   "ms_update_host": round(float(np.mean(head["ms_update"])), 1),
+                   "device_memory": memory},
         "roofline": roof,
         "triangular_sweeps": sweep,
         "cpu_baseline": cpu,

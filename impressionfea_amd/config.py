@@ -6,7 +6,9 @@
 Newton blend, tolerance, iteration limits, PCG budget.
 
 Dies are analytic height fields (``DieSpec``): kind 0 is a flat plane, kind 1 a top die with a hexagonal cavity,
-flash land and gutter. The same height function is evaluated by the host driver and by the oracle.
+flash land and gutter, kind 2 one half of a connecting-rod closed die (two bosses joined by an I-section shank, flash
+land and gutter; the bottom half mirrors the top). The same height function is evaluated by the host driver and by
+the oracle.
 """
 from __future__ import annotations
 
@@ -18,8 +20,11 @@ import numpy as np
 @dataclass
 class DieSpec:
     side: int          # 0 bottom (metal above), 1 top (metal below)
-    kind: int = 0      # 0 flat, 1 hex cavity
+    kind: int = 0      # 0 flat, 1 hex cavity, 2 connecting-rod half
     p: tuple = (0.0,) * 8
+
+    # connecting-rod footprint (kind 2): big boss (centre x, radius), small boss, gutter depth
+    CONROD = (-42.0, 20.0, 49.5, 12.5, 5.0)
 
     # height of the die surface in the closed position (top dies are offset by the current gap)
     def height(self, x, y):
@@ -27,13 +32,28 @@ class DieSpec:
         y = np.asarray(y, dtype=np.float64)
         if self.kind == 0:
             return np.full(np.broadcast(x, y).shape, self.p[0])
-        s3 = np.sqrt(3.0) / 2
-        d = np.maximum(np.abs(x), np.maximum(np.abs(0.5 * x + s3 * y), np.abs(-0.5 * x + s3 * y))) - 0.5 * self.p[2]
 
         def smooth(u):
             u = np.clip(u, 0.0, 1.0)
             return u * u * (3 - 2 * u)
 
+        if self.kind == 2:
+            # p = (flash thickness t, big-boss depth, small-boss depth, flange depth, web depth, shank width W,
+            #      wall transition width, land width); depths are per half, measured from the parting face
+            t, db_, ds_, df_, dw_, w_, wall, land = self.p
+            xb, rb, xs, rs, g = self.CONROD
+            fl = 0.25 * w_
+            db = np.hypot(x - xb, y) - rb
+            ds = np.hypot(x - xs, y) - rs
+            dsh = np.maximum(np.maximum(xb - x, x - xs), np.abs(y) - 0.5 * w_)
+            shank = dw_ + (df_ - dw_) * smooth((np.abs(y) - (0.5 * w_ - fl) + 0.5 * wall) / wall)
+            depth = np.maximum(np.maximum(db_ * smooth((-db + 0.5 * wall) / wall),
+                                          ds_ * smooth((-ds + 0.5 * wall) / wall)),
+                               shank * smooth((-dsh + 0.5 * wall) / wall))
+            h = 0.5 * t + depth + g * smooth((np.minimum(np.minimum(db, ds), dsh) - land) / wall)
+            return h if self.side == 1 else -h
+        s3 = np.sqrt(3.0) / 2
+        d = np.maximum(np.abs(x), np.maximum(np.abs(0.5 * x + s3 * y), np.abs(-0.5 * x + s3 * y))) - 0.5 * self.p[2]
         h = self.p[0] + self.p[1] * smooth((-d + 0.5 * self.p[3]) / self.p[3])
         h = h + self.p[5] * smooth((d - self.p[4]) / self.p[3])
         return h
@@ -100,6 +120,20 @@ class Workload:
     quarter: bool = False
     increments: int = 20
     gap0: float = 0.0
+    axis: str = "z"          # billet axis: "z" upright (bottom at z = 0), "x" lying along x, centred on x = 0
+    center_z: float = 0.0    # height of the axis of a lying billet
+
+    def mesh(self):
+        """The workload's jittered Kuhn billet: (x [nn,3], tets [ne,4], onX0, onY0)."""
+        from .mesh import kuhn_cylinder
+        x, t, ox, oy = kuhn_cylinder(self.diameter, self.height, self.nxy, self.nz, seed=self.seed,
+                                     quarter=self.quarter)
+        if self.axis == "x":
+            # rotate the upright billet about y by 90 degrees (orientation kept): axis -> x, and centre it
+            x = np.stack([x[:, 2] - 0.5 * self.height, x[:, 1], self.center_z - x[:, 0]], axis=1)
+            ox = np.zeros_like(ox)
+            oy = np.zeros_like(oy)
+        return x, t, ox, oy
 
 
 def upset_small(seed: int = 0, quarter: bool = False, nxy: int = 10, nz: int = 10, **over) -> Workload:
@@ -129,3 +163,21 @@ def hexdie_production(seed: int = 0, nxy: int = 42, nz: int = 64, **over) -> Wor
     cfg = replace(cfg, **over)
     gap0 = H - (t + depth)                               # billet top touches the cavity ceiling
     return Workload("hexdie-D30H45", cfg, D, H, nxy, nz, seed=seed, increments=40, gap0=gap0)
+
+
+def conrod_production(seed: int = 0, nxy: int = 40, nz: int = 192, **over) -> Workload:
+    """Connecting-rod closed die: a billet D 25 x L 120 mm lying along x, pressed between two die halves with a big
+    boss (diameter 40), a small boss (diameter 25) and an I-section shank (width 16, flanges 8 mm and web 3 mm deep per
+    half), flash 2.5 mm with a land of w/t = 3 and a gutter; 40x40x192 Kuhn cells = 324,433 nodes, 1,843,200 tets,
+    973,299 DOF, full model. Flow law of the hex-die workload; 60 increments close the initial 16.5 mm gap."""
+    D, L = 25.0, 120.0
+    t = 2.5
+    p = (t, 12.0, 9.0, 8.0, 3.0, 16.0, 2.0, 3 * t)
+    zc, gap0 = 8.25, 16.5      # the billet rests on the bottom web; the top die first touches the top web
+    cfg = StepConfig(C=80.0, n=0.05, m=0.2, eps0=0.01, friction=0.4, ram=10.0, penalty_factor=1e6, tolerance=1e-6,
+                     size=D, stroke_increment=gap0 / 60, h=max(D / nxy, L / nz), sym_factor=1,
+                     bottom=DieSpec(0, 2, p), top=DieSpec(1, 2, p))
+    cfg = replace(cfg, **over)
+    return Workload("conrod-D25L120", cfg, D, L, nxy, nz, seed=seed, increments=60, gap0=gap0, axis="x",
+                    center_z=zc)
+

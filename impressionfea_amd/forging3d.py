@@ -19,7 +19,7 @@ import numpy as np
 from . import _native as N
 from .config import StepConfig, Workload
 from .errors import MeshDistortionError, SolverDivergenceError, raise_for
-from .mesh import TetMesh, build_pattern, kuhn_cylinder
+from .mesh import TetMesh, build_pattern
 
 FREE, BOTTOM, TOP = 0, 1, 2
 
@@ -203,9 +203,9 @@ class Forging3DSolver:
 
     @classmethod
     def from_workload(cls, wl: Workload, precision: int = 0, **kw):
-        x, tets, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=wl.seed, quarter=wl.quarter)
+        x, tets, ox, oy = wl.mesh()
         s = cls(wl.cfg, x, tets, ox, oy, wl.gap0, precision=precision, **kw)
-        s.initial(zmin=0.0)
+        s.initial(zmin=float(x[:, 2].min()))
         return s
 
     # ------------------------------------------------------------------ constructor tail (:158-182)

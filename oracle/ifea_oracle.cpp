@@ -67,6 +67,10 @@ double max3(double xx, double yy, double zz, double xy, double yz, double zx) {
 // Analytic height-field dies. side 0 = bottom (metal above), 1 = top (metal below). kind 0: flat plane at z = p[0].
 // kind 1: top die with a hexagonal cavity: land at z = p[0] (the flash thickness t at closure), cavity ceiling at
 // p[0] + p[1], across-flats p[2], wall transition width p[3], land width p[4], gutter depth p[5].
+// kind 2: one half of a connecting-rod closed die, mirrored about the parting plane z = 0: a big boss (centre x = -42,
+// radius 20) and a small boss (x = 49.5, radius 12.5) joined by an I-section shank of width p[5] (flanges p[3] and
+// web p[4] deep), boss depths p[1] and p[2], flash thickness p[0] at closure, wall transition p[6], land width p[7],
+// gutter depth 5; the surface sits at +-(p[0]/2 + cavity depth + gutter).
 // Clearance follows HeightFieldDie.Clearance (HeightFieldDie.cs:130-134); Project is the vertical projection onto
 // the surface with the normal of the analytic surface pointing out of the die (the reference searches the closest
 // point on a triangulated height field, HeightFieldDie.cs:157-191; on flat regions both agree).
@@ -76,6 +80,17 @@ struct Die {
   static double smooth(double u) { u = std::min(1.0, std::max(0.0, u)); return u * u * (3 - 2 * u); }
   double height(double x, double y) const {
     if (kind == 0) return p[0];
+    if (kind == 2) {
+      const double xb = -42.0, rb = 20.0, xs = 49.5, rs = 12.5, g = 5.0;
+      const double t = p[0], wall = p[6], land = p[7], w = p[5], fl = 0.25 * w;
+      double db = std::hypot(x - xb, y) - rb, ds = std::hypot(x - xs, y) - rs;
+      double dsh = std::max(std::max(xb - x, x - xs), std::fabs(y) - 0.5 * w);
+      double shank = p[4] + (p[3] - p[4]) * smooth((std::fabs(y) - (0.5 * w - fl) + 0.5 * wall) / wall);
+      double depth = std::max(std::max(p[1] * smooth((-db + 0.5 * wall) / wall), p[2] * smooth((-ds + 0.5 * wall) / wall)),
+                              shank * smooth((-dsh + 0.5 * wall) / wall));
+      double h = 0.5 * t + depth + g * smooth((std::min(std::min(db, ds), dsh) - land) / wall);
+      return side == 1 ? h : -h;
+    }
     const double s3 = std::sqrt(3.0) / 2;
     double d = std::max(std::fabs(x), std::max(std::fabs(0.5 * x + s3 * y), std::fabs(-0.5 * x + s3 * y))) - 0.5 * p[2];
     double h = p[0] + p[1] * smooth((-d + 0.5 * p[3]) / p[3]);
@@ -1397,5 +1412,13 @@ void orc_pardiso_perm(void* p, int* perm) { auto* q = (PardisoDL*)p; std::copy(q
 void orc_pardiso_destroy(void* p) { delete (PardisoDL*)p; }
 int orc_pardiso_available() { PardisoDL p; return p.ok() ? 1 : 0; }
 void orc_set_threads(int n) { PardisoDL::set_threads(n); }
+
+// die surface heights at n points (the height field every contact routine of the oracle reads)
+void orc_die_height(int side, int kind, const double* p, int n, const double* x, const double* y, double* out) {
+  Die d;
+  d.side = side; d.kind = kind;
+  std::copy(p, p + 8, d.p);
+  for (int i = 0; i < n; ++i) out[i] = d.height(x[i], y[i]);
+}
 
 }  // extern "C"

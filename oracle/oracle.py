@@ -89,6 +89,7 @@ def lib():
         L.orc_pardiso_destroy.argtypes = [vp]
         L.orc_pardiso_available.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_die_height.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -314,6 +315,17 @@ def min_degree_nodes(nn, nbrStart, nbr):
     ns, nb = I32(nbrStart), I32(nbr)
     lib().orc_min_degree_nodes(nn, _p(ns, C.c_int), _p(nb, C.c_int), _p(perm, C.c_int))
     return perm
+
+
+def die_height(die, x, y):
+    """Surface height of a DieSpec at points (x, y), evaluated by the oracle's Die::height."""
+    x, y = np.broadcast_arrays(F64(x), F64(y))
+    x, y = F64(x.ravel()), F64(y.ravel())
+    out = np.empty_like(x)
+    p = F64(np.asarray(die.p, dtype=np.float64))
+    lib().orc_die_height(die.side, die.kind, _p(p, C.c_double), len(x), _p(x, C.c_double), _p(y, C.c_double),
+                         _p(out, C.c_double))
+    return out
 
 
 def pardiso_available() -> bool:
