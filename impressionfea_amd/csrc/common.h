@@ -55,11 +55,19 @@ struct DBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Workgroups for one item per thread. A dispatch covers fewer than 2^32 work-items: larger work needs a grid-stride
+// kernel (grid_stride below), and asking for it here is an error rather than a silently shortened grid.
 inline unsigned grid_for(long long work, int block) {
   long long g = (work + block - 1) / block;
   if (g < 1) g = 1;
-  if (g > 1 << 30) g = 1 << 30;
+  if (g * block > 0xffffffffLL) throw Error(-3, "launch of " + std::to_string(work) + " items exceeds one dispatch");
   return (unsigned)g;
+}
+
+// Workgroups of a grid-stride kernel over `work` items: at most 2^20 workgroups.
+inline unsigned grid_stride(long long work, int block) {
+  long long g = (work + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : (g > (1 << 20) ? (1 << 20) : g));
 }
 
 }  // namespace ifea

@@ -25,9 +25,11 @@ constexpr int BIG_K = 256;      // supernodes wider than this, or with panels la
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
 constexpr int SB = 512;         // diagonal block of the blocked sweeps (explicit inverse per block)
 constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step
-constexpr int GEMV_COLS = 4;    // columns per workgroup of the backward GEMV (one wave each)
+constexpr int GEMV_COLS = 16;   // columns per workgroup of the backward GEMV (four waves, GEMV_COLS / 4 columns each)
+constexpr int GEMV_RB = 8;      // rows of each column a backward-GEMV lane loads before using them
 constexpr int BT = 32;          // loads a lane issues before using them in the sweep kernels
 constexpr int DIAG_WAVES = 16;  // waves of a diagonal-step workgroup (they split the step's columns)
+constexpr int RECT_WAVES = 16;  // waves of a forward update-vector workgroup (they split the front's columns)
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -59,6 +61,14 @@ __device__ __forceinline__ void wave_sync_lds() {
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
+}
+
+// value of v in lane l of the wave (l uniform)
+__device__ __forceinline__ double lane_read(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
 // ------------------------------------------------------------------------------------------------ kernels
@@ -686,18 +696,18 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
   }
 }
 
-// Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 rows per workgroup (waves split the step's columns,
-// loads issued BT at a time); on the front's last step the rows are its update-vector rows and go straight to V.
-// task: (s, c0, first row, last)
+// Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 of the front's own rows below the step per
+// workgroup (waves split the step's columns, loads issued BT at a time). task: (s, c0, first row, 0)
 template <typename T>
 __device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
-                                                    double* __restrict__ Wg, double* __restrict__ V) {
+                                                    double* __restrict__ Wg) {
   __shared__ double xs[SB + BT], part[4][NB];
   const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
-  const long long m = sn_m(sd, s);
+  const long long m = k;  // rows beyond the own columns are left to the update-vector launch
+  const long long ld = sn_m(sd, s);
   const int wd = min(SB, k - c0);
   for (int q = threadIdx.x; q < SB + BT; q += 256) xs[q] = q < wd ? y[f + c0 + q] : 0.0;
   __syncthreads();
@@ -707,11 +717,11 @@ __device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
   const int q0 = (SB / 4) * wv, q1 = min(wd, q0 + SB / 4);
   if (row < m) {
     // loads past the wave's columns are clamped in bounds and weighted by zero, so all BT are issued unconditionally
-    const T* pr = L + sd.Loff[s] + (long long)c0 * m + row;
+    const T* pr = L + sd.Loff[s] + (long long)c0 * ld + row;
     for (int base = q0; base < q1; base += BT) {
       T v[BT];
 #pragma unroll
-      for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * m];
+      for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * ld];
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * (base + u < q1 ? xs[base + u] : 0.0);
@@ -721,9 +731,51 @@ __device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
   __syncthreads();
   if (threadIdx.x < NB && row < m) {
     double* W = Wg + sd.Woff[s];
-    const double v = W[row] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
-    if (tk.w) V[sd.Voff[s] + row - k] = v;
-    else W[row] = v;
+    W[row] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  }
+}
+
+// Forward, last part: V(row) = W(row) - L(row, 0 .. k) y_own for the update-vector rows (row >= k) of a wide front,
+// after all of its diagonal steps: one launch per level with many workgroups, 64 rows each, the waves splitting the
+// front's columns (loads issued BT at a time) and combining their partial sums in a fixed order.
+// task: (s, first row, 0, 0)
+template <typename T>
+__device__ __forceinline__ void fwdb_rect_item(const int4 tk, SymDev sd, const T* __restrict__ L,
+                                                   const double* __restrict__ y, const double* __restrict__ Wg,
+                                                   double* __restrict__ V) {
+  __shared__ double part[RECT_WAVES][NB];
+  const int s = tk.x;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long row = (long long)tk.y + lane;
+  const int per = (((k + RECT_WAVES - 1) / RECT_WAVES + BT - 1) / BT) * BT;
+  const int q0 = wv * per, q1 = min(k, q0 + per);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (q0 < q1) {
+    // every lane runs the loop (the lane reads below need all lanes' operands); rows past the front repeat its last
+    // row and are not stored
+    const T* pr = L + sd.Loff[s] + min(row, m - 1);
+    const double* yo = y + f;
+    for (int base = q0; base < q1; base += BT) {
+      // lane u < BT holds the batch's y_own[base + u]; each product takes it by a lane read
+      const double yv = yo[min(base + lane, q1 - 1)];
+      T v[BT];
+#pragma unroll
+      for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * m];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < BT; u++) acc[u & 3] += base + u < q1 ? (double)v[u] * lane_read(yv, u) : 0.0;
+    }
+  }
+  part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  __syncthreads();
+  if (threadIdx.x < NB && row < m) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < RECT_WAVES; q++) a += part[q][lane];
+    V[sd.Voff[s] + row - k] = Wg[sd.Woff[s] + row] - a;
   }
 }
 
@@ -738,39 +790,56 @@ __device__ __forceinline__ void bwdb_xb_item(const int4 tk, SymDev sd,
   V[sd.Voff[s] + i - k] = y[sd.sn_rows[sd.sn_rowptr[s] + i]];
 }
 
-// z_q = y_own[q] - L21(:, q)^T xb for the own columns q of this task: one wave per column, BT loads of each operand
-// in flight per lane. task: (s, first column, end column, 0), at most 4 columns.
+// z_q = y_own[q] - L21(:, q)^T xb for the own columns q of this task: each wave takes GEMV_COLS / 4 consecutive
+// columns and shares its loads of xb among them; GEMV_RB rows of every column in flight per lane. Columns past the
+// task are clamped to its last column and their sums dropped. task: (s, first column, end column, 0)
 template <typename T>
 __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T* __restrict__ L,
                                                    const double* __restrict__ y, const double* __restrict__ V,
                                                    double* __restrict__ Wg) {
+  constexpr int NQ = GEMV_COLS / 4;
   const int s = tk.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int q = tk.y + wv;
-  if (q >= tk.z) return;
+  const int q0 = tk.y + NQ * wv;
+  if (q0 >= tk.z) return;
+  const int nq = min(NQ, tk.z - q0);
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   const long long mb = m - k;
-  const T* cq = L + sd.Loff[s] + (long long)q * m + k;
+  const T* cq[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; j++) cq[j] = L + sd.Loff[s] + (long long)(q0 + min(j, nq - 1)) * m + k;
   const double* xb = V + sd.Voff[s];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[NQ][2];
+#pragma unroll
+  for (int j = 0; j < NQ; j++) acc[j][0] = acc[j][1] = 0.0;
   long long base = lane;
-  for (; base + 64 * (BT - 1) < mb; base += 64 * BT) {  // full batches: BT independent loads of each operand
-    T v[BT];
-    double xv[BT];
+  for (; base + 64 * (GEMV_RB - 1) < mb; base += 64 * GEMV_RB) {  // full batches: every load issued before use
+    double xv[GEMV_RB];
+    T v[NQ][GEMV_RB];
 #pragma unroll
-    for (int u = 0; u < BT; u++) {
-      v[u] = cq[base + 64 * u];
-      xv[u] = xb[base + 64 * u];
-    }
-    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of their uses (one wait, not one per pair)
+    for (int u = 0; u < GEMV_RB; u++) xv[u] = xb[base + 64 * u];
 #pragma unroll
-    for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * xv[u];
+    for (int j = 0; j < NQ; j++)
+#pragma unroll
+      for (int u = 0; u < GEMV_RB; u++) v[j][u] = cq[j][base + 64 * u];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < NQ; j++)
+#pragma unroll
+      for (int u = 0; u < GEMV_RB; u++) acc[j][u & 1] += (double)v[j][u] * xv[u];
   }
-  for (; base < mb; base += 64) acc[0] += (double)cq[base] * xb[base];
-  const double a = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
-  if (lane == 0) Wg[sd.Woff[s] + q] = y[f + q] - a;
+  for (; base < mb; base += 64) {
+    const double x = xb[base];
+#pragma unroll
+    for (int j = 0; j < NQ; j++) acc[j][0] += (double)cq[j][base] * x;
+  }
+#pragma unroll
+  for (int j = 0; j < NQ; j++) {
+    const double a = wave_sum(acc[j][0] + acc[j][1]);
+    if (lane == 0 && j < nq) Wg[sd.Woff[s] + q0 + j] = y[f + q0 + j] - a;
+  }
 }
 
 // Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...), lane per
@@ -869,8 +938,16 @@ __global__ __launch_bounds__(DIAG_WAVES * 64) void k_fwdb_diag(const int4* __res
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tasks, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
-                                                    double* __restrict__ Wg, double* __restrict__ V) {
-  fwdb_panel_item<T>(tasks[blockIdx.x], sd, L, y, Wg, V);
+                                                    double* __restrict__ Wg) {
+  fwdb_panel_item<T>(tasks[blockIdx.x], sd, L, y, Wg);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RECT_WAVES * 64) void k_fwdb_rect(const int4* __restrict__ tasks, SymDev sd,
+                                                               const T* __restrict__ L, const double* __restrict__ y,
+                                                               const double* __restrict__ Wg,
+                                                               double* __restrict__ V) {
+  fwdb_rect_item<T>(tasks[blockIdx.x], sd, L, y, Wg, V);
 }
 
 __global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks, SymDev sd,
@@ -898,9 +975,10 @@ __global__ __launch_bounds__(256) void k_bwdb_panel(const int4* __restrict__ tas
   bwdb_panel_item<T>(tasks[blockIdx.x], sd, L, y, Wg);
 }
 
+// grid-stride: the factor can hold more than 2^32 entries (one dispatch covers fewer work-items)
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) b[i] = (float)a[i];
 }
 
 __global__ void k_perm_in(const double* __restrict__ b, const int* __restrict__ perm, double* __restrict__ y, int n) {
@@ -1139,13 +1217,16 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
         int k = sn_kk(s);
         if (k <= c0) continue;
         int wd = std::min(SB, k - c0);
-        for (long long r0 = c0 + wd; r0 < sn_mm(s); r0 += NB)
-          fwt.push_back(make_int4(s, c0, (int)r0, c0 + wd == k ? 1 : 0));
+        for (int r0 = c0 + wd; r0 < k; r0 += NB) fwt.push_back(make_int4(s, c0, r0, 0));
       }
       pn.end = (long long)fwt.size();
       L.fwd_diag.push_back(d);
       L.fwd_panel.push_back(pn);
     }
+    L.fwd_rect.begin = (long long)fwt.size();
+    for (int s : big)
+      for (long long r0 = sn_kk(s); r0 < sn_mm(s); r0 += NB) fwt.push_back(make_int4(s, (int)r0, 0, 0));
+    L.fwd_rect.end = (long long)fwt.size();
     // backward
     L.bwd_xb.begin = (long long)bwt.size();
     for (int s : big)
@@ -1318,7 +1399,7 @@ int SpdDevice::factorize_device(const double* d_vals, int precision) {
   if (rc == 0 && precision == 2) {
     long long n = std::max<long long>(1, S_.Lsize);
     L32_.alloc(n);
-    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, 256)), dim3(256), 0, stream_, (const double*)L_.p, L32_.p, n);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_stride(n, 256)), dim3(256), 0, stream_, (const double*)L_.p, L32_.p, n);
     IFEA_HIP(hipGetLastError());
   }
   IFEA_HIP(hipEventRecord(e1, stream_));
@@ -1357,8 +1438,12 @@ void SpdDevice::sweeps(const T* L) {
     launch(k_fwdb_gather, lv.fwd_gather, fwd_tasks_.p, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.fwd_diag.size(); t++) {
       launch_diag(k_fwdb_diag, lv.fwd_diag[t], fwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
-      launch(k_fwdb_panel<T>, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p, V_.p);
+      launch(k_fwdb_panel<T>, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
+    if (lv.fwd_rect.end > lv.fwd_rect.begin)
+      hipLaunchKernelGGL(k_fwdb_rect<T>, dim3((unsigned)(lv.fwd_rect.end - lv.fwd_rect.begin)), dim3(RECT_WAVES * 64),
+                         0, stream_, fwd_tasks_.p + lv.fwd_rect.begin, sd_, L, (const double*)y_.p,
+                         (const double*)W_.p, V_.p);
   }
   for (size_t l = levels_.size(); l-- > 0;) {
     const Level& lv = levels_[l];

@@ -52,6 +52,7 @@ class StepDevice {
   bool need_analyze_ = true, have_factor_ = false;
   int pcg_failures_ = 0, pcg_skip_ = 0;
   bool fp32_unstable_ = false;  // an FP32 factorisation was not positive definite: factor in FP64 from now on
+  bool fp32_sweeps_unstable_ = false;  // PCG on a fresh FP64 factor failed with FP32 sweeps: FP64 sweeps from now on
   // topology (device)
   DBuf<int> tets_, nt_start_, nt_list_, colPtr_, rowIdx_, nbr_start_, nbr_, p_start_, p_node_, diagpos_;
   DBuf<int> nf_start_, nf_list_, faces_, face_owner_;

@@ -731,8 +731,10 @@ int StepDevice::solve_linear(double* b, double* x, ifea_step_stats* st) {
     if (++pcg_failures_ >= 2) pcg_skip_ = std::min(64, 1 << (pcg_failures_ - 1));
   }
   // precision 1: FP32 factor; after an FP32 factor that is not positive definite (penalty scale beyond FP32), FP64
-  // factors with an FP32 copy for the solves. Both are refined by PCG in FP64 against the FP64 matrix.
-  int prec = P_.precision == 1 ? (fp32_unstable_ ? 2 : 1) : (P_.precision == 2 ? 2 : 0);
+  // factors with an FP32 copy for the solves. Both are refined by PCG in FP64 against the FP64 matrix. When the
+  // refinement of a fresh factor with FP32 sweeps stalls, the FP32 copy does not resolve this matrix: plain FP64
+  // from then on (trying the copy again after every factorisation would cost a stalled PCG each time).
+  int prec = P_.precision == 0 ? 0 : (fp32_sweeps_unstable_ ? 0 : (P_.precision == 1 && !fp32_unstable_ ? 1 : 2));
   int rc = refactor(prec, st);
   if (rc == -4 && prec == 1) {
     fp32_unstable_ = true;
@@ -748,6 +750,7 @@ int StepDevice::solve_linear(double* b, double* x, ifea_step_stats* st) {
     if (st) st->fp64_fallbacks++;
     if (prec == 2) {
       spd_->use_fp64_solves(true);
+      fp32_sweeps_unstable_ = true;
     } else {
       rc = refactor(0, st);
       if (rc) { have_factor_ = false; return rc; }

@@ -13,7 +13,7 @@ from impressionfea_amd import spd
 from impressionfea_amd.config import upset_small
 from impressionfea_amd.errors import MatrixNotPositiveDefiniteError
 from impressionfea_amd.forging3d import Forging3DSolver
-from impressionfea_amd.mesh import kuhn_cylinder
+from impressionfea_amd.mesh import TetMesh, build_pattern, kuhn_cylinder
 
 pytestmark = pytest.mark.gpu
 
@@ -108,6 +108,44 @@ def test_spd_mixed_precision_refines(c1_system):
         # FP64 refinement reaches the FP64 factor's own backward error (~1e-9 on this penalty system)
         assert np.linalg.norm(As @ x - b) <= 1e-8 * np.linalg.norm(b)
         assert st["refine_iterations"] >= 1 or st["fp64_fallbacks"] >= 1
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_spd_wide_fronts(precision):
+    """Two-ring pattern of a 14x14x20 Kuhn cylinder (14,175 DOF) under the geometric ordering: its top fronts span
+    several 512-column sweep steps (dense trailing triangle of 2,160 columns) and carry update-vector rectangles, the
+    blocked multi-launch sweep path. Random diagonally dominant SPD values; the solution must match scipy's sparse LU
+    (FP64 factor), or reach the same backward error after refinement with the FP32 copy of the factor."""
+    import scipy.sparse.linalg as spla
+    x, t, ox, oy = kuhn_cylinder(40, 60, 14, 20, seed=1)
+    m = TetMesh(x.shape[0], t, ox, oy)
+    _, _, cp, ri = build_pattern(m)
+    n = 3 * m.nn
+    rng = np.random.default_rng(7)
+    A = sp.csc_matrix((rng.uniform(-1, 1, cp[-1]), ri, cp), shape=(n, n))
+    A = (0.5 * (A + A.T)).tocsc()
+    A.sort_indices()
+    col = np.repeat(np.arange(n), np.diff(A.indptr))
+    d = np.asarray(abs(A).sum(axis=0)).ravel() + 1.0
+    A.data[A.indices == col] = d[col[A.indices == col]]
+    assert np.array_equal(A.indptr, cp) and np.array_equal(A.indices, ri)
+    b = rng.standard_normal(n)
+    xr = spla.splu(A.tocsc()).solve(b)
+    with spd.HipSpdSolver(precision=precision, refine_tol=1e-14) as h:
+        h.set_coordinates(np.repeat(x, 3, axis=0))
+        h.Analyze(n, cp, ri)
+        _, _, cc = h.symbolic()
+        tail = 0
+        while tail < n and cc[n - 1 - tail] == tail + 1:
+            tail += 1
+        assert tail > 1024
+        h.Factorize(A.data)
+        for _ in range(2):  # the second solve replays the captured sweep graph
+            xs = h.Solve(b)
+            assert np.linalg.norm(A @ xs - b) <= 1e-13 * np.linalg.norm(b)
+            assert np.linalg.norm(xs - xr) <= 1e-12 * np.linalg.norm(xr)
+        if precision == 2:
+            assert h.stats()["refine_iterations"] >= 1
 
 
 def test_spd_not_positive_definite(c1_system):
