@@ -24,7 +24,7 @@ struct SymDev {
   const int* child_ptr;
   const int* child_list;
   const long long* Ioff;   // inverse diagonal blocks (64x64 per 64 columns) of each supernode
-  const long long* Xoff;   // wide fronts: inverse diagonal blocks of the sweeps (256x256 per 256 columns)
+  const long long* Xoff;   // wide fronts: explicit inverses of the sweep steps' diagonal blocks (SB x SB, last one smaller)
   const long long* Goff;   // wide fronts: gptr + Goff[s] has m + 1 entries into gsrc
   const long long* gptr;   // row-wise child contributions (indices into the update vectors V), in child order
   const long long* gsrc;

@@ -23,13 +23,15 @@ constexpr int EXT_TARGET = 8192;
 constexpr long long ZERO_TARGET = 1 << 16;  // update-block entries zeroed per workgroup
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
-constexpr int SB = 512;         // diagonal block of the blocked sweeps (explicit inverse per block)
+constexpr int SB = 2048;        // diagonal block of the blocked sweeps (explicit inverse per block)
 constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step
 constexpr int GEMV_COLS = 16;   // columns per workgroup of the backward GEMV (four waves, GEMV_COLS / 4 columns each)
 constexpr int GEMV_RB = 8;      // rows of each column a backward-GEMV lane loads before using them
 constexpr int BT = 32;          // loads a lane issues before using them in the sweep kernels
 constexpr int DIAG_WAVES = 16;  // waves of a diagonal-step workgroup (they split the step's columns)
-constexpr int RECT_WAVES = 16;  // waves of a forward update-vector workgroup (they split the front's columns)
+constexpr int RECT_WAVES = 16;  // waves of a forward update-vector or panel workgroup (they split the columns)
+constexpr int BP_ROWS = 512;    // rows of a backward panel step loaded per batch (times 4 columns per wave)
+static_assert(RECT_WAVES == DIAG_WAVES, "the wide sweep launches share one workgroup size");
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -175,20 +177,8 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
     colb[j & 1][i] = lij;
     wave_sync_lds();
     const double* cb = colb[j & 1];
-    // eight columns at a time, all loads before the stores: the row and the column vector are separate arrays, but
-    // stated this way no store has to wait for the loads of the next columns
-    for (int q0 = j + 1; q0 < NB; q0 += 8) {
-      double cv[8], rv[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int q = min(q0 + u, NB - 1);
-        cv[u] = cb[q];
-        rv[u] = row[q];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (q0 + u < NB) row[q0 + u] = rv[u] - lij * cv[u];
-    }
+#pragma unroll 8
+    for (int q = j + 1; q < NB; q++) row[q] -= lij * cb[q];
     wave_sync_lds();
   }
   for (int q = i + 1; q < NB; q++) row[q] = 0.0;
@@ -198,23 +188,15 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   for (int q = 0; q < NB; q++)
     if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)row[q];
   if (bad && i == 0) atomicOr(err, 1);
-  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; four partial sums per row keep
-  // the dependent chain of a row a quarter as long
+  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]
   const int c = i;
   double x[NB];
 #pragma unroll
   for (int r = 0; r < NB; r++) {
-    double a0 = (r == c) ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double acc = (r == c) ? 1.0 : 0.0;
 #pragma unroll
-    for (int p = 0; p + 3 < r; p += 4) {
-      a0 -= D[r][p] * x[p];
-      a1 -= D[r][p + 1] * x[p + 1];
-      a2 -= D[r][p + 2] * x[p + 2];
-      a3 -= D[r][p + 3] * x[p + 3];
-    }
-#pragma unroll
-    for (int p = r & ~3; p < r; p++) a0 -= D[r][p] * x[p];
-    x[r] = ((a0 + a1) + (a2 + a3)) / D[r][r];
+    for (int p = 0; p < r; p++) acc -= D[r][p] * x[p];
+    x[r] = acc / D[r][r];
   }
   wave_sync_lds();
   // transpose through LDS so that the stores of Lt[c][r] = Linv[r][c] are contiguous over the lanes
@@ -565,10 +547,16 @@ __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev
 
 
 // ---------------------------------------------------------------------------------------- blocked sweeps (wide fronts)
-// Wide fronts are swept in steps of SB = 512 columns. Each step is two launches: a diagonal launch x_t = X_t w_t with
-// X_t the explicit inverse of the step's SB x SB diagonal block (built once per factorization from the kept 64x64
-// inverses, k_inv_diag), and a panel launch that applies the step to every other row (forward) or column (backward) of
-// the front with many workgroups. X_t is stored column by column: X[c * SB + r] = inv(L_tt)[r][c].
+// Wide fronts are swept in steps of SB = 2048 columns. Each step is two launches: a diagonal launch x_t = X_t w_t with
+// X_t the explicit inverse of the step's wd x wd diagonal block (wd = min(SB, k - c0); built once per factorization
+// from the kept 64x64 inverses, k_inv_diag), and a panel launch that applies the step to the front's later own rows
+// (forward) or earlier own columns (backward) with many workgroups. X_t is stored column by column with leading
+// dimension wd, X[c * wd + r] = inv(L_tt)[r][c], and XT holds its transpose; every step but a front's last is full,
+// so the step at c0 starts SB * SB * (c0 / SB) entries into the front's blocks.
+
+__device__ __forceinline__ long long x_step_off(const SymDev& sd, int s, int c0) {
+  return sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+}
 
 // Inverse of one SB-column diagonal block, one 64-column strip j per workgroup: X_jj = Linv_j and, going down,
 // X_ij = -Linv_i * sum_{p=j}^{i-1} L_ip X_pj. Thread (r, q): row r of the 64-row tile, columns [16 q, 16 q + 16).
@@ -577,7 +565,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
                                                 const double* __restrict__ Linv, double* __restrict__ X,
                                                 double* __restrict__ XT) {
-  __shared__ double Ts[NB][NB + 1];
+  __shared__ double Ts[NB][NB + 1], Ls[NB][NB + 1];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y, j = tk.z;
   const int k = sn_k(sd, s);
@@ -585,20 +573,25 @@ __global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks
   const int wd = min(SB, k - c0);
   const int nsub = (wd + NB - 1) / NB;
   const T* P = L + sd.Loff[s];
-  double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
-  double* XTb = XT + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  const long long ld = wd;
+  double* Xb = X + x_step_off(sd, s, c0);
+  double* XTb = XT + x_step_off(sd, s, c0);
   const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
   const double* Ltj = Linv + sd.Ioff[s] + (long long)((c0 + NB * j) / NB) * NB * NB;
+  // only entries inside the wd x wd block are stored (the last strip of a step can be narrower than 64)
   for (int cc = 0; cc < 16; cc++) {
-    const int c = 16 * q + cc;
-    double* xc = Xb + (long long)(NB * j + c) * SB;
+    const int c = NB * j + 16 * q + cc;
+    if (c >= wd) break;
+    double* xc = Xb + (long long)c * ld;
     for (int i = 0; i < j; i++) {
       xc[NB * i + r] = 0.0;
-      XTb[(long long)(NB * i + r) * SB + NB * j + c] = 0.0;
+      XTb[(long long)(NB * i + r) * ld + c] = 0.0;
     }
-    const double d = Ltj[c * NB + r];
-    xc[NB * j + r] = d;
-    XTb[(long long)(NB * j + r) * SB + NB * j + c] = d;
+    if (NB * j + r < wd) {
+      const double d = Ltj[(c - NB * j) * NB + r];
+      xc[NB * j + r] = d;
+      XTb[(long long)(NB * j + r) * ld + c] = d;
+    }
   }
   __syncthreads();
   for (int i = j + 1; i < nsub; i++) {
@@ -607,14 +600,25 @@ __global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks
     for (int cc = 0; cc < 16; cc++) acc[cc] = 0.0;
     const bool rok = NB * i + r < wd;
     for (int p0 = j; p0 < i; p0++) {
-      for (int idx = threadIdx.x; idx < NB * NB; idx += 256) {
-        const int pp = idx & 63, c = idx >> 6;
-        Ts[pp][c] = Xb[(long long)(NB * j + c) * SB + NB * p0 + pp];
+      // stage X_pj and the panel tile L_ip (rows of strip i, columns of strip p0) with all loads issued at once
+      double xv[16], lv[16];
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        const int idx = threadIdx.x + 256 * t;
+        const int rr = idx & 63, cq = idx >> 6;
+        xv[t] = Xb[(long long)(NB * j + cq) * ld + NB * p0 + rr];  // strips p0 < i < nsub and j < i are full
+        lv[t] = NB * i + rr < wd ? (double)P[(long long)(c0 + NB * p0 + cq) * m + c0 + NB * i + rr] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        const int idx = threadIdx.x + 256 * t;
+        const int rr = idx & 63, cq = idx >> 6;
+        Ts[rr][cq] = xv[t];
+        Ls[rr][cq] = lv[t];
       }
       __syncthreads();
-      const T* lrow = P + (long long)(c0 + NB * p0) * m + c0 + NB * i + r;
       for (int pp = 0; pp < NB; pp++) {
-        const double a = rok ? (double)lrow[(long long)pp * m] : 0.0;
+        const double a = Ls[r][pp];
 #pragma unroll
         for (int cc = 0; cc < 16; cc++) acc[cc] += a * Ts[pp][16 * q + cc];
       }
@@ -631,10 +635,12 @@ __global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks
 #pragma unroll
       for (int cc = 0; cc < 16; cc++) acc[cc] += a * Ts[pp][16 * q + cc];
     }
+    if (rok) {
 #pragma unroll
-    for (int cc = 0; cc < 16; cc++) {
-      Xb[(long long)(NB * j + 16 * q + cc) * SB + NB * i + r] = -acc[cc];
-      XTb[(long long)(NB * i + r) * SB + NB * j + 16 * q + cc] = -acc[cc];
+      for (int cc = 0; cc < 16; cc++) {
+        Xb[(long long)(NB * j + 16 * q + cc) * ld + NB * i + r] = -acc[cc];
+        XTb[(long long)(NB * i + r) * ld + NB * j + 16 * q + cc] = -acc[cc];
+      }
     }
     __syncthreads();
   }
@@ -668,12 +674,13 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
-  const double* Xb = X + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  const double* Xb = X + x_step_off(sd, s, c0);
   const double* w = Wg + sd.Woff[s] + c0;
   for (int q = threadIdx.x; q < SB + BT; q += DIAG_WAVES * 64) ws[q] = q < wd ? w[q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = NB * rb + lane;
+  const int rr = min(r, wd - 1);  // rows past the block read its last row and are not stored
   const int cend = min(wd, NB * rb + NB);
   const int per = (((cend + DIAG_WAVES - 1) / DIAG_WAVES + BT - 1) / BT) * BT;
   const int lo = wv * per, hi = min(cend, lo + per);
@@ -681,7 +688,7 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
   for (int base = lo; base < hi; base += BT) {
     double v[BT];
 #pragma unroll
-    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * SB + r];
+    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * wd + rr];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < BT; u++) acc[u & 3] += base + u < hi ? v[u] * ws[base + u] : 0.0;
@@ -697,41 +704,45 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
 }
 
 // Forward panel step: W(row) -= L(row, c0 .. c0+wd) x_t for 64 of the front's own rows below the step per
-// workgroup (waves split the step's columns, loads issued BT at a time). task: (s, c0, first row, 0)
+// workgroup; the waves split the step's columns (loads issued BT at a time, x_t taken by lane reads) and combine
+// their partial sums in a fixed order. task: (s, c0, first row, 0)
 template <typename T>
 __device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
                                                     double* __restrict__ Wg) {
-  __shared__ double xs[SB + BT], part[4][NB];
+  __shared__ double part[RECT_WAVES][NB];
   const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
-  const long long m = k;  // rows beyond the own columns are left to the update-vector launch
   const long long ld = sn_m(sd, s);
   const int wd = min(SB, k - c0);
-  for (int q = threadIdx.x; q < SB + BT; q += 256) xs[q] = q < wd ? y[f + c0 + q] : 0.0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long long row = (long long)tk.z + lane;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long row = (long long)tk.z + lane;  // own rows only: < k
+  const int per = (((wd + RECT_WAVES - 1) / RECT_WAVES + BT - 1) / BT) * BT;
+  const int q0 = c0 + wv * per, q1 = min(c0 + wd, q0 + per);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int q0 = (SB / 4) * wv, q1 = min(wd, q0 + SB / 4);
-  if (row < m) {
-    // loads past the wave's columns are clamped in bounds and weighted by zero, so all BT are issued unconditionally
-    const T* pr = L + sd.Loff[s] + (long long)c0 * ld + row;
+  if (q0 < q1) {
+    // every lane runs the loop (the lane reads need all lanes' operands); rows past the front's own rows repeat its
+    // last own row and are not stored
+    const T* pr = L + sd.Loff[s] + min(row, (long long)k - 1);
+    const double* yo = y + f;
     for (int base = q0; base < q1; base += BT) {
+      const double yv = yo[min(base + lane, q1 - 1)];  // lane u < BT: x_t entry base + u
       T v[BT];
 #pragma unroll
       for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * ld];
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int u = 0; u < BT; u++) acc[u & 3] += (double)v[u] * (base + u < q1 ? xs[base + u] : 0.0);
+      for (int u = 0; u < BT; u++) acc[u & 3] += base + u < q1 ? (double)v[u] * lane_read(yv, u) : 0.0;
     }
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
-  if (threadIdx.x < NB && row < m) {
-    double* W = Wg + sd.Woff[s];
-    W[row] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  if (threadIdx.x < NB && row < k) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < RECT_WAVES; q++) a += part[q][lane];
+    Wg[sd.Woff[s] + row] -= a;
   }
 }
 
@@ -843,7 +854,7 @@ __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T
 }
 
 // Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...), lane per
-// column over the transposed copy XT[r * SB + c] = X_t[r][c]; the 16 waves split the rows r >= 64 tk.z.
+// column over the transposed copy XT[r * wd + c] = X_t[r][c]; the 16 waves split the rows r >= 64 tk.z.
 // task: (s, c0, column block, 0)
 __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
                                                    const double* __restrict__ XT, double* __restrict__ y,
@@ -853,12 +864,13 @@ __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
-  const double* Xb = XT + sd.Xoff[s] + (long long)(c0 / SB) * SB * SB;
+  const double* Xb = XT + x_step_off(sd, s, c0);
   const double* z = Wg + sd.Woff[s] + c0;
   for (int q = threadIdx.x; q < SB + BT; q += DIAG_WAVES * 64) zs[q] = q < wd ? z[q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c = NB * cb + lane;
+  const int cc = min(c, wd - 1);  // columns past the block read its last column and are not stored
   const int span = wd - NB * cb;
   const int per = (((span + DIAG_WAVES - 1) / DIAG_WAVES + BT - 1) / BT) * BT;
   const int lo = NB * cb + wv * per, hi = min(wd, lo + per);
@@ -866,7 +878,7 @@ __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
   for (int base = lo; base < hi; base += BT) {
     double v[BT];
 #pragma unroll
-    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * SB + c];
+    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * wd + cc];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < BT; u++) acc[u & 3] += base + u < hi ? v[u] * zs[base + u] : 0.0;
@@ -882,43 +894,49 @@ __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
 }
 
 // Backward panel step: z_j -= L(c0 .. c0+wd, j)^T x_t for the columns j of this task (< c0): wave wv takes columns
-// j0 + wv + 4u, lanes over the step's rows, all loads of a lane (4 columns x SB/64 rows) issued before the
-// reductions.
-// task: (s, c0, first column, end column), at most 16 columns.
+// j0 + wv + 4u, lanes over the step's rows, BP_ROWS / 64 rows of each of its 4 columns loaded per batch before the
+// products. task: (s, c0, first column, end column), at most 16 columns.
 template <typename T>
 __device__ __forceinline__ void bwdb_panel_item(const int4 tk, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
                                                     double* __restrict__ Wg) {
-  __shared__ double xs[SB];
+  constexpr int NCB = BP_ROWS / 64;
+  __shared__ double xs[SB + BP_ROWS];
   const int s = tk.x, c0 = tk.y;
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   const int wd = min(SB, k - c0);
-  for (int q = threadIdx.x; q < SB; q += 256) xs[q] = q < wd ? y[f + c0 + q] : 0.0;
+  for (int q = threadIdx.x; q < SB + BP_ROWS; q += 256) xs[q] = q < wd ? y[f + c0 + q] : 0.0;
   __syncthreads();
   const T* P = L + sd.Loff[s];
   double* z = Wg + sd.Woff[s];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  constexpr int NC = SB / 64;  // row chunks of a step
-  T v[4][NC];
+  // columns past the task and rows past the step are clamped in bounds (weighted by zero or not stored)
+  const T* cj[4];
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
-    // columns past the task and rows past the step are clamped in bounds (weighted by zero or not stored)
-    const int j = min(tk.z + wv + 4 * u, tk.w - 1);
-    const T* cj = P + (long long)j * m + c0;
+  for (int u = 0; u < 4; u++) cj[u] = P + (long long)min(tk.z + wv + 4 * u, tk.w - 1) * m + c0;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t0 = 0; t0 < wd; t0 += BP_ROWS) {
+    T v[4][NCB];
 #pragma unroll
-    for (int t = 0; t < NC; t++) v[u][t] = cj[min(lane + 64 * t, wd - 1)];
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int t = 0; t < NCB; t++) v[u][t] = cj[u][min(t0 + lane + 64 * t, wd - 1)];
+    __builtin_amdgcn_sched_barrier(0);  // the batch's loads in flight before the first product
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int t = 0; t < NCB; t++) {
+        const int r = t0 + lane + 64 * t;
+        a[u] += r < wd ? (double)v[u][t] * xs[r] : 0.0;
+      }
   }
-  __builtin_amdgcn_sched_barrier(0);  // all loads of the lane in flight before the first reduction
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int j = tk.z + wv + 4 * u;
-    double a = 0.0;
-#pragma unroll
-    for (int t = 0; t < NC; t++) a += lane + 64 * t < wd ? (double)v[u][t] * xs[lane + 64 * t] : 0.0;
-    a = wave_sum(a);
-    if (lane == 0 && j < tk.w) z[j] -= a;
+    const double t = wave_sum(a[u]);
+    if (lane == 0 && j < tk.w) z[j] -= t;
   }
 }
 
@@ -936,7 +954,7 @@ __global__ __launch_bounds__(DIAG_WAVES * 64) void k_fwdb_diag(const int4* __res
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwdb_panel(const int4* __restrict__ tasks, SymDev sd,
+__global__ __launch_bounds__(RECT_WAVES * 64) void k_fwdb_panel(const int4* __restrict__ tasks, SymDev sd,
                                                     const T* __restrict__ L, const double* __restrict__ y,
                                                     double* __restrict__ Wg) {
   fwdb_panel_item<T>(tasks[blockIdx.x], sd, L, y, Wg);
@@ -1162,7 +1180,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   for (int s = 0; s < ns; s++) {
     long long k = sn_kk(s), m = sn_mm(s);
     isbig[s] = k > BIG_K || k * m > BIG_PANEL;
-    Xoff[s + 1] = Xoff[s] + (isbig[s] ? ((k + SB - 1) / SB) * SB * SB : 0);
+    Xoff[s + 1] = Xoff[s] + (isbig[s] ? (long long)(k / SB) * SB * SB + (long long)(k % SB) * (k % SB) : 0);
     if (!isbig[s]) continue;
     // row-wise inverse of the children's relative maps, contributions of a row in child order
     Goff[s] = (long long)gptr.size();
@@ -1438,7 +1456,7 @@ void SpdDevice::sweeps(const T* L) {
     launch(k_fwdb_gather, lv.fwd_gather, fwd_tasks_.p, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.fwd_diag.size(); t++) {
       launch_diag(k_fwdb_diag, lv.fwd_diag[t], fwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
-      launch(k_fwdb_panel<T>, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p);
+      launch_diag(k_fwdb_panel<T>, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
     if (lv.fwd_rect.end > lv.fwd_rect.begin)
       hipLaunchKernelGGL(k_fwdb_rect<T>, dim3((unsigned)(lv.fwd_rect.end - lv.fwd_rect.begin)), dim3(RECT_WAVES * 64),
