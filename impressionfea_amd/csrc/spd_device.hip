@@ -149,6 +149,7 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   T* P = L + sd.Loff[s];
   __shared__ double D[NB][NB + 1];
   __shared__ double colb[2][NB];
+  __shared__ double rdiag[NB];  // 1 / L[j][j]
   const int i = threadIdx.x;
   const T* prow = P + (long long)c0 * m + c0 + i;
   // the 64 column loads of the block are independent: issue them 16 at a time
@@ -168,17 +169,31 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   wave_sync_lds();
   bool bad = false;
   double* row = &D[i][0];
-  for (int j = 0; j < NB; j++) {
-    const double piv = D[j][j];
-    bad |= !(piv > 0.0) || !isfinite(piv);
-    const double d = sqrt(piv > 0.0 ? piv : 1.0);
-    const double lij = (i > j) ? row[j] / d : ((i == j) ? d : 0.0);
-    row[j] = lij;
-    colb[j & 1][i] = lij;
+  // two columns per pass over the trailing rows: column j, its update of column j + 1, column j + 1, then one pass
+  // that applies both (in the order of two single-column passes, so the results are the same)
+  for (int j = 0; j < NB; j += 2) {
+    const double piv0 = D[j][j];
+    bad |= !(piv0 > 0.0) || !isfinite(piv0);
+    const double d0 = sqrt(piv0 > 0.0 ? piv0 : 1.0);
+    const double l0 = (i > j) ? row[j] / d0 : ((i == j) ? d0 : 0.0);
+    if (i == j) rdiag[j] = 1.0 / d0;
+    row[j] = l0;
+    colb[0][i] = l0;
     wave_sync_lds();
-    const double* cb = colb[j & 1];
+    row[j + 1] -= l0 * colb[0][j + 1];
+    wave_sync_lds();
+    const double piv1 = D[j + 1][j + 1];
+    bad |= !(piv1 > 0.0) || !isfinite(piv1);
+    const double d1 = sqrt(piv1 > 0.0 ? piv1 : 1.0);
+    const double l1 = (i > j + 1) ? row[j + 1] / d1 : ((i == j + 1) ? d1 : 0.0);
+    if (i == j + 1) rdiag[j + 1] = 1.0 / d1;
+    row[j + 1] = l1;
+    colb[1][i] = l1;
+    wave_sync_lds();
+    const double* c0b = colb[0];
+    const double* c1b = colb[1];
 #pragma unroll 8
-    for (int q = j + 1; q < NB; q++) row[q] -= lij * cb[q];
+    for (int q = j + 2; q < NB; q++) row[q] = (row[q] - l0 * c0b[q]) - l1 * c1b[q];
     wave_sync_lds();
   }
   for (int q = i + 1; q < NB; q++) row[q] = 0.0;
@@ -188,15 +203,23 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   for (int q = 0; q < NB; q++)
     if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)row[q];
   if (bad && i == 0) atomicOr(err, 1);
-  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]
+  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; four partial sums keep a row's
+  // dependent chain a quarter as long, and the reciprocal of the diagonal replaces the division
   const int c = i;
   double x[NB];
 #pragma unroll
   for (int r = 0; r < NB; r++) {
-    double acc = (r == c) ? 1.0 : 0.0;
+    double a0 = (r == c) ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
-    for (int p = 0; p < r; p++) acc -= D[r][p] * x[p];
-    x[r] = acc / D[r][r];
+    for (int p = 0; p + 3 < r; p += 4) {
+      a0 -= D[r][p] * x[p];
+      a1 -= D[r][p + 1] * x[p + 1];
+      a2 -= D[r][p + 2] * x[p + 2];
+      a3 -= D[r][p + 3] * x[p + 3];
+    }
+#pragma unroll
+    for (int p = r & ~3; p < r; p++) a0 -= D[r][p] * x[p];
+    x[r] = ((a0 + a1) + (a2 + a3)) * rdiag[r];
   }
   wave_sync_lds();
   // transpose through LDS so that the stores of Lt[c][r] = Linv[r][c] are contiguous over the lanes
