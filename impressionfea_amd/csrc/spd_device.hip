@@ -28,10 +28,10 @@ constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel ste
 constexpr int GEMV_COLS = 16;   // columns per workgroup of the backward GEMV (four waves, GEMV_COLS / 4 columns each)
 constexpr int GEMV_RB = 8;      // rows of each column a backward-GEMV lane loads before using them
 constexpr int BT = 32;          // loads a lane issues before using them in the sweep kernels
-constexpr int DIAG_WAVES = 16;  // waves of a diagonal-step workgroup (they split the step's columns)
+constexpr int DIAG_WAVES = 4;   // waves of a diagonal-step workgroup (one pair of rows or columns each)
+constexpr int DB = 8;           // loads of X (and of the vector) a lane issues before using them in a diagonal step
 constexpr int RECT_WAVES = 16;  // waves of a forward update-vector or panel workgroup (they split the columns)
 constexpr int BP_ROWS = 512;    // rows of a backward panel step loaded per batch (times 4 columns per wave)
-static_assert(RECT_WAVES == DIAG_WAVES, "the wide sweep launches share one workgroup size");
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
 template <typename T> struct Mfma;
@@ -350,10 +350,12 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 
 
 // Outer trailing update with 128x128 output tiles (mode 1 of k_syrk): NT/64 waves, two along the rows (64 each) and
-// NT/128 along the columns, each wave a block of 16x16 MFMA tiles (64x64 with 4 waves, 64x32 with 8 waves: the FP64
-// accumulators of 8 waves fit two waves per SIMD). X is streamed in 16-column chunks through a double-buffered LDS
-// tile; the global loads of the next two chunks are in flight (in two register sets) while the current chunk feeds
-// the MFMAs, one barrier per chunk. task: (supernode s, row tile I, column tile J, 0), I >= J.
+// NT/128 along the columns, each wave a block of 16x16 MFMA tiles (64x32 with 8 waves). X is streamed in 16-column
+// chunks through a double-buffered LDS tile; the global loads of the next chunk are in flight while the current
+// chunk feeds the MFMAs, one barrier per chunk. Register use is kept at or below 128 VGPRs (one chunk of loads in
+// registers, 32-bit offsets from a wave-uniform column base) so that two 8-wave workgroups share a CU and one
+// workgroup's MFMAs run while the other waits at its barrier. task: (supernode s, row tile I, column tile J, 0),
+// I >= J. Fronts are below 2^27 rows (checked at analysis), so chunk offsets fit in 32 bits.
 template <typename T, int NT, int KC>
 __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
                                                  T* __restrict__ L, T* __restrict__ U) {
@@ -368,35 +370,36 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
   const int4 tk = tasks[xcd * per_x + min(xcd, extra) + slot];
   const int s = tk.x;
   const int k = sn_k(sd, s);
-  const long long m = sn_m(sd, s);
+  const int m = (int)sn_m(sd, s);
   const int kw = min(OB, k - kb);
-  const long long rb = kb + kw;
-  const T* P = L + sd.Loff[s];
+  const int rb = kb + kw;
+  const T* P = L + sd.Loff[s] + (long long)kb * m;  // column kb of the front
   __shared__ T Xi[2][KC][LD];
   __shared__ T Xj[2][KC][LD];
-  const long long i0 = rb + 128LL * tk.y, j0 = rb + 128LL * tk.z;
+  const int i0 = rb + 128 * tk.y, j0 = rb + 128 * tk.z;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ib = (wv & 1) * 64, jb = (wv >> 1) * (128 / WJ);
   const int l16 = lane & 15;
   const int nch = (kw + KC - 1) / KC;
-  T ai[PER], aj[PER], bi[PER], bj[PER];  // two chunks in flight: loads have two chunks of MFMAs to arrive
-  auto fetch = [&](int c, T* ri, T* rj) {
+  // a thread stages tile row rr of columns q0, q0 + NT/128, ... of every chunk
+  const int rr = threadIdx.x & 127, q0 = threadIdx.x >> 7;
+  const bool iok = i0 + rr < m, jok = j0 + rr < m;
+  T ri[PER], rj[PER];
+  auto fetch = [&](int c) {
+    const T* cb = P + (long long)(c * KC) * m;  // wave-uniform base of the chunk
 #pragma unroll
     for (int t = 0; t < PER; t++) {
-      int idx = threadIdx.x + NT * t;
-      int q = idx >> 7, rr = idx & 127;
-      int col = c * KC + q;
-      const T* cp = P + (long long)(kb + col) * m;
-      long long ra = i0 + rr, rc = j0 + rr;
-      ri[t] = (col < kw && ra < m) ? cp[ra] : T(0);
-      rj[t] = (col < kw && rc < m) ? cp[rc] : T(0);
+      const int q = q0 + (NT / 128) * t;
+      const bool cok = c * KC + q < kw;
+      const unsigned off = (unsigned)(q * m);
+      ri[t] = (cok && iok) ? cb[off + (unsigned)(i0 + rr)] : T(0);
+      rj[t] = (cok && jok) ? cb[off + (unsigned)(j0 + rr)] : T(0);
     }
   };
-  auto stage = [&](int buf, const T* ri, const T* rj) {
+  auto stage = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < PER; t++) {
-      int idx = threadIdx.x + NT * t;
-      int q = idx >> 7, rr = idx & 127;
+      const int q = q0 + (NT / 128) * t;
       Xi[buf][q][rr] = ri[t];
       Xj[buf][q][rr] = rj[t];
     }
@@ -422,20 +425,17 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
         for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
     }
   };
-  fetch(0, ai, aj);
-  stage(0, ai, aj);
-  if (1 < nch) fetch(1, ai, aj);
-  if (2 < nch) fetch(2, bi, bj);
+  fetch(0);
+  stage(0);
+  if (1 < nch) fetch(1);
   __syncthreads();
-  for (int c = 0; c < nch; c += 2) {
-    compute(0);                                  // chunk c
-    if (c + 1 < nch) stage(1, ai, aj);           // chunk c + 1
-    if (c + 3 < nch) fetch(c + 3, ai, aj);
-    __syncthreads();
-    if (c + 1 >= nch) break;
-    compute(1);                                  // chunk c + 1
-    if (c + 2 < nch) stage(0, bi, bj);           // chunk c + 2
-    if (c + 4 < nch) fetch(c + 4, bi, bj);
+  for (int c = 0; c < nch; c++) {
+    const int buf = c & 1;
+    compute(buf);                 // chunk c
+    if (c + 1 < nch) {
+      stage(buf ^ 1);             // chunk c + 1 (its buffer was last read before the previous barrier)
+      if (c + 2 < nch) fetch(c + 2);
+    }
     __syncthreads();
   }
   T* Ls = L + sd.Loff[s];
@@ -686,43 +686,71 @@ __device__ __forceinline__ void fwdb_gather_item(const int4 tk, SymDev sd,
   Wg[sd.Woff[s] + i] = w;
 }
 
-// Forward diagonal step: x_r = sum_c X_t[r][c] w_c for the 64 rows [c0 + 64 tk.z, ...) of the step; the 16 waves
-// split the columns c < 64 (tk.z + 1) (X_t is lower triangular) and combine their partial sums in a fixed order.
-// Loads are issued BT at a time. task: (s, c0, row block, 0)
+// Forward diagonal step: x_r = sum_{c <= r} X_t[r][c] w_c. One wave per pair of rows (p, wd - 1 - p), so every wave
+// reads wd + 1 entries, from the row-major copy XT_t (XT[r * wd + c] = X_t[r][c]) with the lanes over c and DB loads
+// of X and of w (L1/L2-resident) in flight per lane; the wave sums in a fixed order. A step runs as ~wd / 2 waves
+// spread over the whole chip. task: (s, c0, first pair, 0), DIAG_WAVES pairs per workgroup.
 __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
-                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const double* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  __shared__ double ws[SB + BT], part[DIAG_WAVES][NB];
-  const int s = tk.x, c0 = tk.y, rb = tk.z;
-  const int f = sd.sn_first[s];
+  const int s = tk.x, c0 = tk.y;
+  const int lane = threadIdx.x & 63;
+  const int p = tk.z + (threadIdx.x >> 6);
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
-  const double* Xb = X + x_step_off(sd, s, c0);
+  if (p >= (wd + 1) / 2) return;
+  const int f = sd.sn_first[s];
+  const double* Xb = XT + x_step_off(sd, s, c0);
   const double* w = Wg + sd.Woff[s] + c0;
-  for (int q = threadIdx.x; q < SB + BT; q += DIAG_WAVES * 64) ws[q] = q < wd ? w[q] : 0.0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r = NB * rb + lane;
-  const int rr = min(r, wd - 1);  // rows past the block read its last row and are not stored
-  const int cend = min(wd, NB * rb + NB);
-  const int per = (((cend + DIAG_WAVES - 1) / DIAG_WAVES + BT - 1) / BT) * BT;
-  const int lo = wv * per, hi = min(cend, lo + per);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int base = lo; base < hi; base += BT) {
-    double v[BT];
+  for (int h = 0; h < 2; h++) {
+    const int r = h == 0 ? p : wd - 1 - p;
+    if (h == 1 && r == p) break;
+    const double* xr = Xb + (long long)r * wd;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int base = 0; base <= r; base += 64 * DB) {
+      double xv[DB], wv[DB];
 #pragma unroll
-    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * wd + rr];
+      for (int u = 0; u < DB; u++) {
+        const int c = min(base + lane + 64 * u, r);  // entries past the row repeat its last one, weighted by zero
+        xv[u] = xr[c];
+        wv[u] = w[c];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < DB; u++) acc[u & 3] += base + lane + 64 * u <= r ? xv[u] * wv[u] : 0.0;
+    }
+    const double t = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+    if (lane == 0) y[f + c0 + r] = t;
+  }
+}
+
+// acc[u % 4] += sum over columns q in [q0, q1) of pr[q * ld] * yo[q]: a lane's row of a column-major panel against
+// a vector every lane of the wave shares. Full batches issue BT column loads at 32-bit offsets from a wave-uniform
+// base and take yo by wave-uniform (scalar) loads; the last, partial batch clamps its columns in bounds and weights
+// the extra ones by zero. q0, q1 are wave-uniform; ld < 2^27 so BT * ld fits in 32 bits.
+template <typename T>
+__device__ __forceinline__ void sweep_rows_dot(const T* __restrict__ pr, int ld, const double* __restrict__ yo, int q0,
+                                               int q1, double acc[4]) {
+  int base = q0;
+  for (; base + BT <= q1; base += BT) {
+    const T* pc = pr + (long long)base * ld;
+    T v[BT];
+#pragma unroll
+    for (int u = 0; u < BT; u++) v[u] = pc[(unsigned)(u * ld)];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < BT; u++) acc[u & 3] += base + u < hi ? v[u] * ws[base + u] : 0.0;
+    for (int u = 0; u < BT; u++) acc[u & 3] = fma((double)v[u], yo[base + u], acc[u & 3]);
   }
-  part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-  __syncthreads();
-  if (threadIdx.x < NB && r < wd) {
-    double x = 0.0;
+  if (base < q1) {
+    T v[BT];
 #pragma unroll
-    for (int q = 0; q < DIAG_WAVES; q++) x += part[q][lane];
-    y[f + c0 + r] = x;
+    for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * ld];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < BT; u++) {
+      const double yu = base + u < q1 ? yo[min(base + u, q1 - 1)] : 0.0;
+      acc[u & 3] = fma((double)v[u], yu, acc[u & 3]);
+    }
   }
 }
 
@@ -748,16 +776,7 @@ __device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
     // every lane runs the loop (the lane reads need all lanes' operands); rows past the front's own rows repeat its
     // last own row and are not stored
     const T* pr = L + sd.Loff[s] + min(row, (long long)k - 1);
-    const double* yo = y + f;
-    for (int base = q0; base < q1; base += BT) {
-      const double yv = yo[min(base + lane, q1 - 1)];  // lane u < BT: x_t entry base + u
-      T v[BT];
-#pragma unroll
-      for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * ld];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int u = 0; u < BT; u++) acc[u & 3] += base + u < q1 ? (double)v[u] * lane_read(yv, u) : 0.0;
-    }
+    sweep_rows_dot<T>(pr, (int)ld, y + f, q0, q1, acc);
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
@@ -791,17 +810,7 @@ __device__ __forceinline__ void fwdb_rect_item(const int4 tk, SymDev sd, const T
     // every lane runs the loop (the lane reads below need all lanes' operands); rows past the front repeat its last
     // row and are not stored
     const T* pr = L + sd.Loff[s] + min(row, m - 1);
-    const double* yo = y + f;
-    for (int base = q0; base < q1; base += BT) {
-      // lane u < BT holds the batch's y_own[base + u]; each product takes it by a lane read
-      const double yv = yo[min(base + lane, q1 - 1)];
-      T v[BT];
-#pragma unroll
-      for (int u = 0; u < BT; u++) v[u] = pr[(long long)min(base + u, q1 - 1) * m];
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int u = 0; u < BT; u++) acc[u & 3] += base + u < q1 ? (double)v[u] * lane_read(yv, u) : 0.0;
-    }
+    sweep_rows_dot<T>(pr, (int)m, y + f, q0, q1, acc);
   }
   part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
@@ -876,43 +885,41 @@ __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T
   }
 }
 
-// Backward diagonal step: x_c = sum_r X_t[r][c] z_r (X_t^T z_t) for the 64 columns [c0 + 64 tk.z, ...), lane per
-// column over the transposed copy XT[r * wd + c] = X_t[r][c]; the 16 waves split the rows r >= 64 tk.z.
-// task: (s, c0, column block, 0)
+// Backward diagonal step: x_c = sum_{r >= c} X_t[r][c] z_r (X_t^T z_t). One wave per pair of columns
+// (p, wd - 1 - p), every wave reading wd + 1 entries of the column-major X_t (X[c * wd + r] = X_t[r][c]) with the
+// lanes over r, DB loads of X and of z in flight per lane, summed in a fixed order. task: (s, c0, first pair, 0),
+// DIAG_WAVES pairs per workgroup.
 __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
-                                                   const double* __restrict__ XT, double* __restrict__ y,
+                                                   const double* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  __shared__ double zs[SB + BT], part[DIAG_WAVES][NB];
-  const int s = tk.x, c0 = tk.y, cb = tk.z;
-  const int f = sd.sn_first[s];
+  const int s = tk.x, c0 = tk.y;
+  const int lane = threadIdx.x & 63;
+  const int p = tk.z + (threadIdx.x >> 6);
   const int k = sn_k(sd, s);
   const int wd = min(SB, k - c0);
-  const double* Xb = XT + x_step_off(sd, s, c0);
+  if (p >= (wd + 1) / 2) return;
+  const int f = sd.sn_first[s];
+  const double* Xb = X + x_step_off(sd, s, c0);
   const double* z = Wg + sd.Woff[s] + c0;
-  for (int q = threadIdx.x; q < SB + BT; q += DIAG_WAVES * 64) zs[q] = q < wd ? z[q] : 0.0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int c = NB * cb + lane;
-  const int cc = min(c, wd - 1);  // columns past the block read its last column and are not stored
-  const int span = wd - NB * cb;
-  const int per = (((span + DIAG_WAVES - 1) / DIAG_WAVES + BT - 1) / BT) * BT;
-  const int lo = NB * cb + wv * per, hi = min(wd, lo + per);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int base = lo; base < hi; base += BT) {
-    double v[BT];
+  for (int h = 0; h < 2; h++) {
+    const int c = h == 0 ? p : wd - 1 - p;
+    if (h == 1 && c == p) break;
+    const double* xc = Xb + (long long)c * wd;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int base = c; base < wd; base += 64 * DB) {
+      double xv[DB], zv[DB];
 #pragma unroll
-    for (int u = 0; u < BT; u++) v[u] = Xb[(long long)min(base + u, hi - 1) * wd + cc];
-    __builtin_amdgcn_sched_barrier(0);
+      for (int u = 0; u < DB; u++) {
+        const int r = min(base + lane + 64 * u, wd - 1);  // entries past the column repeat its last one, weighted by zero
+        xv[u] = xc[r];
+        zv[u] = z[r];
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < BT; u++) acc[u & 3] += base + u < hi ? v[u] * zs[base + u] : 0.0;
-  }
-  part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-  __syncthreads();
-  if (threadIdx.x < NB && c < wd) {
-    double x = 0.0;
-#pragma unroll
-    for (int q = 0; q < DIAG_WAVES; q++) x += part[q][lane];
-    y[f + c0 + c] = x;
+      for (int u = 0; u < DB; u++) acc[u & 3] += base + lane + 64 * u < wd ? xv[u] * zv[u] : 0.0;
+    }
+    const double t = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+    if (lane == 0) y[f + c0 + c] = t;
   }
 }
 
@@ -971,9 +978,9 @@ __global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ ta
 }
 
 __global__ __launch_bounds__(DIAG_WAVES * 64) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
-                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const double* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  fwdb_diag_item(tasks[blockIdx.x], sd, X, y, Wg);
+  fwdb_diag_item(tasks[blockIdx.x], sd, XT, y, Wg);
 }
 
 template <typename T>
@@ -1004,9 +1011,9 @@ __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ task
 }
 
 __global__ __launch_bounds__(DIAG_WAVES * 64) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
-                                                   const double* __restrict__ XT, double* __restrict__ y,
+                                                   const double* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  bwdb_diag_item(tasks[blockIdx.x], sd, XT, y, Wg);
+  bwdb_diag_item(tasks[blockIdx.x], sd, X, y, Wg);
 }
 
 template <typename T>
@@ -1060,6 +1067,10 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   if (rc) throw Error(rc == -2 ? -2 : (rc == -1 ? -1 : -3), "symbolic analysis failed: " + S_.error);
   const Symbolic& S = S_;
   const int ns = S.ns;
+  // the outer-update kernel addresses a front's chunk of 16 columns with 32-bit offsets
+  for (int s = 0; s < ns; s++)
+    if (S.sn_rowptr[s + 1] - S.sn_rowptr[s] >= (1LL << 27))
+      throw Error(-3, "a supernode front has 2^27 rows or more");
   // children lists (ascending)
   std::vector<int> cptr(ns + 1, 0), clist;
   for (int s = 0; s < ns; s++) if (S.sn_parent[s] >= 0) cptr[S.sn_parent[s] + 1]++;
@@ -1250,7 +1261,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
         int k = sn_kk(s);
         if (k <= c0) continue;
         int wd = std::min(SB, k - c0);
-        for (int rb = 0; rb * NB < wd; rb++) fwt.push_back(make_int4(s, c0, rb, 0));
+        for (int pb = 0; pb < (wd + 1) / 2; pb += DIAG_WAVES) fwt.push_back(make_int4(s, c0, pb, 0));
         for (int j = 0; j * NB < wd; j++) invt.push_back(make_int4(s, c0, j, 0));
       }
       d.end = pn.begin = (long long)fwt.size();
@@ -1284,7 +1295,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
         int k = sn_kk(s);
         if (k <= c0) continue;
         int wd = std::min(SB, k - c0);
-        for (int cb = 0; cb * NB < wd; cb++) bwt.push_back(make_int4(s, c0, cb, 0));
+        for (int pb = 0; pb < (wd + 1) / 2; pb += DIAG_WAVES) bwt.push_back(make_int4(s, c0, pb, 0));
       }
       d.end = pn.begin = (long long)bwt.size();
       for (int s : big) {
@@ -1463,10 +1474,10 @@ void SpdDevice::sweeps(const T* L) {
       hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(256), 0, stream_, tasks + ll.begin, sd_,
                          args...);
   };
-  auto launch_diag = [&](auto kern, const LaunchList& ll, const int4* tasks, auto... args) {
+  auto launch_n = [&](auto kern, int threads, const LaunchList& ll, const int4* tasks, auto... args) {
     if (ll.end > ll.begin)
-      hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(DIAG_WAVES * 64), 0, stream_,
-                         tasks + ll.begin, sd_, args...);
+      hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(threads), 0, stream_, tasks + ll.begin, sd_,
+                         args...);
   };
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
@@ -1478,8 +1489,9 @@ void SpdDevice::sweeps(const T* L) {
     }
     launch(k_fwdb_gather, lv.fwd_gather, fwd_tasks_.p, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.fwd_diag.size(); t++) {
-      launch_diag(k_fwdb_diag, lv.fwd_diag[t], fwd_tasks_.p, (const double*)X_.p, y_.p, (const double*)W_.p);
-      launch_diag(k_fwdb_panel<T>, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p);
+      launch_n(k_fwdb_diag, DIAG_WAVES * 64, lv.fwd_diag[t], fwd_tasks_.p, (const double*)XT_.p, y_.p,
+               (const double*)W_.p);
+      launch_n(k_fwdb_panel<T>, RECT_WAVES * 64, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
     if (lv.fwd_rect.end > lv.fwd_rect.begin)
       hipLaunchKernelGGL(k_fwdb_rect<T>, dim3((unsigned)(lv.fwd_rect.end - lv.fwd_rect.begin)), dim3(RECT_WAVES * 64),
@@ -1497,7 +1509,8 @@ void SpdDevice::sweeps(const T* L) {
     launch(k_bwdb_xb, lv.bwd_xb, bwd_tasks_.p, (const double*)y_.p, V_.p);
     launch(k_bwdb_gemv<T>, lv.bwd_gemv, bwd_tasks_.p, L, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.bwd_diag.size(); t++) {
-      launch_diag(k_bwdb_diag, lv.bwd_diag[t], bwd_tasks_.p, (const double*)XT_.p, y_.p, (const double*)W_.p);
+      launch_n(k_bwdb_diag, DIAG_WAVES * 64, lv.bwd_diag[t], bwd_tasks_.p, (const double*)X_.p, y_.p,
+               (const double*)W_.p);
       launch(k_bwdb_panel<T>, lv.bwd_panel[t], bwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
   }

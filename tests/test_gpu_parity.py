@@ -111,13 +111,15 @@ def test_spd_mixed_precision_refines(c1_system):
 
 
 @pytest.mark.parametrize("precision", [0, 2])
-def test_spd_wide_fronts(precision):
-    """Two-ring pattern of a 14x14x20 Kuhn cylinder (14,175 DOF) under the geometric ordering: its top fronts span
-    several 512-column sweep steps (dense trailing triangle of 2,160 columns) and carry update-vector rectangles, the
-    blocked multi-launch sweep path. Random diagonally dominant SPD values; the solution must match scipy's sparse LU
-    (FP64 factor), or reach the same backward error after refinement with the FP32 copy of the factor."""
+@pytest.mark.parametrize("nxy,nz,min_tail", [(14, 20, 2048), (20, 28, 4096)])
+def test_spd_wide_fronts(precision, nxy, nz, min_tail):
+    """Two-ring pattern of a Kuhn cylinder under the geometric ordering: 14x14x20 cells (14,175 DOF, dense trailing
+    triangle of 2,160 columns: two 2048-column sweep steps) and 20x20x28 cells (38,367 DOF, 4,284 columns: three
+    steps, the last one 188 wide), with update-vector rectangles on the fronts below; the blocked multi-launch sweep
+    path. Random diagonally dominant SPD values; the solution must reach a backward error of 1e-13 with the FP64
+    factor, or after refinement with the FP32 copy of the factor, and match scipy's sparse LU on the smaller case."""
     import scipy.sparse.linalg as spla
-    x, t, ox, oy = kuhn_cylinder(40, 60, 14, 20, seed=1)
+    x, t, ox, oy = kuhn_cylinder(40, 60, nxy, nz, seed=1)
     m = TetMesh(x.shape[0], t, ox, oy)
     _, _, cp, ri = build_pattern(m)
     n = 3 * m.nn
@@ -130,7 +132,7 @@ def test_spd_wide_fronts(precision):
     A.data[A.indices == col] = d[col[A.indices == col]]
     assert np.array_equal(A.indptr, cp) and np.array_equal(A.indices, ri)
     b = rng.standard_normal(n)
-    xr = spla.splu(A.tocsc()).solve(b)
+    xr = spla.splu(A.tocsc()).solve(b) if n < 20000 else None
     with spd.HipSpdSolver(precision=precision, refine_tol=1e-14) as h:
         h.set_coordinates(np.repeat(x, 3, axis=0))
         h.Analyze(n, cp, ri)
@@ -138,12 +140,13 @@ def test_spd_wide_fronts(precision):
         tail = 0
         while tail < n and cc[n - 1 - tail] == tail + 1:
             tail += 1
-        assert tail > 1024
+        assert tail > min_tail
         h.Factorize(A.data)
         for _ in range(2):  # the second solve replays the captured sweep graph
             xs = h.Solve(b)
             assert np.linalg.norm(A @ xs - b) <= 1e-13 * np.linalg.norm(b)
-            assert np.linalg.norm(xs - xr) <= 1e-12 * np.linalg.norm(xr)
+            if xr is not None:
+                assert np.linalg.norm(xs - xr) <= 1e-12 * np.linalg.norm(xr)
         if precision == 2:
             assert h.stats()["refine_iterations"] >= 1
 
