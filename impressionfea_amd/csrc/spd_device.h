@@ -125,6 +125,7 @@ class SpdDevice {
   size_t sweep_tsize_ = 0;
   void drop_graph();
   DBuf<double> V_, W_, y_, Linv_, X_, XT_;
+  DBuf<float> X32_, XT32_;  // FP32 copies of X_ and XT_ for the FP32 sweeps (precision 1 and 2)
   DBuf<int> err_;
   SymDev sd_{};
   bool profiling_ = false;

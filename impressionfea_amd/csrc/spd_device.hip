@@ -31,6 +31,7 @@ constexpr int BT = 32;          // loads a lane issues before using them in the 
 constexpr int DIAG_WAVES = 4;   // waves of a diagonal-step workgroup (one pair of rows or columns each)
 constexpr int DB = 8;           // loads of X (and of the vector) a lane issues before using them in a diagonal step
 constexpr int RECT_WAVES = 16;  // waves of a forward update-vector or panel workgroup (they split the columns)
+constexpr int BWD_NC = 8;       // columns a wave reduces at once in the narrow-front backward sweep
 constexpr int BP_ROWS = 512;    // rows of a backward panel step loaded per batch (times 4 columns per wave)
 
 // ------------------------------------------------------------------------------------------------ MFMA wrappers
@@ -204,31 +205,31 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
     if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)row[q];
   if (bad && i == 0) atomicOr(err, 1);
   // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; four partial sums keep a row's
-  // dependent chain a quarter as long, and the reciprocal of the diagonal replaces the division
+  // dependent chain a quarter as long, and the reciprocal of the diagonal replaces the division. x[r] (zero for
+  // r < c) is kept in lane c's own row of D at columns r >= c: D's strict lower triangle still holds L and its
+  // diagonal is no longer read (rdiag has it), so no lane writes what another lane reads and the loop needs no
+  // synchronisation. The loop over r stays rolled: a fully unrolled 64x64 inverse is ~50 KB of straight-line code
+  // that the instruction cache fetches once per launch, on the factorization's critical path.
   const int c = i;
-  double x[NB];
-#pragma unroll
+  double* xc = &D[c][0];
   for (int r = 0; r < NB; r++) {
     double a0 = (r == c) ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-    for (int p = 0; p + 3 < r; p += 4) {
-      a0 -= D[r][p] * x[p];
-      a1 -= D[r][p + 1] * x[p + 1];
-      a2 -= D[r][p + 2] * x[p + 2];
-      a3 -= D[r][p + 3] * x[p + 3];
+    const double* Dr = &D[r][0];
+    const int r4 = r & ~3;
+    for (int p = 0; p < r4; p += 4) {
+      a0 -= Dr[p] * (p >= c ? xc[p] : 0.0);
+      a1 -= Dr[p + 1] * (p + 1 >= c ? xc[p + 1] : 0.0);
+      a2 -= Dr[p + 2] * (p + 2 >= c ? xc[p + 2] : 0.0);
+      a3 -= Dr[p + 3] * (p + 3 >= c ? xc[p + 3] : 0.0);
     }
-#pragma unroll
-    for (int p = r & ~3; p < r; p++) a0 -= D[r][p] * x[p];
-    x[r] = ((a0 + a1) + (a2 + a3)) * rdiag[r];
+    for (int p = r4; p < r; p++) a0 -= Dr[p] * (p >= c ? xc[p] : 0.0);
+    if (r >= c) xc[r] = ((a0 + a1) + (a2 + a3)) * rdiag[r];
   }
   wave_sync_lds();
-  // transpose through LDS so that the stores of Lt[c][r] = Linv[r][c] are contiguous over the lanes
-#pragma unroll
-  for (int r = 0; r < NB; r++) D[c][r] = x[r];
-  wave_sync_lds();
+  // Lt[t][i] = Linv[i][t] = x_t[i]: lane i reads column i of the upper triangle, so the stores are contiguous
   double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
 #pragma unroll 16
-  for (int t = 0; t < NB; t++) Lt[t * NB + i] = D[t][i];
+  for (int t = 0; t < NB; t++) Lt[t * NB + i] = i >= t ? D[t][i] : 0.0;
 }
 
 // Rows below the diagonal block: X = A L11^{-T} as a 64x64x64 FP64 MFMA product per task, written back into the
@@ -517,52 +518,80 @@ __global__ __launch_bounds__(256) void k_fwd(const int* __restrict__ sns, SymDev
   for (long long i = k + tid; i < m; i += bd) vs[i - k] = w[i];
 }
 
-// Backward for narrow fronts: z = y_own - L21^T x_below (wave per column), then descending 64-column steps
-// x_t = Linv_t^T z_t and z_j -= L(c0.., j)^T x_t for the columns j < c0 (wave per column, lanes over the step).
+// Backward for narrow fronts. w[0, k) = z, w[k, m) = the solution at the front's lower rows (gathered once); then
+// z = y_own - L21^T x_below and descending 64-column steps x_t = Linv_t^T z_t, z_j -= L(c0.., j)^T x_t for the
+// columns j < c0. Every product is a column dot over lanes; a wave keeps BWD_NC columns (16 for the 64x64 inverse)
+// in flight at once and reduces them together, so the dependent chain is a few batches, not one column at a time.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd(const int* __restrict__ sns, SymDev sd, const T* __restrict__ L,
                                              const double* __restrict__ Linv, double* __restrict__ y,
                                              double* __restrict__ Wg, int cap) {
   extern __shared__ double shz[];
   __shared__ double xs[NB];
+  constexpr int NC = BWD_NC;
   const int s = sns[blockIdx.x];
   const int f = sd.sn_first[s];
   const int k = sn_k(sd, s);
-  const long long m = sn_m(sd, s);
+  const int m = (int)sn_m(sd, s);  // narrow: k * m <= BIG_PANEL
   const int* rows = sd.sn_rows + sd.sn_rowptr[s];
-  double* z = (k <= cap) ? shz : Wg + sd.Woff[s];
+  double* w = (m <= cap) ? shz : Wg + sd.Woff[s];
   const T* P = L + sd.Loff[s];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int q = wv; q < k; q += 4) {
-    const T* cq = P + (long long)q * m;
-    double a0 = 0.0, a1 = 0.0;
-    long long i = k + lane;
-    for (; i + 64 < m; i += 128) {
-      a0 += (double)cq[i] * y[rows[i]];
-      a1 += (double)cq[i + 64] * y[rows[i + 64]];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = k + tid; i < m; i += 256) w[i] = y[rows[i]];
+  __syncthreads();
+  for (int qb = NC * wv; qb < k; qb += 4 * NC) {
+    const T* cq[NC];
+    double acc[NC];
+#pragma unroll
+    for (int u = 0; u < NC; u++) {
+      cq[u] = P + (long long)min(qb + u, k - 1) * m;  // columns past the front repeat its last one, not stored
+      acc[u] = 0.0;
     }
-    if (i < m) a0 += (double)cq[i] * y[rows[i]];
-    const double acc = wave_sum(a0 + a1);
-    if (lane == 0) z[q] = y[f + q] - acc;
+    for (int i = k + lane; i < m; i += 64) {
+      const double xv = w[i];
+#pragma unroll
+      for (int u = 0; u < NC; u++) acc[u] += (double)cq[u][i] * xv;
+    }
+#pragma unroll
+    for (int u = 0; u < NC; u++) {
+      const double a = wave_sum(acc[u]);
+      if (lane == 0 && qb + u < k) w[qb + u] = y[f + qb + u] - a;
+    }
   }
   __syncthreads();
   for (int c0 = ((k - 1) / NB) * NB; c0 >= 0; c0 -= NB) {
     const int wd = min(NB, k - c0);
     const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
-    const double zl = lane < wd ? z[c0 + lane] : 0.0;
-    for (int c = wv; c < wd; c += 4) {
-      // x_c = sum_r Linv[r][c] z_r = sum_r Lt[c][r] z_r
-      double a = lane < wd ? Lt[c * NB + lane] * zl : 0.0;
-      a = wave_sum(a);
-      if (lane == 0) xs[c] = a;
+    const double zl = lane < wd ? w[c0 + lane] : 0.0;
+    {
+      // x_c = sum_r Linv[r][c] z_r = sum_r Lt[c][r] z_r for the wave's 16 columns c
+      double a[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int c = 16 * wv + u;
+        a[u] = (c < wd && lane < wd) ? Lt[c * NB + lane] * zl : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const double t = wave_sum(a[u]);
+        if (lane == 0) xs[16 * wv + u] = t;
+      }
     }
     __syncthreads();
-    if (threadIdx.x < wd) y[f + c0 + threadIdx.x] = xs[threadIdx.x];
+    if (tid < wd) y[f + c0 + tid] = xs[tid];
     const double xl = lane < wd ? xs[lane] : 0.0;
-    for (int j = wv; j < c0; j += 4) {
-      double a = lane < wd ? (double)P[(long long)j * m + c0 + lane] * xl : 0.0;
-      a = wave_sum(a);
-      if (lane == 0) z[j] -= a;
+    for (int jb = NC * wv; jb < c0; jb += 4 * NC) {
+      double a[NC];
+#pragma unroll
+      for (int u = 0; u < NC; u++) {
+        const int j = min(jb + u, c0 - 1);
+        a[u] = lane < wd ? (double)P[(long long)j * m + c0 + lane] * xl : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < NC; u++) {
+        const double t = wave_sum(a[u]);
+        if (lane == 0 && jb + u < c0) w[jb + u] -= t;
+      }
     }
     __syncthreads();
   }
@@ -690,8 +719,9 @@ __device__ __forceinline__ void fwdb_gather_item(const int4 tk, SymDev sd,
 // reads wd + 1 entries, from the row-major copy XT_t (XT[r * wd + c] = X_t[r][c]) with the lanes over c and DB loads
 // of X and of w (L1/L2-resident) in flight per lane; the wave sums in a fixed order. A step runs as ~wd / 2 waves
 // spread over the whole chip. task: (s, c0, first pair, 0), DIAG_WAVES pairs per workgroup.
+template <typename TX>
 __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
-                                                   const double* __restrict__ XT, double* __restrict__ y,
+                                                   const TX* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
   const int s = tk.x, c0 = tk.y;
   const int lane = threadIdx.x & 63;
@@ -700,15 +730,16 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
   const int wd = min(SB, k - c0);
   if (p >= (wd + 1) / 2) return;
   const int f = sd.sn_first[s];
-  const double* Xb = XT + x_step_off(sd, s, c0);
+  const TX* Xb = XT + x_step_off(sd, s, c0);
   const double* w = Wg + sd.Woff[s] + c0;
   for (int h = 0; h < 2; h++) {
     const int r = h == 0 ? p : wd - 1 - p;
     if (h == 1 && r == p) break;
-    const double* xr = Xb + (long long)r * wd;
+    const TX* xr = Xb + (long long)r * wd;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int base = 0; base <= r; base += 64 * DB) {
-      double xv[DB], wv[DB];
+      TX xv[DB];
+      double wv[DB];
 #pragma unroll
       for (int u = 0; u < DB; u++) {
         const int c = min(base + lane + 64 * u, r);  // entries past the row repeat its last one, weighted by zero
@@ -717,7 +748,7 @@ __device__ __forceinline__ void fwdb_diag_item(const int4 tk, SymDev sd,
       }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int u = 0; u < DB; u++) acc[u & 3] += base + lane + 64 * u <= r ? xv[u] * wv[u] : 0.0;
+      for (int u = 0; u < DB; u++) acc[u & 3] += base + lane + 64 * u <= r ? (double)xv[u] * wv[u] : 0.0;
     }
     const double t = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
     if (lane == 0) y[f + c0 + r] = t;
@@ -889,8 +920,9 @@ __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T
 // (p, wd - 1 - p), every wave reading wd + 1 entries of the column-major X_t (X[c * wd + r] = X_t[r][c]) with the
 // lanes over r, DB loads of X and of z in flight per lane, summed in a fixed order. task: (s, c0, first pair, 0),
 // DIAG_WAVES pairs per workgroup.
+template <typename TX>
 __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
-                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const TX* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
   const int s = tk.x, c0 = tk.y;
   const int lane = threadIdx.x & 63;
@@ -899,15 +931,16 @@ __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
   const int wd = min(SB, k - c0);
   if (p >= (wd + 1) / 2) return;
   const int f = sd.sn_first[s];
-  const double* Xb = X + x_step_off(sd, s, c0);
+  const TX* Xb = X + x_step_off(sd, s, c0);
   const double* z = Wg + sd.Woff[s] + c0;
   for (int h = 0; h < 2; h++) {
     const int c = h == 0 ? p : wd - 1 - p;
     if (h == 1 && c == p) break;
-    const double* xc = Xb + (long long)c * wd;
+    const TX* xc = Xb + (long long)c * wd;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int base = c; base < wd; base += 64 * DB) {
-      double xv[DB], zv[DB];
+      TX xv[DB];
+      double zv[DB];
 #pragma unroll
       for (int u = 0; u < DB; u++) {
         const int r = min(base + lane + 64 * u, wd - 1);  // entries past the column repeat its last one, weighted by zero
@@ -916,7 +949,7 @@ __device__ __forceinline__ void bwdb_diag_item(const int4 tk, SymDev sd,
       }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int u = 0; u < DB; u++) acc[u & 3] += base + lane + 64 * u < wd ? xv[u] * zv[u] : 0.0;
+      for (int u = 0; u < DB; u++) acc[u & 3] += base + lane + 64 * u < wd ? (double)xv[u] * zv[u] : 0.0;
     }
     const double t = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
     if (lane == 0) y[f + c0 + c] = t;
@@ -977,10 +1010,11 @@ __global__ __launch_bounds__(256) void k_fwdb_gather(const int4* __restrict__ ta
   fwdb_gather_item(tasks[blockIdx.x], sd, y, V, Wg);
 }
 
+template <typename TX>
 __global__ __launch_bounds__(DIAG_WAVES * 64) void k_fwdb_diag(const int4* __restrict__ tasks, SymDev sd,
-                                                   const double* __restrict__ XT, double* __restrict__ y,
+                                                   const TX* __restrict__ XT, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  fwdb_diag_item(tasks[blockIdx.x], sd, XT, y, Wg);
+  fwdb_diag_item<TX>(tasks[blockIdx.x], sd, XT, y, Wg);
 }
 
 template <typename T>
@@ -1010,10 +1044,11 @@ __global__ __launch_bounds__(256) void k_bwdb_gemv(const int4* __restrict__ task
   bwdb_gemv_item<T>(tasks[blockIdx.x], sd, L, y, V, Wg);
 }
 
+template <typename TX>
 __global__ __launch_bounds__(DIAG_WAVES * 64) void k_bwdb_diag(const int4* __restrict__ tasks, SymDev sd,
-                                                   const double* __restrict__ X, double* __restrict__ y,
+                                                   const TX* __restrict__ X, double* __restrict__ y,
                                                    const double* __restrict__ Wg) {
-  bwdb_diag_item(tasks[blockIdx.x], sd, X, y, Wg);
+  bwdb_diag_item<TX>(tasks[blockIdx.x], sd, X, y, Wg);
 }
 
 template <typename T>
@@ -1056,7 +1091,8 @@ hipEvent_t SpdDevice::event(size_t i) {
 size_t SpdDevice::device_bytes() const {
   return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
          sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() + zero_tasks_.bytes() +
-         Linv_.bytes() + X_.bytes() + XT_.bytes() + gptr_.bytes() + gsrc_.bytes() + fwd_tasks_.bytes() + bwd_tasks_.bytes();
+         Linv_.bytes() + X_.bytes() + XT_.bytes() + X32_.bytes() + XT32_.bytes() + gptr_.bytes() + gsrc_.bytes() +
+         fwd_tasks_.bytes() + bwd_tasks_.bytes();
 }
 
 void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* perm, const SymbolicOptions& opt) {
@@ -1454,6 +1490,16 @@ int SpdDevice::factorize_device(const double* d_vals, int precision) {
     hipLaunchKernelGGL(k_to_f32, dim3(grid_stride(n, 256)), dim3(256), 0, stream_, (const double*)L_.p, L32_.p, n);
     IFEA_HIP(hipGetLastError());
   }
+  if (rc == 0 && precision != 0) {
+    // the FP32 sweeps read FP32 copies of the step-block inverses (computed in FP64 by k_inv_diag)
+    long long nx = (long long)X_.n;
+    X32_.alloc(nx);
+    XT32_.alloc(nx);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_stride(nx, 256)), dim3(256), 0, stream_, (const double*)X_.p, X32_.p, nx);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_stride(nx, 256)), dim3(256), 0, stream_, (const double*)XT_.p, XT32_.p,
+                       nx);
+    IFEA_HIP(hipGetLastError());
+  }
   IFEA_HIP(hipEventRecord(e1, stream_));
   IFEA_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -1469,6 +1515,9 @@ int SpdDevice::factorize_device(const double* d_vals, int precision) {
 template <typename T>
 void SpdDevice::sweeps(const T* L) {
   const int cap = 7680;  // doubles of LDS per forward/backward work vector (plus 2.5 KB of static LDS)
+  // explicit inverses of the wide fronts' step blocks in the sweep's precision (FP32 copies made per factorization)
+  const T* Xs = sizeof(T) == 4 ? (const T*)X32_.p : (const T*)X_.p;
+  const T* XTs = sizeof(T) == 4 ? (const T*)XT32_.p : (const T*)XT_.p;
   auto launch = [&](auto kern, const LaunchList& ll, const int4* tasks, auto... args) {
     if (ll.end > ll.begin)
       hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(256), 0, stream_, tasks + ll.begin, sd_,
@@ -1489,8 +1538,7 @@ void SpdDevice::sweeps(const T* L) {
     }
     launch(k_fwdb_gather, lv.fwd_gather, fwd_tasks_.p, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.fwd_diag.size(); t++) {
-      launch_n(k_fwdb_diag, DIAG_WAVES * 64, lv.fwd_diag[t], fwd_tasks_.p, (const double*)XT_.p, y_.p,
-               (const double*)W_.p);
+      launch_n(k_fwdb_diag<T>, DIAG_WAVES * 64, lv.fwd_diag[t], fwd_tasks_.p, XTs, y_.p, (const double*)W_.p);
       launch_n(k_fwdb_panel<T>, RECT_WAVES * 64, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
     if (lv.fwd_rect.end > lv.fwd_rect.begin)
@@ -1509,8 +1557,7 @@ void SpdDevice::sweeps(const T* L) {
     launch(k_bwdb_xb, lv.bwd_xb, bwd_tasks_.p, (const double*)y_.p, V_.p);
     launch(k_bwdb_gemv<T>, lv.bwd_gemv, bwd_tasks_.p, L, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.bwd_diag.size(); t++) {
-      launch_n(k_bwdb_diag, DIAG_WAVES * 64, lv.bwd_diag[t], bwd_tasks_.p, (const double*)X_.p, y_.p,
-               (const double*)W_.p);
+      launch_n(k_bwdb_diag<T>, DIAG_WAVES * 64, lv.bwd_diag[t], bwd_tasks_.p, Xs, y_.p, (const double*)W_.p);
       launch(k_bwdb_panel<T>, lv.bwd_panel[t], bwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
   }
