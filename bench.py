@@ -38,9 +38,10 @@ def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
     from impressionfea_amd.forging3d import Forging3DSolver
     s = Forging3DSolver.from_workload(wl, precision=precision, warm_start=warm_start)
     s.set_params(profile=profile)
-    loads, ms_solve, ms_update, stats = [], [], [], []
+    loads, ms_solve, ms_update, stats, all_loads = [], [], [], [], []
     for k in range(warmup):
         r = s.step()
+        all_loads.append(r.load)
         log(f"  warmup {k}: load {r.load:.6e} N it {r.iterations} fact {r.stats['factorizations']} "
             f"{r.ms_solve:.0f} ms")
     s.info(reset=True)
@@ -48,6 +49,7 @@ def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
     for k in range(steps):
         r = s.step()
         loads.append(r.load)
+        all_loads.append(r.load)
         ms_solve.append(r.ms_solve)
         ms_update.append(r.ms_update)
         stats.append(r.stats)
@@ -59,7 +61,7 @@ def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
     used = device_memory_used()
     s.close()
     return dict(loads=loads, ms_solve=ms_solve, ms_update=ms_update, stats=stats, wall=wall, info=info,
-                device_used=used)
+                device_used=used, all_loads=all_loads)
 
 
 def device_memory_used():
@@ -145,6 +147,38 @@ def cpu_baseline(wl, gpu_stats, threads, nnz_gpu=0):
                         f"counts (assemblies {asm_m:.1f}+{asm_r:.1f}, factorisations {fac:.2f}, solves {sol:.1f})"))
 
 
+def oracle_parity(wl, res, n, threads):
+    """The FP64 oracle (C++ restatement of the reference solver, MKL PARDISO with the reference's settings) on the
+    first n increments of the same workload, from the same initial state; the ram load of every GPU arm's matching
+    increment (warmup included) is compared with it. Test infrastructure used as the checker: bounded by n."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    if not O.pardiso_available():
+        log("oracle parity: MKL PARDISO not available, skipped")
+        return None
+    O.set_threads(threads)
+    x, t, ox, oy = wl.mesh()
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0, linear=1)
+    o.initial(float(x[:, 2].min()))
+    loads, its, secs = [], [], []
+    for k in range(n):
+        t0 = time.perf_counter()
+        r = o.step()
+        secs.append(time.perf_counter() - t0)
+        loads.append(r["load"])
+        its.append(r["iterations"])
+        log(f"  oracle increment {k}: load {r['load']:.9e} N it {r['iterations']} fact {r['factorizations']} "
+            f"{secs[-1]:.1f} s")
+    out = dict(increments=n, oracle="FP64 C++ restatement + MKL PARDISO (iparm as the reference)", threads=threads,
+               oracle_loads=loads, oracle_iterations=its, oracle_seconds=[round(v, 1) for v in secs])
+    for arm, v in res.items():
+        g = v["all_loads"][:n]
+        out[f"{arm}_loads"] = g
+        out[f"{arm}_rel_err"] = [abs(a - b) / abs(b) for a, b in zip(g, loads)]
+    return out
+
+
 def copy_probe(nbytes: int, reps: int = 3):
     """Device-to-device copies of a known size (the calibration kernel for HBM byte counters)."""
     import ctypes as C
@@ -175,6 +209,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the triangular sweeps one by one instead of replaying them as a HIP graph "
                          "(for kernel-trace profilers)")
+    ap.add_argument("--oracle-increments", type=int, default=0, metavar="N",
+                    help="also run the FP64 CPU oracle (MKL PARDISO) on the first N increments and report every "
+                         "arm's ram-load error against it (minutes per increment at 360k DOF)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of this size (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -291,6 +328,11 @@ def main():
             out["cpu_baseline"] = cpu_baseline(wl, res["fp64"]["stats"], args.cpu_threads, info["nnz_l"])
         except Exception as ex:  # noqa: BLE001 - the baseline is reported, never required
             log(f"cpu baseline failed: {ex}")
+    if args.oracle_increments > 0:
+        try:
+            out["oracle_parity"] = oracle_parity(wl, res, args.oracle_increments, args.cpu_threads)
+        except Exception as ex:  # noqa: BLE001 - reported, never required
+            log(f"oracle parity failed: {ex}")
     print(json.dumps(out))
 
 
