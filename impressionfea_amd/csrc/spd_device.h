@@ -109,7 +109,7 @@ class SpdDevice {
     // triangular sweeps: narrow fronts one workgroup each; wide fronts blocked over several launches
     long long small_begin = 0, small_end = 0, big_begin = 0, big_end = 0;  // into sweep_sn_
     long long small_max_m = 0;
-    LaunchList fwd_gather, fwd_rect, bwd_xb, bwd_gemv;
+    LaunchList fwd_gather, fwd_rect, fwd_rect_sum, bwd_xb, bwd_gemv;
     std::vector<LaunchList> fwd_diag, fwd_panel, bwd_diag, bwd_panel;  // per SB-column step (backward descending)
     double solve_bytes = 0;              // panel bytes per element of storage (times sizeof(T) at run time)
     long long max_m = 0;
@@ -125,7 +125,8 @@ class SpdDevice {
   size_t sweep_tsize_ = 0;
   void drop_graph();
   DBuf<double> V_, W_, y_, Linv_, X_, XT_;
-  DBuf<float> X32_, XT32_;  // FP32 copies of X_ and XT_ for the FP32 sweeps (precision 1 and 2)
+  DBuf<float> X32_, XT32_;
+  DBuf<double> Pr_;  // partial sums of the column ranges of the forward update-vector tasks (one level at a time)  // FP32 copies of X_ and XT_ for the FP32 sweeps (precision 1 and 2)
   DBuf<int> err_;
   SymDev sd_{};
   bool profiling_ = false;

@@ -31,6 +31,7 @@ constexpr int BT = 32;          // loads a lane issues before using them in the 
 constexpr int DIAG_WAVES = 4;   // waves of a diagonal-step workgroup (one pair of rows or columns each)
 constexpr int DB = 8;           // loads of X (and of the vector) a lane issues before using them in a diagonal step
 constexpr int RECT_WAVES = 16;  // waves of a forward update-vector or panel workgroup (they split the columns)
+constexpr int RECT_SPLIT = 2048; // widest column range of one forward update-vector task
 constexpr int BWD_NC = 8;       // columns a wave reduces at once in the narrow-front backward sweep
 constexpr int BP_ROWS = 512;    // rows of a backward panel step loaded per batch (times 4 columns per wave)
 
@@ -205,31 +206,31 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
     if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)row[q];
   if (bad && i == 0) atomicOr(err, 1);
   // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; four partial sums keep a row's
-  // dependent chain a quarter as long, and the reciprocal of the diagonal replaces the division. x[r] (zero for
-  // r < c) is kept in lane c's own row of D at columns r >= c: D's strict lower triangle still holds L and its
-  // diagonal is no longer read (rdiag has it), so no lane writes what another lane reads and the loop needs no
-  // synchronisation. The loop over r stays rolled: a fully unrolled 64x64 inverse is ~50 KB of straight-line code
-  // that the instruction cache fetches once per launch, on the factorization's critical path.
+  // dependent chain a quarter as long, and the reciprocal of the diagonal replaces the division
   const int c = i;
-  double* xc = &D[c][0];
+  double x[NB];
+#pragma unroll
   for (int r = 0; r < NB; r++) {
     double a0 = (r == c) ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    const double* Dr = &D[r][0];
-    const int r4 = r & ~3;
-    for (int p = 0; p < r4; p += 4) {
-      a0 -= Dr[p] * (p >= c ? xc[p] : 0.0);
-      a1 -= Dr[p + 1] * (p + 1 >= c ? xc[p + 1] : 0.0);
-      a2 -= Dr[p + 2] * (p + 2 >= c ? xc[p + 2] : 0.0);
-      a3 -= Dr[p + 3] * (p + 3 >= c ? xc[p + 3] : 0.0);
+#pragma unroll
+    for (int p = 0; p + 3 < r; p += 4) {
+      a0 -= D[r][p] * x[p];
+      a1 -= D[r][p + 1] * x[p + 1];
+      a2 -= D[r][p + 2] * x[p + 2];
+      a3 -= D[r][p + 3] * x[p + 3];
     }
-    for (int p = r4; p < r; p++) a0 -= Dr[p] * (p >= c ? xc[p] : 0.0);
-    if (r >= c) xc[r] = ((a0 + a1) + (a2 + a3)) * rdiag[r];
+#pragma unroll
+    for (int p = r & ~3; p < r; p++) a0 -= D[r][p] * x[p];
+    x[r] = ((a0 + a1) + (a2 + a3)) * rdiag[r];
   }
   wave_sync_lds();
-  // Lt[t][i] = Linv[i][t] = x_t[i]: lane i reads column i of the upper triangle, so the stores are contiguous
+  // transpose through LDS so that the stores of Lt[c][r] = Linv[r][c] are contiguous over the lanes
+#pragma unroll
+  for (int r = 0; r < NB; r++) D[c][r] = x[r];
+  wave_sync_lds();
   double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
 #pragma unroll 16
-  for (int t = 0; t < NB; t++) Lt[t * NB + i] = i >= t ? D[t][i] : 0.0;
+  for (int t = 0; t < NB; t++) Lt[t * NB + i] = D[t][i];
 }
 
 // Rows below the diagonal block: X = A L11^{-T} as a 64x64x64 FP64 MFMA product per task, written back into the
@@ -820,13 +821,15 @@ __device__ __forceinline__ void fwdb_panel_item(const int4 tk, SymDev sd,
 }
 
 // Forward, last part: V(row) = W(row) - L(row, 0 .. k) y_own for the update-vector rows (row >= k) of a wide front,
-// after all of its diagonal steps: one launch per level with many workgroups, 64 rows each, the waves splitting the
-// front's columns (loads issued BT at a time) and combining their partial sums in a fixed order.
-// task: (s, first row, 0, 0)
+// after all of its diagonal steps: one launch per level with many workgroups, 64 rows each, the waves splitting a
+// column range of the front and combining their partial sums in a fixed order. Fronts wider than RECT_SPLIT columns
+// are cut into column ranges so that the upper levels (few row blocks, thousands of columns) fill the chip: such a
+// task writes its 64 partial sums to Pp[task] and k_fwdb_rect_sum adds the ranges of a row block in range order.
+// task: (s, first row, first column, end column); tid: the task's index in the level's task list.
 template <typename T>
-__device__ __forceinline__ void fwdb_rect_item(const int4 tk, SymDev sd, const T* __restrict__ L,
+__device__ __forceinline__ void fwdb_rect_item(const int4 tk, long long tid, SymDev sd, const T* __restrict__ L,
                                                    const double* __restrict__ y, const double* __restrict__ Wg,
-                                                   double* __restrict__ V) {
+                                                   double* __restrict__ V, double* __restrict__ Pp) {
   __shared__ double part[RECT_WAVES][NB];
   const int s = tk.x;
   const int f = sd.sn_first[s];
@@ -834,12 +837,12 @@ __device__ __forceinline__ void fwdb_rect_item(const int4 tk, SymDev sd, const T
   const long long m = sn_m(sd, s);
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long row = (long long)tk.y + lane;
-  const int per = (((k + RECT_WAVES - 1) / RECT_WAVES + BT - 1) / BT) * BT;
-  const int q0 = wv * per, q1 = min(k, q0 + per);
+  const int qa = tk.z, qb = tk.w;
+  const int per = (((qb - qa + RECT_WAVES - 1) / RECT_WAVES + BT - 1) / BT) * BT;
+  const int q0 = qa + wv * per, q1 = min(qb, q0 + per);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (q0 < q1) {
-    // every lane runs the loop (the lane reads below need all lanes' operands); rows past the front repeat its last
-    // row and are not stored
+    // every lane runs the loop (wave-uniform column range); rows past the front repeat its last row, not stored
     const T* pr = L + sd.Loff[s] + min(row, m - 1);
     sweep_rows_dot<T>(pr, (int)m, y + f, q0, q1, acc);
   }
@@ -849,8 +852,24 @@ __device__ __forceinline__ void fwdb_rect_item(const int4 tk, SymDev sd, const T
     double a = 0.0;
 #pragma unroll
     for (int q = 0; q < RECT_WAVES; q++) a += part[q][lane];
-    V[sd.Voff[s] + row - k] = Wg[sd.Woff[s] + row] - a;
+    if (qa == 0 && qb == k) V[sd.Voff[s] + row - k] = Wg[sd.Woff[s] + row] - a;
+    else Pp[tid * NB + lane] = a;
   }
+}
+
+// V(row) = W(row) - sum over the row block's column ranges (in range order) of their partial sums.
+// task: (s, first row, index of the first range's task, number of ranges)
+__device__ __forceinline__ void fwdb_rect_sum_item(const int4 tk, SymDev sd, const double* __restrict__ Wg,
+                                                       double* __restrict__ V, const double* __restrict__ Pp) {
+  const int s = tk.x;
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int lane = threadIdx.x;
+  const long long row = (long long)tk.y + lane;
+  if (row >= m) return;
+  double a = 0.0;
+  for (int g = 0; g < tk.w; g++) a += Pp[((long long)tk.z + g) * NB + lane];
+  V[sd.Voff[s] + row - k] = Wg[sd.Woff[s] + row] - a;
 }
 
 // Backward, first part: the solution at a wide front's lower rows, gathered into one contiguous vector (the front's
@@ -1025,11 +1044,18 @@ __global__ __launch_bounds__(RECT_WAVES * 64) void k_fwdb_panel(const int4* __re
 }
 
 template <typename T>
-__global__ __launch_bounds__(RECT_WAVES * 64) void k_fwdb_rect(const int4* __restrict__ tasks, SymDev sd,
-                                                               const T* __restrict__ L, const double* __restrict__ y,
-                                                               const double* __restrict__ Wg,
-                                                               double* __restrict__ V) {
-  fwdb_rect_item<T>(tasks[blockIdx.x], sd, L, y, Wg, V);
+__global__ __launch_bounds__(RECT_WAVES * 64) void k_fwdb_rect(const int4* __restrict__ tasks, long long tbase,
+                                                               SymDev sd, const T* __restrict__ L,
+                                                               const double* __restrict__ y,
+                                                               const double* __restrict__ Wg, double* __restrict__ V,
+                                                               double* __restrict__ Pp) {
+  fwdb_rect_item<T>(tasks[blockIdx.x], tbase + blockIdx.x, sd, L, y, Wg, V, Pp);
+}
+
+__global__ __launch_bounds__(64) void k_fwdb_rect_sum(const int4* __restrict__ tasks, SymDev sd,
+                                                      const double* __restrict__ Wg, double* __restrict__ V,
+                                                      const double* __restrict__ Pp) {
+  fwdb_rect_sum_item(tasks[blockIdx.x], sd, Wg, V, Pp);
 }
 
 __global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks, SymDev sd,
@@ -1089,7 +1115,7 @@ hipEvent_t SpdDevice::event(size_t i) {
 }
 
 size_t SpdDevice::device_bytes() const {
-  return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
+  return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + Pr_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
          sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() + zero_tasks_.bytes() +
          Linv_.bytes() + X_.bytes() + XT_.bytes() + X32_.bytes() + XT32_.bytes() + gptr_.bytes() + gsrc_.bytes() +
          fwd_tasks_.bytes() + bwd_tasks_.bytes();
@@ -1272,6 +1298,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       for (long long i = 0; i < mu; i++) gsrc[base + cnt[rel[i]]++] = S.Voff[c] + i;
     }
   }
+  long long rect_part_max = 0;  // tasks of the largest forward rect launch (partial-sum slots)
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
     std::vector<int> small, big;
@@ -1311,10 +1338,26 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       L.fwd_diag.push_back(d);
       L.fwd_panel.push_back(pn);
     }
+    // update-vector rows in 64-row blocks; fronts wider than RECT_SPLIT columns in equal column ranges (partial sums
+    // added by a second launch, k_fwdb_rect_sum, in range order)
     L.fwd_rect.begin = (long long)fwt.size();
-    for (int s : big)
-      for (long long r0 = sn_kk(s); r0 < sn_mm(s); r0 += NB) fwt.push_back(make_int4(s, (int)r0, 0, 0));
+    std::vector<int4> sums;
+    for (int s : big) {
+      const int k = sn_kk(s);
+      const int g = (k + RECT_SPLIT - 1) / RECT_SPLIT;
+      const int cw = (k + g - 1) / g;
+      for (long long r0 = k; r0 < sn_mm(s); r0 += NB) {
+        const long long t0 = (long long)fwt.size() - L.fwd_rect.begin;
+        int nr = 0;
+        for (int q = 0; q < k; q += cw, nr++) fwt.push_back(make_int4(s, (int)r0, q, std::min(k, q + cw)));
+        if (nr > 1) sums.push_back(make_int4(s, (int)r0, (int)t0, nr));
+      }
+    }
     L.fwd_rect.end = (long long)fwt.size();
+    rect_part_max = std::max(rect_part_max, L.fwd_rect.end - L.fwd_rect.begin);
+    L.fwd_rect_sum.begin = (long long)fwt.size();
+    fwt.insert(fwt.end(), sums.begin(), sums.end());
+    L.fwd_rect_sum.end = (long long)fwt.size();
     // backward
     L.bwd_xb.begin = (long long)bwt.size();
     for (int s : big)
@@ -1380,6 +1423,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   level_sn_.upload(lvl_sn, stream_);
   V_.alloc(std::max<long long>(1, S.Vsize));
   W_.alloc(std::max<long long>(1, Woff[ns]));
+  Pr_.alloc(std::max<long long>(1, rect_part_max * NB));
   y_.alloc(n);
   err_.alloc(1);
   sd_ = SymDev{sn_first_.p, sn_rowptr_.p, sn_rows_.p, Loff_.p, Uoff_.p, Voff_.p, Woff_.p, rel_off_.p, rel_.p,
@@ -1543,8 +1587,9 @@ void SpdDevice::sweeps(const T* L) {
     }
     if (lv.fwd_rect.end > lv.fwd_rect.begin)
       hipLaunchKernelGGL(k_fwdb_rect<T>, dim3((unsigned)(lv.fwd_rect.end - lv.fwd_rect.begin)), dim3(RECT_WAVES * 64),
-                         0, stream_, fwd_tasks_.p + lv.fwd_rect.begin, sd_, L, (const double*)y_.p,
-                         (const double*)W_.p, V_.p);
+                         0, stream_, fwd_tasks_.p + lv.fwd_rect.begin, 0LL, sd_, L, (const double*)y_.p,
+                         (const double*)W_.p, V_.p, Pr_.p);
+    launch_n(k_fwdb_rect_sum, 64, lv.fwd_rect_sum, fwd_tasks_.p, (const double*)W_.p, V_.p, (const double*)Pr_.p);
   }
   for (size_t l = levels_.size(); l-- > 0;) {
     const Level& lv = levels_[l];

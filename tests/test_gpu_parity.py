@@ -151,6 +151,37 @@ def test_spd_wide_fronts(precision, nxy, nz, min_tail):
             assert h.stats()["refine_iterations"] >= 1
 
 
+@pytest.mark.parametrize("precision", [0, 2])
+def test_spd_split_sweep_tasks(precision):
+    """Dense blocks [A B 0; B' B C; 0 C' C] of 2,100 columns each under the identity ordering: two supernodes, the
+    first with 2,100 columns and 2,100 update-vector rows, so its forward update-vector products run as two column
+    ranges (partial sums added in range order by the summing launch); the root front (4,200 columns) takes three
+    sweep steps. Random diagonally dominant SPD values; backward error 1e-13 with the FP64 factor, or after
+    refinement with the FP32 copy, and the dense LAPACK solution."""
+    a = 2100
+    n = 3 * a
+    rng = np.random.default_rng(11)
+    M = rng.uniform(-1, 1, (n, n))
+    M = 0.5 * (M + M.T)
+    M[:a, 2 * a:] = 0.0
+    M[2 * a:, :a] = 0.0
+    M[np.arange(n), np.arange(n)] = np.abs(M).sum(axis=1) + 1.0
+    A = sp.csc_matrix(M)
+    A.sort_indices()
+    b = rng.standard_normal(n)
+    xr = np.linalg.solve(M, b)
+    del M
+    with spd.HipSpdSolver(precision=precision, refine_tol=1e-14) as h:
+        h.Analyze(n, A.indptr, A.indices, perm=np.arange(n, dtype=np.int32))
+        _, parent, cc = h.symbolic()
+        assert cc[0] == 2 * a and (parent < 0).sum() == 1
+        h.Factorize(A.data)
+        for _ in range(2):  # the second solve replays the captured sweep graph
+            xs = h.Solve(b)
+            assert np.linalg.norm(A @ xs - b) <= 1e-13 * np.linalg.norm(b)
+            assert np.linalg.norm(xs - xr) <= 1e-12 * np.linalg.norm(xr)
+
+
 def test_spd_not_positive_definite(c1_system):
     o, cp, ri, A, b = c1_system
     bad = A.copy()
