@@ -197,7 +197,9 @@ def copy_probe(nbytes: int, reps: int = 3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=8,
+                    help="timed stroke increments per arm (after the warmup increments); the first increments after "
+                         "the cold start, with the contact set still developing, are the slowest")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hexdie", choices=["hexdie", "upset", "conrod"],
                     help="hexdie: the 360k-DOF hex-head closed die (the headline line); conrod: the ~1M-DOF "
