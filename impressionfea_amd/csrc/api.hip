@@ -352,6 +352,11 @@ int ifea_step_set_v(ifea_step_t h, const double* v) {
   IFEA_GUARD(h->dev->set_v(v))
 }
 
+int ifea_step_quality(ifea_step_t h, const double* x, double qmin, int* first_bad, double* quality) {
+  if (!h || !x || !first_bad) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(*first_bad = h->dev->quality(x, qmin, quality))
+}
+
 int ifea_step_info(ifea_step_t h, int reset, double* out24) {
   if (!h || !out24) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
   IFEA_GUARD(h->dev->info(reset, out24))

@@ -30,6 +30,8 @@ class StepDevice {
   void advance(double dt, double* eps, double* sigma, double* rate);
   void get_v(double* v);
   void set_v(const double* v);
+  // index of the first tet whose quality at the host positions x is <= qmin, or -1; per-tet values into q if given
+  int quality(const double* x, double qmin, double* q);
   void info(int reset, double* out);
 
  private:
@@ -64,6 +66,7 @@ class StepDevice {
   // iteration
   DBuf<double> v_, r_, dv_, vtrial_, val_, e_wm_, e_wn_, e_g_, e_energy_, e_power_, n_energy_, n_power_;
   DBuf<double> pr_, pz_, pp_, pq_, tmp_;
+  DBuf<double> xq_, qual_;  // positions and per-tet values of the quality check
   DBuf<unsigned long long> maxdiag_;
 };
 

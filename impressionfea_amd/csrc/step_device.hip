@@ -464,6 +464,39 @@ __global__ void k_final_nodes(int nn, const double* __restrict__ ax, const doubl
   comp[n] = cp;
 }
 
+// TetMesh.Quality (TetMesh.cs:107-112) of every tet at the node positions x: 6 sqrt(2) V / L_rms^3 with L_rms^2 the
+// mean squared edge length (1 for a regular tet, <= 0 inverted, -1 for a degenerate one). The smallest index of a tet
+// at or below qmin goes to *first by an integer atomic minimum (the same index whatever the order of the lanes).
+__global__ void k_quality(int ne, const int* __restrict__ tets, const double* __restrict__ x, double qmin,
+                          double* __restrict__ q, int* __restrict__ first) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double p[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int n = tets[4 * e + a];
+#pragma unroll
+    for (int c = 0; c < 3; c++) p[a][c] = x[3 * n + c];
+  }
+  auto d2 = [&](int a, int b) {
+    const double dx = p[b][0] - p[a][0], dy = p[b][1] - p[a][1], dz = p[b][2] - p[a][2];
+    return dx * dx + dy * dy + dz * dz;
+  };
+  const double l2 = (d2(0, 1) + d2(0, 2) + d2(0, 3) + d2(1, 2) + d2(1, 3) + d2(2, 3)) / 6.0;
+  double u[3], v[3], w[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    u[c] = p[1][c] - p[0][c];
+    v[c] = p[2][c] - p[0][c];
+    w[c] = p[3][c] - p[0][c];
+  }
+  const double vol = (u[0] * (v[1] * w[2] - v[2] * w[1]) + u[1] * (v[2] * w[0] - v[0] * w[2]) +
+                      u[2] * (v[0] * w[1] - v[1] * w[0])) / 6.0;
+  const double qe = l2 > 0.0 ? 6.0 * sqrt(2.0) * vol / (l2 * sqrt(l2)) : -1.0;
+  if (q) q[e] = qe;
+  if (qe <= qmin) atomicMin(first, e);
+}
+
 __global__ void k_advance(int ne, double dt, const double* __restrict__ rate, double* __restrict__ eps) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < ne) eps[e] += rate[e] * dt;
@@ -868,6 +901,22 @@ void StepDevice::advance(double dt, double* eps, double* sigma, double* rate) {
   if (sigma) IFEA_HIP(hipMemcpyAsync(sigma, el_sig_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
   if (rate) IFEA_HIP(hipMemcpyAsync(rate, el_rate_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
   IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+int StepDevice::quality(const double* x, double qmin, double* q) {
+  xq_.upload(x, 3 * (size_t)nn_, s_);
+  bad_.alloc(1);
+  const int big = 0x7fffffff;
+  IFEA_HIP(hipMemcpyAsync(bad_.p, &big, sizeof(int), hipMemcpyHostToDevice, s_));
+  if (q) qual_.alloc(ne_);
+  hipLaunchKernelGGL(k_quality, dim3(grid_for(ne_, 256)), dim3(256), 0, s_, ne_, tets_.p, xq_.p, qmin,
+                     q ? qual_.p : nullptr, bad_.p);
+  IFEA_HIP(hipGetLastError());
+  int first = big;
+  IFEA_HIP(hipMemcpyAsync(&first, bad_.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+  if (q) IFEA_HIP(hipMemcpyAsync(q, qual_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+  return first == big ? -1 : first;
 }
 
 void StepDevice::get_v(double* v) {

@@ -52,6 +52,13 @@ def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
         vals[idx, k[m]] = nrm @ dvel
         die_idx[idx] = k[m]
         k[m] += 1
+    # nodes without constraints keep the identity frame: the Gram-Schmidt below runs on the constrained ones only
+    full_e, full_ev = np.broadcast_to(np.eye(3), (nn, 3, 3)).copy(), np.zeros((nn, 3))
+    full_mc, full_da = np.zeros(nn, dtype=np.int64), np.full(nn, -1, dtype=np.int64)
+    sub = np.nonzero(k > 0)[0]
+    dirs, vals, k, die_idx = dirs[sub], vals[sub], k[sub], die_idx[sub]
+    nn_all, nn = nn, sub.size
+    ar = np.arange(nn)
     e = np.zeros((nn, 3, 3))
     ev = np.zeros((nn, 3))
     mc = np.zeros(nn, dtype=np.int64)
@@ -88,22 +95,14 @@ def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
     if m2.any():
         e2 = np.cross(e[m2, 0], e[m2, 1])
         e[m2, 2] = e2 / np.linalg.norm(e2, axis=1, keepdims=True)
+    full_e[sub], full_ev[sub], full_mc[sub], full_da[sub] = e, ev, mc, die_axis
+    e, ev, mc, die_axis, nn = full_e, full_ev, full_mc, full_da, nn_all
     fixed_val = np.zeros(3 * nn)
     fv = fixed_val.reshape(nn, 3)
     for q in range(3):
         mq = q < mc
         fv[mq, q] = ev[mq, q]
     return e, mc.astype(np.int32), die_axis.astype(np.int32), fixed_val
-
-
-def tet_quality(x, tets):
-    """TetMesh.Quality (TetMesh.cs:107-112), vectorised."""
-    a, b, c, d = (x[tets[:, i]] for i in range(4))
-    l2 = (np.sum((b - a) ** 2, 1) + np.sum((c - a) ** 2, 1) + np.sum((d - a) ** 2, 1) + np.sum((c - b) ** 2, 1)
-          + np.sum((d - b) ** 2, 1) + np.sum((d - c) ** 2, 1)) / 6.0
-    vol = np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a)) / 6.0
-    with np.errstate(invalid="ignore", divide="ignore"):
-        return np.where(l2 > 0, 6 * np.sqrt(2.0) * vol / l2 ** 1.5, -1.0)
 
 
 @dataclass
@@ -296,6 +295,16 @@ class Forging3DSolver:
         self.set_v(cands[best])
         return best
 
+    def first_collapsed(self, p, qmin: float, quality=None) -> int:
+        """Index of the first tet whose TetMesh.Quality at the node positions p is <= qmin, or -1 (the collapse check
+        of TryUpdate, Forging3DSolver.cs:1509-1516), evaluated on the device; per-tet values into `quality` if given."""
+        ps = N.f64(p)
+        first = C.c_int(-1)
+        qp = N.ptr(quality, C.c_double) if quality is not None else None
+        self._check(N.lib().ifea_step_quality(self._h, N.ptr(ps, C.c_double), float(qmin), C.byref(first), qp),
+                    "quality")
+        return first.value
+
     def solve_velocity(self):
         st = N.StepStats()
         self._check(N.lib().ifea_step_solve_velocity(self._h, int(self.warm), C.byref(st)), "SolveVelocity")
@@ -353,9 +362,9 @@ class Forging3DSolver:
         p = self._sym(p)
         if max_pen > 0.5:
             return "die penetration exceeds half the local element size"
-        q = tet_quality(p, self.mesh.tets)
-        if (q <= 0.005).any():
-            return f"tetrahedron {int(np.argmax(q <= 0.005))} collapsed"
+        bad = self.first_collapsed(p, 0.005)
+        if bad >= 0:
+            return f"tetrahedron {bad} collapsed"
         self.x = p
         self.contact = cT
         self.gap = gapT

@@ -94,6 +94,11 @@ int ifea_step_advance(ifea_step_t h, double dt, double* eps, double* sigma, doub
 int ifea_step_get_v(ifea_step_t h, double* v);
 int ifea_step_set_v(ifea_step_t h, const double* v);
 
+/* The collapse check of TryUpdate (Forging3DSolver.cs:1509-1516): TetMesh.Quality (TetMesh.cs:107-112) of every tet at
+ * the node positions x[3nn] (host), first_bad = the smallest index of a tet with quality <= qmin, or -1; quality[ne]
+ * receives every tet's value when not NULL. */
+int ifea_step_quality(ifea_step_t h, const double* x, double qmin, int* first_bad, double* quality);
+
 /* Device and factor statistics (24 doubles): [0] nnz(L), [1] factor flops, [2] supernodes, [3] device bytes,
  * [4] last factor ms, [5] L words, [6] update-pool words, [7] levels; with params.profile set, accumulated since the
  * last call with reset: [8] inner (64-column) MFMA update launches, [9] their ms, [10] their algorithmic flops,

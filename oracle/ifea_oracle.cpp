@@ -1346,6 +1346,13 @@ void orc_reset_factor(void* h) { auto* o = (Oracle*)h; o->haveFactor = false; o-
 
 long long orc_factor_nnz(void* h) { return ((Oracle*)h)->FactorNonZeros(); }
 
+// TetMesh.Quality (TetMesh.cs:107-112) of every tet at the node positions x[3nn]: the restatement TryUpdate uses
+void orc_quality(int ne, const int* tets, const double* x, double* q) {
+  auto P = [&](int n) { return V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]); };
+  for (int e = 0; e < ne; e++)
+    q[e] = Oracle::Quality(P(tets[4 * e]), P(tets[4 * e + 1]), P(tets[4 * e + 2]), P(tets[4 * e + 3]));
+}
+
 // ---- stand-alone sparse Cholesky (full symmetric CSC in, FP64) for tests and the CPU baseline
 void* orc_chol_create(int n, const int* colPtr, const int* rowIdx, const int* perm, int* parent_out, int* colcount_out, int* rc) {
   auto* c = new SparseChol();

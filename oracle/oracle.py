@@ -71,6 +71,7 @@ def lib():
         L.orc_reset_factor.argtypes = [vp]
         L.orc_factor_nnz.restype = C.c_longlong
         L.orc_factor_nnz.argtypes = [vp]
+        L.orc_quality.argtypes = [C.c_int, ip, dp, dp]
         L.orc_chol_create.restype = vp
         L.orc_chol_create.argtypes = [C.c_int, ip, ip, ip, ip, ip, ip]
         L.orc_chol_factor.argtypes = [vp, dp]
@@ -298,6 +299,14 @@ class OracleCholesky:
         if getattr(self, "h", None):
             lib().orc_chol_destroy(self.h)
             self.h = None
+
+
+def quality(x, tets):
+    """TetMesh.Quality (TetMesh.cs:107-112) of every tet at the node positions x (oracle restatement)."""
+    t, xs = I32(tets), F64(x)
+    q = np.zeros(t.shape[0])
+    lib().orc_quality(t.shape[0], _p(t, C.c_int), _p(xs, C.c_double), _p(q, C.c_double))
+    return q
 
 
 def brute_symbolic(n, colPtr, rowIdx, perm):

@@ -307,6 +307,31 @@ def test_increments_parity_config1(oracle_lib):
         assert rs.load == pytest.approx(ro["load"], rel=1e-6)
 
 
+def test_quality_check_matches_oracle(oracle_lib):
+    """TryUpdate's collapse check on the device (ifea_step_quality): TetMesh.Quality of every tet of a jittered upset
+    mesh against the oracle's restatement, and the first tet at or below the threshold after one node is moved into
+    the plane of three others (the reference stops at the first such tet, Forging3DSolver.cs:1509-1516)."""
+    import oracle as O
+    wl = upset_small()
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=3)
+    s = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0)
+    q = np.zeros(t.shape[0])
+    assert s.first_collapsed(x, 0.005, q) == -1
+    qo = O.quality(x, t)
+    assert np.max(np.abs(q - qo)) <= 1e-13 * np.max(np.abs(qo))
+    assert s.first_collapsed(x, 2.0) == 0
+    e = t.shape[0] // 2
+    a, b, c, d = (x[t[e, i]] for i in range(4))
+    nrm = np.cross(b - a, c - a)
+    y = x.copy()
+    y[t[e, 3]] = d - np.dot(d - a, nrm) / np.dot(nrm, nrm) * nrm  # onto the plane of a, b, c
+    qy = O.quality(y, t)
+    want = int(np.argmax(qy <= 0.005))
+    assert qy[want] <= 0.005 and want <= e
+    assert s.first_collapsed(y, 0.005) == want
+    s.close()
+
+
 def test_frictionless_upset_gpu():
     """Solver3DTests.cs:30-49 on the GPU path: F = sigma(rate) A within 1 %."""
     wl = upset_small(nxy=6, nz=6, friction=0.0, tolerance=1e-8)

@@ -411,3 +411,22 @@ def test_conrod_billet_placement(oracle_lib):
     depth = top.height(gx, gy) - 0.5 * top.p[0]     # the gutter term vanishes inside the land
     cavity = 2 * np.where(near, depth, 0.0).sum() * h * h
     assert 0.6 * vol.sum() < cavity < vol.sum()
+
+
+def test_quality_known_answers(oracle_lib):
+    """TetMesh.Quality (TetMesh.cs:107-112) in the oracle: 1 for a regular tet, -1 for its mirror image, 0 for a flat
+    one, -1 for a point, invariant under scaling and translation, and sqrt(2) / (5/3)^1.5 for the Kuhn tet of a unit
+    cube (edges 1, 1, 1, sqrt2, sqrt2, sqrt3: mean squared edge 5/3, volume 1/6)."""
+    import oracle as O
+    reg = np.array([[0, 0, 0], [1, 0, 0], [0.5, np.sqrt(3) / 2, 0], [0.5, np.sqrt(3) / 6, np.sqrt(2 / 3)]])
+    kuhn = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [1, 1, 1.0]])
+    flat = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0.3, 0.4, 0.0]])
+    pts = np.vstack([reg, reg[[1, 0, 2, 3]], flat, np.zeros((4, 3)), 7.3 * reg + [2.0, -5.0, 11.0], kuhn])
+    tets = np.arange(24).reshape(6, 4)
+    q = O.quality(pts, tets)
+    assert q[0] == pytest.approx(1.0, abs=1e-14)
+    assert q[1] == pytest.approx(-1.0, abs=1e-14)
+    assert abs(q[2]) < 1e-15
+    assert q[3] == -1.0
+    assert q[4] == pytest.approx(1.0, abs=1e-13)
+    assert q[5] == pytest.approx(np.sqrt(2) / (5 / 3) ** 1.5, rel=1e-14)
