@@ -134,11 +134,11 @@ __global__ __launch_bounds__(256) void k_extend(const int4* __restrict__ tasks, 
   }
 }
 
-// Diagonal block of one 64-column step: FP64 Cholesky in one wave, lane i owning row i of the block in LDS; the
-// factored column goes to a separate (double-buffered) LDS vector, so the row updates of a step are independent
-// loads and stores that pipeline. Then the inverse of the factor (lane c solves L x = e_c with the factor broadcast
-// from LDS). The factor goes into the panel; the inverse is kept (Lt[c][r] = Linv[r][c]) for the TRSM launch that
-// follows and for the triangular sweeps. Columns beyond the block width are the identity. task: (supernode s, 0, 0, 0)
+// Diagonal block of one 64-column step: FP64 Cholesky in one wave, lane i owning row i of the block (in LDS, and in
+// registers for the 16-column panel being factored). Then the inverse of the factor (lane c solves L x = e_c with the
+// factor broadcast from LDS). The factor goes into the panel; the inverse is kept (Lt[c][r] = Linv[r][c]) for the TRSM
+// launch that follows and for the triangular sweeps. Columns beyond the block width are the identity.
+// task: (supernode s, 0, 0, 0)
 template <typename T>
 __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks, int c0, SymDev sd,
                                                   T* __restrict__ L, double* __restrict__ Linv,
@@ -150,52 +150,54 @@ __global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks
   const int w = min(NB, k - c0);
   T* P = L + sd.Loff[s];
   __shared__ double D[NB][NB + 1];
-  __shared__ double colb[2][NB];
   __shared__ double rdiag[NB];  // 1 / L[j][j]
   const int i = threadIdx.x;
   const T* prow = P + (long long)c0 * m + c0 + i;
-  // the 64 column loads of the block are independent: issue them 16 at a time
-  for (int j0 = 0; j0 < NB; j0 += 16) {
-    double v[16];
+  // the 64 column loads of the block are independent: all of them in flight at once
+  {
+    double v[NB];
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-      const int j = j0 + t;
-      v[t] = (i < w && j < w && i >= j) ? (double)prow[(long long)j * m] : 0.0;
-    }
+    for (int j = 0; j < NB; j++) v[j] = (i < w && j < w && i >= j) ? (double)prow[(long long)j * m] : 0.0;
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-      const int j = j0 + t;
-      D[i][j] = (i < w && j < w) ? v[t] : ((i == j) ? 1.0 : 0.0);
-    }
+    for (int j = 0; j < NB; j++) D[i][j] = (i < w && j < w) ? v[j] : ((i == j) ? 1.0 : 0.0);
   }
   wave_sync_lds();
   bool bad = false;
   double* row = &D[i][0];
-  // two columns per pass over the trailing rows: column j, its update of column j + 1, column j + 1, then one pass
-  // that applies both (in the order of two single-column passes, so the results are the same)
-  for (int j = 0; j < NB; j += 2) {
-    const double piv0 = D[j][j];
-    bad |= !(piv0 > 0.0) || !isfinite(piv0);
-    const double d0 = sqrt(piv0 > 0.0 ? piv0 : 1.0);
-    const double l0 = (i > j) ? row[j] / d0 : ((i == j) ? d0 : 0.0);
-    if (i == j) rdiag[j] = 1.0 / d0;
-    row[j] = l0;
-    colb[0][i] = l0;
+  // 16-column panels: lane i keeps its row of the panel in registers and factors the panel's columns with the pivots
+  // and column entries taken by lane reads (no LDS round trip inside the panel); then every column right of the panel
+  // takes one 16-term update from the panel's rows in LDS. Entries right of the diagonal are cleared afterwards.
+  for (int pc = 0; pc < NB; pc += 16) {
+    double r[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) r[jj] = row[pc + jj];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int j = pc + jj;
+      const double piv = lane_read(r[jj], j);
+      bad |= !(piv > 0.0) || !isfinite(piv);
+      const double d = sqrt(piv > 0.0 ? piv : 1.0);
+      const double inv = 1.0 / d;
+      const double l = (i > j) ? r[jj] * inv : ((i == j) ? d : 0.0);
+      if (i == j) rdiag[j] = inv;
+      r[jj] = l;
+#pragma unroll
+      for (int kk = jj + 1; kk < 16; kk++) r[kk] -= l * lane_read(l, pc + kk);
+    }
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) row[pc + jj] = r[jj];
     wave_sync_lds();
-    row[j + 1] -= l0 * colb[0][j + 1];
-    wave_sync_lds();
-    const double piv1 = D[j + 1][j + 1];
-    bad |= !(piv1 > 0.0) || !isfinite(piv1);
-    const double d1 = sqrt(piv1 > 0.0 ? piv1 : 1.0);
-    const double l1 = (i > j + 1) ? row[j + 1] / d1 : ((i == j + 1) ? d1 : 0.0);
-    if (i == j + 1) rdiag[j + 1] = 1.0 / d1;
-    row[j + 1] = l1;
-    colb[1][i] = l1;
-    wave_sync_lds();
-    const double* c0b = colb[0];
-    const double* c1b = colb[1];
-#pragma unroll 8
-    for (int q = j + 2; q < NB; q++) row[q] = (row[q] - l0 * c0b[q]) - l1 * c1b[q];
+#pragma unroll 4
+    for (int q = pc + 16; q < NB; q++) {
+      const double* Dq = &D[q][pc];
+      double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 16; jj += 2) {
+        a0 += r[jj] * Dq[jj];
+        a1 += r[jj + 1] * Dq[jj + 1];
+      }
+      row[q] -= a0 + a1;
+    }
     wave_sync_lds();
   }
   for (int q = i + 1; q < NB; q++) row[q] = 0.0;
