@@ -20,6 +20,9 @@ namespace ifea {
 constexpr int MAX_DEG = 192;    // two-ring neighbours per node
 constexpr int MAX_PATCH = 64;   // one-ring patch size per node
 constexpr int NODE_WAVES = 4;   // nodes per 256-thread block
+// dynamic LDS of the per-node assembly kernels for a mesh whose largest neighbour list has md nodes (sized to the
+// mesh, not to MAX_DEG, so that more workgroups share a CU)
+inline size_t node_lds_bytes(int md) { return (size_t)NODE_WAVES * md * (9 * sizeof(double) + sizeof(int)); }
 
 __device__ __forceinline__ double law_stress(const ifea_step_params& P, double strain, double rate) {
   return P.C * pow(P.eps0 + fmax(strain, 0.0), P.n) * pow(fmax(rate, 1e-12), P.m);
@@ -106,19 +109,18 @@ __global__ void k_gcoef(int nn, const int* __restrict__ tets, const int* __restr
   }
 }
 
-__global__ __launch_bounds__(256) void k_kvol(int nn, double kpen, const int* __restrict__ colPtr,
+__global__ __launch_bounds__(256) void k_kvol(int md, int nn, double kpen, const int* __restrict__ colPtr,
                                               const int* __restrict__ nbs, const int* __restrict__ nb,
                                               const int* __restrict__ ps, const int* __restrict__ pn,
                                               const double* __restrict__ ax, const double* __restrict__ vnode,
                                               const double* __restrict__ gcoef, double* __restrict__ kvol) {
-  __shared__ int s_nbr[NODE_WAVES][MAX_DEG];
-  __shared__ double s_acc[NODE_WAVES][9 * MAX_DEG];
+  extern __shared__ double node_lds[];  // node_lds_bytes(md): per wave 9 md accumulators, then md neighbour ids
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int j = blockIdx.x * NODE_WAVES + wv;
   if (j >= nn) return;
   const int deg = nbs[j + 1] - nbs[j];
-  int* nbr = s_nbr[wv];
-  double* acc = s_acc[wv];
+  double* acc = node_lds + (long long)wv * 9 * md;
+  int* nbr = (int*)(node_lds + (long long)NODE_WAVES * 9 * md) + wv * md;
   for (int q = lane; q < deg; q += 64) nbr[q] = nb[nbs[j] + q];
   for (int q = lane; q < 9 * deg; q += 64) acc[q] = 0.0;
   wave_sync_lds();
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(256) void k_elem(int ne, ifea_step_params P, int ne
 // ------------------------------------------------------------------------------- Assemble (:1115-1217), node pass
 // One wave per node j: its 3 CSC columns (and its 3 residual rows by symmetry). mode 0: raw residual; mode 1:
 // BC identity rows/cols, r[fixed] = 0 and r = -r (SolveVelocity :892-917); mode 2: energy only.
-__global__ __launch_bounds__(256) void k_node(int nn, ifea_step_params P, int matrix, int newton, int mode,
+__global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P, int matrix, int newton, int mode,
                                               const int* __restrict__ tets, const int* __restrict__ nts,
                                               const int* __restrict__ ntl, const int* __restrict__ colPtr,
                                               const int* __restrict__ rowIdx, const int* __restrict__ nbs,
@@ -283,15 +285,16 @@ __global__ __launch_bounds__(256) void k_node(int nn, ifea_step_params P, int ma
                                               double* __restrict__ r, double* __restrict__ val,
                                               double* __restrict__ n_energy, double* __restrict__ n_power,
                                               unsigned long long* __restrict__ maxdiag) {
-  __shared__ int s_nbr[NODE_WAVES][MAX_DEG];
-  __shared__ double s_acc[NODE_WAVES][9 * MAX_DEG];
+  // with matrix set: node_lds_bytes(md) of dynamic LDS (per wave 9 md accumulators, then md neighbour ids); the
+  // residual-only assemblies touch no LDS and are launched without it
+  extern __shared__ double node_lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int j = blockIdx.x * NODE_WAVES + wv;
   if (j >= nn) return;
   const int deg = nbs[j + 1] - nbs[j];
   const int d3 = 3 * deg;
-  int* nbr = s_nbr[wv];
-  double* acc = s_acc[wv];
+  double* acc = node_lds + (long long)wv * 9 * md;
+  int* nbr = (int*)(node_lds + (long long)NODE_WAVES * 9 * md) + wv * md;
   if (matrix) {
     for (int q = lane; q < deg; q += 64) nbr[q] = nb[nbs[j] + q];
     for (int c = 0; c < 3; c++) {
@@ -645,8 +648,9 @@ void StepDevice::prepare(const double* x, const double* ax, const int* nfix, con
   hipLaunchKernelGGL(k_vnode, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, nt_start_.p, nt_list_.p, vol_.p, vnode_.p);
   hipLaunchKernelGGL(k_gcoef, dim3(grid_for(nn_, 128)), dim3(128), 0, s_, nn_, tets_.p, nt_start_.p, nt_list_.p,
                      p_start_.p, p_node_.p, vol_.p, vnode_.p, grad_.p, gcoef_.p);
-  hipLaunchKernelGGL(k_kvol, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), 0, s_, nn_, P_.kpen, colPtr_.p, nbr_start_.p,
-                     nbr_.p, p_start_.p, p_node_.p, ax_.p, vnode_.p, gcoef_.p, kvol_.p);
+  hipLaunchKernelGGL(k_kvol, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), node_lds_bytes(max_deg_), s_, max_deg_, nn_,
+                     P_.kpen, colPtr_.p, nbr_start_.p, nbr_.p, p_start_.p, p_node_.p, ax_.p, vnode_.p, gcoef_.p,
+                     kvol_.p);
   hipLaunchKernelGGL(k_fric, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, nf_start_.p, nf_list_.p, faces_.p,
                      face_owner_.p, x_.p, el_sig_.p, afric_.p, kfric_.p);
   hipLaunchKernelGGL(k_vguess, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, ax_.p, nfix_.p, fixed_val_.p, tmp_.p,
@@ -666,7 +670,8 @@ double StepDevice::assemble(bool matrix, bool newton, const double* d_v, int mod
                      grad_.p, vol_.p, eps_.p, e_wm_.p, e_wn_.p, e_g_.p, e_energy_.p, e_power_.p, el_sig_.p,
                      el_rate_.p, el_dev_.p);
   if (mode == 1) IFEA_HIP(hipMemsetAsync(maxdiag_.p, 0, sizeof(unsigned long long), s_));
-  hipLaunchKernelGGL(k_node, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), 0, s_, nn_, P_, matrix ? 1 : 0,
+  hipLaunchKernelGGL(k_node, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), matrix ? node_lds_bytes(max_deg_) : 0, s_,
+                     max_deg_, nn_, P_, matrix ? 1 : 0,
                      newton ? 1 : 0, mode, tets_.p, nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p, nbr_start_.p,
                      nbr_.p, ax_.p, nfix_.p, contact_.p, afric_.p, kfric_.p, kvol_.p, d_v, grad_.p, e_wm_.p, e_wn_.p,
                      e_g_.p, r_.p, val_.p, n_energy_.p, n_power_.p, maxdiag_.p);
