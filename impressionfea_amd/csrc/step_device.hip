@@ -272,6 +272,44 @@ __global__ __launch_bounds__(256) void k_elem(int ne, ifea_step_params P, int ne
 // ------------------------------------------------------------------------------- Assemble (:1115-1217), node pass
 // One wave per node j: its 3 CSC columns (and its 3 residual rows by symmetry). mode 0: raw residual; mode 1:
 // BC identity rows/cols, r[fixed] = 0 and r = -r (SolveVelocity :892-917); mode 2: energy only.
+// Friction of a contact node on its free axes (Forging3DSolver.cs:1175-1214): energy, power and residual terms added
+// to en, pw, rj; the slip quantities are returned for the tangent.
+struct NodeFriction {
+  bool on;
+  double c, s[3], s2, sl, z, gs;
+};
+__device__ __forceinline__ NodeFriction node_friction(const ifea_step_params& P, int j, int nf, int ct,
+                                                      const double* __restrict__ Rj, const double* __restrict__ v,
+                                                      const double* __restrict__ afric,
+                                                      const double* __restrict__ kfric, double& en, double& pw,
+                                                      double rj[3]) {
+  NodeFriction f{};
+  f.on = false;
+  if (P.friction > 0 && ct != 0 && nf < 3) {
+    const double c = P.friction * kfric[j] * afric[j] * (2 / M_PI);
+    if (c > 0) {
+      // die velocity in local axes: top die moves with (0, 0, -ram)
+      double vd[3];
+      for (int qq = 0; qq < 3; qq++) vd[qq] = ct == 2 ? -P.ram * Rj[3 * qq + 2] : 0.0;
+      double s2 = 0;
+      for (int qq = 0; qq < 3; qq++) f.s[qq] = 0.0;
+      for (int qq = nf; qq < 3; qq++) { f.s[qq] = v[3 * j + qq] - vd[qq]; s2 += f.s[qq] * f.s[qq]; }
+      const double sl = sqrt(s2), z = sl / P.u0, at = atan(z);
+      en += c * (sl * at - 0.5 * P.u0 * log(1 + z * z));
+      pw += c * at * sl;
+      const double gs = sl > 1e-12 * P.u0 ? at / sl : 1 / P.u0;
+      for (int qq = nf; qq < 3; qq++) rj[qq] += c * gs * f.s[qq];
+      f.on = true;
+      f.c = c;
+      f.s2 = s2;
+      f.sl = sl;
+      f.z = z;
+      f.gs = gs;
+    }
+  }
+  return f;
+}
+
 __global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P, int matrix, int newton, int mode,
                                               const int* __restrict__ tets, const int* __restrict__ nts,
                                               const int* __restrict__ ntl, const int* __restrict__ colPtr,
@@ -365,18 +403,27 @@ __global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P
       // die velocity in local axes: top die moves with (0, 0, -ram)
       double vd[3];
       for (int qq = 0; qq < 3; qq++) vd[qq] = ct == 2 ? -P.ram * Rj[3 * qq + 2] : 0.0;
+      // loops over all three axes with the fixed ones skipped: every array index is a compile-time constant, so the
+      // small arrays stay in registers
       double s[3] = {0, 0, 0};
       double s2 = 0;
-      for (int qq = nf; qq < 3; qq++) { s[qq] = v[3 * j + qq] - vd[qq]; s2 += s[qq] * s[qq]; }
+#pragma unroll
+      for (int qq = 0; qq < 3; qq++)
+        if (qq >= nf) { s[qq] = v[3 * j + qq] - vd[qq]; s2 += s[qq] * s[qq]; }
       double sl = sqrt(s2), z = sl / P.u0, at = atan(z);
       en += c * (sl * at - 0.5 * P.u0 * log(1 + z * z));
       pw += c * at * sl;
       double gs = sl > 1e-12 * P.u0 ? at / sl : 1 / P.u0;
-      for (int qq = nf; qq < 3; qq++) rj[qq] += c * gs * s[qq];
+#pragma unroll
+      for (int qq = 0; qq < 3; qq++)
+        if (qq >= nf) rj[qq] += c * gs * s[qq];
       if (matrix) {
         double tang = c / (P.u0 * (1 + z * z));
-        for (int qq = nf; qq < 3; qq++)
-          for (int p = nf; p < 3; p++) {
+#pragma unroll
+        for (int qq = 0; qq < 3; qq++)
+#pragma unroll
+          for (int p = 0; p < 3; p++) {
+            if (qq < nf || p < nf) continue;
             double kqp;
             if (newton && sl > 1e-12 * P.u0) kqp = c * gs * ((qq == p ? 1 : 0) - s[qq] * s[p] / s2) + tang * s[qq] * s[p] / s2;
             else kqp = qq == p ? c * gs : 0;
@@ -391,8 +438,14 @@ __global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P
   if (matrix) {
     const int pj = bsearch_lds(nbr, deg, j);
     if (have_fk && lane < 9 && pj >= 0) {
-      int qq = lane / 3, p = lane % 3;
-      acc[p * d3 + 3 * pj + qq] += fk[qq][p];
+      const int qq = lane / 3, p = lane % 3;
+      double f = 0.0;
+#pragma unroll
+      for (int a2 = 0; a2 < 3; a2++)
+#pragma unroll
+        for (int b2 = 0; b2 < 3; b2++)
+          if (lane == 3 * a2 + b2) f = fk[a2][b2];
+      acc[p * d3 + 3 * pj + qq] += f;
     }
     wave_sync_lds();
     double dmax = 0.0;
@@ -420,7 +473,60 @@ __global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P
     }
   }
   if (lane < 3) {
-    double rv = rj[lane];
+    double rv = lane == 0 ? rj[0] : (lane == 1 ? rj[1] : rj[2]);
+    if (mode == 1) rv = lane < nf ? -0.0 : -rv;
+    r[3 * j + lane] = rv;
+  }
+}
+
+// Residual-only assembly (line-search trials, final evaluation, start judging): one wave per node, the K_vol row dot
+// with one index load and one gather of v for the node's three columns, the elements around the node one per lane
+// (all their loads in flight) summed over the wave in a fixed order, friction, then energy, power and r as k_node
+// writes them for the same mode (0 raw residual, 1 BC + negated residual, 2 energy and power only).
+__global__ __launch_bounds__(256) void k_node_res(int nn, ifea_step_params P, int mode, const int* __restrict__ tets,
+                                                  const int* __restrict__ nts, const int* __restrict__ ntl,
+                                                  const int* __restrict__ colPtr, const int* __restrict__ rowIdx,
+                                                  const double* __restrict__ ax, const int* __restrict__ nfix,
+                                                  const int* __restrict__ contact, const double* __restrict__ afric,
+                                                  const double* __restrict__ kfric, const double* __restrict__ kvol,
+                                                  const double* __restrict__ v, const double* __restrict__ wm,
+                                                  const double* __restrict__ G, double* __restrict__ r,
+                                                  double* __restrict__ n_energy, double* __restrict__ n_power) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j = blockIdx.x * NODE_WAVES + wv;
+  if (j >= nn) return;
+  const int p0 = colPtr[3 * j], p1 = colPtr[3 * j + 1], p2 = colPtr[3 * j + 2];
+  const int d3 = p1 - p0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int q = lane; q < d3; q += 64) {
+    const double vv = v[rowIdx[p0 + q]];
+    s0 += kvol[p0 + q] * vv;
+    s1 += kvol[p1 + q] * vv;
+    s2 += kvol[p2 + q] * vv;
+  }
+  double rj[3] = {wsum(s0), wsum(s1), wsum(s2)};
+  double en = 0.5 * v[3 * j] * rj[0] + 0.5 * v[3 * j + 1] * rj[1] + 0.5 * v[3 * j + 2] * rj[2];
+  double pw = 2.0 * en;
+  const double* Rj = ax + 9 * (long long)j;
+  double c3[3] = {0.0, 0.0, 0.0};
+  for (int q = nts[j] + lane; q < nts[j + 1]; q += 64) {
+    const int e = ntl[q];
+    const int* t = tets + 4 * e;
+    const int b = t[0] == j ? 0 : (t[1] == j ? 1 : (t[2] == j ? 2 : 3));
+    const double w = wm[e];
+    const double* Gb = G + 12 * (long long)e + 3 * b;
+    const double reb0 = w * Gb[0], reb1 = w * Gb[1], reb2 = w * Gb[2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) c3[c] += reb0 * Rj[3 * c] + reb1 * Rj[3 * c + 1] + reb2 * Rj[3 * c + 2];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) rj[c] += wsum(c3[c]);
+  const int nf = nfix[j];
+  node_friction(P, j, nf, contact[j], Rj, v, afric, kfric, en, pw, rj);
+  if (lane == 0) { n_energy[j] = en; n_power[j] = pw; }
+  if (mode == 2) return;
+  if (lane < 3) {
+    double rv = lane == 0 ? rj[0] : (lane == 1 ? rj[1] : rj[2]);
     if (mode == 1) rv = lane < nf ? -0.0 : -rv;
     r[3 * j + lane] = rv;
   }
@@ -670,11 +776,15 @@ double StepDevice::assemble(bool matrix, bool newton, const double* d_v, int mod
                      grad_.p, vol_.p, eps_.p, e_wm_.p, e_wn_.p, e_g_.p, e_energy_.p, e_power_.p, el_sig_.p,
                      el_rate_.p, el_dev_.p);
   if (mode == 1) IFEA_HIP(hipMemsetAsync(maxdiag_.p, 0, sizeof(unsigned long long), s_));
-  hipLaunchKernelGGL(k_node, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), matrix ? node_lds_bytes(max_deg_) : 0, s_,
-                     max_deg_, nn_, P_, matrix ? 1 : 0,
-                     newton ? 1 : 0, mode, tets_.p, nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p, nbr_start_.p,
-                     nbr_.p, ax_.p, nfix_.p, contact_.p, afric_.p, kfric_.p, kvol_.p, d_v, grad_.p, e_wm_.p, e_wn_.p,
-                     e_g_.p, r_.p, val_.p, n_energy_.p, n_power_.p, maxdiag_.p);
+  if (matrix)
+    hipLaunchKernelGGL(k_node, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), node_lds_bytes(max_deg_), s_, max_deg_, nn_,
+                       P_, 1, newton ? 1 : 0, mode, tets_.p, nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p,
+                       nbr_start_.p, nbr_.p, ax_.p, nfix_.p, contact_.p, afric_.p, kfric_.p, kvol_.p, d_v, grad_.p,
+                       e_wm_.p, e_wn_.p, e_g_.p, r_.p, val_.p, n_energy_.p, n_power_.p, maxdiag_.p);
+  else
+    hipLaunchKernelGGL(k_node_res, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), 0, s_, nn_, P_, mode, tets_.p,
+                       nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p, ax_.p, nfix_.p, contact_.p, afric_.p, kfric_.p,
+                       kvol_.p, d_v, e_wm_.p, e_g_.p, r_.p, n_energy_.p, n_power_.p);
   if (matrix && mode == 1)
     hipLaunchKernelGGL(k_reg, dim3(grid_for(ndof_, 256)), dim3(256), 0, s_, nn_, nfix_.p, diagpos_.p, maxdiag_.p,
                        val_.p);
