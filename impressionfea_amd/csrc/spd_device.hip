@@ -358,8 +358,9 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 // chunks through a double-buffered LDS tile; the global loads of the next chunk are in flight while the current
 // chunk feeds the MFMAs, one barrier per chunk. Register use is kept at or below 128 VGPRs (one chunk of loads in
 // registers, 32-bit offsets from a wave-uniform column base) so that two 8-wave workgroups share a CU and one
-// workgroup's MFMAs run while the other waits at its barrier. task: (supernode s, row tile I, column tile J, 0),
-// I >= J. Fronts are below 2^27 rows (checked at analysis), so chunk offsets fit in 32 bits.
+// workgroup's MFMAs run while the other waits at its barrier; the tile is subtracted by atomic adds in L2 (one update
+// per entry per launch, so the result does not depend on their order). task: (supernode s, row tile I, column tile
+// J, 0), I >= J. Fronts are below 2^27 rows (checked at analysis), so chunk offsets fit in 32 bits.
 template <typename T, int NT, int KC>
 __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
                                                  T* __restrict__ L, T* __restrict__ U) {
@@ -454,8 +455,10 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
         long long j = j0 + jb + 16 * a + Mfma<T>::row(lane, r);
         long long i = i0 + ib + 16 * b + l16;
         if (i < m && j < m && i >= j) {
+          // each entry of the trailing triangle belongs to exactly one tile of the launch: a hardware atomic add in
+          // L2 gives the same bits as load, subtract and store, without the tile waiting for the load
           T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
-          *cp -= acc[a][b][r];
+          unsafeAtomicAdd(cp, -acc[a][b][r]);
         }
       }
 }

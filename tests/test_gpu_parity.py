@@ -149,6 +149,9 @@ def test_spd_wide_fronts(precision, nxy, nz, min_tail):
                 assert np.linalg.norm(xs - xr) <= 1e-12 * np.linalg.norm(xr)
         if precision == 2:
             assert h.stats()["refine_iterations"] >= 1
+        # deterministic: the same values factor and solve to the same bits
+        h.Factorize(A.data)
+        assert np.array_equal(h.Solve(b), xs)
 
 
 @pytest.mark.parametrize("precision", [0, 2])
