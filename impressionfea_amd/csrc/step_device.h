@@ -47,7 +47,7 @@ class StepDevice {
   bool have_params_ = false, prepared_ = false;
   int nn_, ne_, ndof_;
   long long nnz_;
-  int max_deg_ = 0, max_patch_ = 0;
+  int max_deg_ = 0, max_patch_ = 0, max_elem_ = 0;
   std::vector<int> user_perm_;
   std::vector<int> h_colPtr_, h_rowIdx_;
   std::vector<double> h_coords_;  // 3 per DOF, for the geometric ordering

@@ -23,6 +23,12 @@ constexpr int NODE_WAVES = 4;   // nodes per 256-thread block
 // dynamic LDS of the per-node assembly kernels for a mesh whose largest neighbour list has md nodes (sized to the
 // mesh, not to MAX_DEG, so that more workgroups share a CU)
 inline size_t node_lds_bytes(int md) { return (size_t)NODE_WAVES * md * (9 * sizeof(double) + sizeof(int)); }
+constexpr int TAN_WAVES = 2;    // nodes per workgroup of the tangent assembly
+// dynamic LDS of the tangent assembly: per wave 18 md + 26 me doubles and md + 9 me ints (md: largest neighbour list,
+// me: most elements around a node)
+inline size_t node_tan_lds_bytes(int md, int me) {
+  return (size_t)TAN_WAVES * ((18 * (size_t)md + 26 * (size_t)me) * sizeof(double) + ((size_t)md + 9 * (size_t)me) * sizeof(int));
+}
 
 __device__ __forceinline__ double law_stress(const ifea_step_params& P, double strain, double rate) {
   return P.C * pow(P.eps0 + fmax(strain, 0.0), P.n) * pow(fmax(rate, 1e-12), P.m);
@@ -310,37 +316,49 @@ __device__ __forceinline__ NodeFriction node_friction(const ifea_step_params& P,
   return f;
 }
 
-__global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P, int matrix, int newton, int mode,
-                                              const int* __restrict__ tets, const int* __restrict__ nts,
-                                              const int* __restrict__ ntl, const int* __restrict__ colPtr,
-                                              const int* __restrict__ rowIdx, const int* __restrict__ nbs,
-                                              const int* __restrict__ nb, const double* __restrict__ ax,
-                                              const int* __restrict__ nfix, const int* __restrict__ contact,
-                                              const double* __restrict__ afric, const double* __restrict__ kfric,
-                                              const double* __restrict__ kvol, const double* __restrict__ v,
-                                              const double* __restrict__ grad, const double* __restrict__ wm,
-                                              const double* __restrict__ wn, const double* __restrict__ G,
-                                              double* __restrict__ r, double* __restrict__ val,
-                                              double* __restrict__ n_energy, double* __restrict__ n_power,
-                                              unsigned long long* __restrict__ maxdiag) {
-  // with matrix set: node_lds_bytes(md) of dynamic LDS (per wave 9 md accumulators, then md neighbour ids); the
-  // residual-only assemblies touch no LDS and are launched without it
+// Tangent assembly (with the residual, energy and power of the same call): one wave per node j, TAN_WAVES nodes per
+// workgroup, node_tan_lds_bytes(md, me) of dynamic LDS (per wave: the three CSC columns of j as 3x3 blocks, the frames
+// of j's neighbours, and the data of the elements around j). Everything an element contributes is staged first with
+// all loads in flight (one element per lane), so the element loop, which accumulates in ascending element order,
+// reads LDS only.
+__global__ __launch_bounds__(64 * TAN_WAVES) void k_node(int md, int me, int nn, ifea_step_params P, int matrix,
+                                                         int newton, int mode, const int* __restrict__ tets,
+                                                         const int* __restrict__ nts, const int* __restrict__ ntl,
+                                                         const int* __restrict__ colPtr,
+                                                         const int* __restrict__ rowIdx, const int* __restrict__ nbs,
+                                                         const int* __restrict__ nb, const double* __restrict__ ax,
+                                                         const int* __restrict__ nfix, const int* __restrict__ contact,
+                                                         const double* __restrict__ afric,
+                                                         const double* __restrict__ kfric,
+                                                         const double* __restrict__ kvol, const double* __restrict__ v,
+                                                         const double* __restrict__ grad,
+                                                         const double* __restrict__ wm, const double* __restrict__ wn,
+                                                         const double* __restrict__ G, double* __restrict__ r,
+                                                         double* __restrict__ val, double* __restrict__ n_energy,
+                                                         double* __restrict__ n_power,
+                                                         unsigned long long* __restrict__ maxdiag) {
   extern __shared__ double node_lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int j = blockIdx.x * NODE_WAVES + wv;
+  const int j = blockIdx.x * TAN_WAVES + wv;
   if (j >= nn) return;
   const int deg = nbs[j + 1] - nbs[j];
   const int d3 = 3 * deg;
-  double* acc = node_lds + (long long)wv * 9 * md;
-  int* nbr = (int*)(node_lds + (long long)NODE_WAVES * 9 * md) + wv * md;
-  if (matrix) {
-    for (int q = lane; q < deg; q += 64) nbr[q] = nb[nbs[j] + q];
-    for (int c = 0; c < 3; c++) {
-      const double* src = kvol + colPtr[3 * j + c];
-      for (int q = lane; q < d3; q += 64) acc[c * d3 + q] = src[q];
-    }
-    wave_sync_lds();
+  const int q0 = nts[j], nej = nts[j + 1] - q0;  // elements around j, nej <= me
+  double* acc = node_lds + (long long)wv * (18 * md + 26 * me);  // 9 md: the node's columns, [c][3 pos + r]
+  double* axn = acc + 9 * md;                                     // 9 md: frames of the neighbours
+  double* eg = axn + 9 * md;                                      // 12 me: element gradients
+  double* eG = eg + 12 * me;                                      // 12 me: element G vectors
+  double* ew = eG + 12 * me;                                      // me: wm
+  double* ewn = ew + me;                                          // me: wn
+  int* nbr = (int*)(node_lds + (long long)TAN_WAVES * (18 * md + 26 * me)) + wv * (md + 9 * me);
+  int* et = nbr + md;                                             // 4 me: neighbour positions of the element's nodes
+  int* eb = et + 4 * me;                                          // me: local index of j in the element
+  for (int q = lane; q < deg; q += 64) nbr[q] = nb[nbs[j] + q];
+  for (int c = 0; c < 3; c++) {
+    const double* src = kvol + colPtr[3 * j + c];
+    for (int q = lane; q < d3; q += 64) acc[c * d3 + q] = src[q];
   }
+  for (int q = lane; q < 9 * deg; q += 64) axn[q] = ax[9 * (long long)nb[nbs[j] + q / 9] + q % 9];
   // r_vol = K_vol v (row j = column j, symmetric); energy 1/2 v.r_vol
   double rj[3];
   for (int c = 0; c < 3; c++) {
@@ -351,22 +369,57 @@ __global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P
   double en = 0.5 * v[3 * j] * rj[0] + 0.5 * v[3 * j + 1] * rj[1] + 0.5 * v[3 * j + 2] * rj[2];
   double pw = 2.0 * en;
   const double* Rj = ax + 9 * (long long)j;
-  const int la = lane / 9, lr = (lane / 3) % 3, lc = lane % 3;
-  // elements around j in ascending order
-  for (int q = nts[j]; q < nts[j + 1]; q++) {
-    const int e = ntl[q];
-    const int* t = tets + 4 * e;
-    const int b = t[0] == j ? 0 : (t[1] == j ? 1 : (t[2] == j ? 2 : 3));
+  double Rjv[9];
+#pragma unroll
+  for (int u = 0; u < 9; u++) Rjv[u] = Rj[u];
+  wave_sync_lds();  // nbr complete before the searches below
+  // stage the elements, one per lane, and add their residual terms ToLocal(j, wm G_b) (wave sum in a fixed order)
+  double c3[3] = {0.0, 0.0, 0.0};
+  for (int q = lane; q < nej; q += 64) {
+    const int e = ntl[q0 + q];
+    int tt[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) tt[a] = tets[4 * e + a];
+    const int b = tt[0] == j ? 0 : (tt[1] == j ? 1 : (tt[2] == j ? 2 : 3));
     const double w = wm[e];
+    eb[q] = b;
+    ew[q] = w;
+    ewn[q] = wn[e];
+    const double* ge = grad + 12 * (long long)e;
     const double* Ge = G + 12 * (long long)e;
-    // residual: ToLocal(j, wm * G_b)
-    double reb0 = w * Ge[3 * b], reb1 = w * Ge[3 * b + 1], reb2 = w * Ge[3 * b + 2];
-    for (int c = 0; c < 3; c++) rj[c] += reb0 * Rj[3 * c] + reb1 * Rj[3 * c + 1] + reb2 * Rj[3 * c + 2];
-    if (matrix && lane < 36) {
-      const int i = t[la];
-      const double* ga = grad + 12 * (long long)e + 3 * la;
-      const double* gb = grad + 12 * (long long)e + 3 * b;
-      const double wnn = wn[e];
+    double gv[12], Gv[12];
+#pragma unroll
+    for (int u = 0; u < 12; u++) {
+      gv[u] = ge[u];
+      Gv[u] = Ge[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 12; u++) {
+      eg[12 * q + u] = gv[u];
+      eG[12 * q + u] = Gv[u];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) et[4 * q + a] = bsearch_lds(nbr, deg, tt[a]);
+    const double reb0 = w * Gv[3 * b], reb1 = w * Gv[3 * b + 1], reb2 = w * Gv[3 * b + 2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) c3[c] += reb0 * Rjv[3 * c] + reb1 * Rjv[3 * c + 1] + reb2 * Rjv[3 * c + 2];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) rj[c] += wsum(c3[c]);
+  wave_sync_lds();
+  // elements around j in ascending order, 36 lanes per element for the 4 x 3 x 3 block entries
+  const int la = lane / 9, lr = (lane / 3) % 3, lc = lane % 3;
+  const double rl0 = lc == 0 ? Rjv[0] : (lc == 1 ? Rjv[3] : Rjv[6]);
+  const double rl1 = lc == 0 ? Rjv[1] : (lc == 1 ? Rjv[4] : Rjv[7]);
+  const double rl2 = lc == 0 ? Rjv[2] : (lc == 1 ? Rjv[5] : Rjv[8]);
+  for (int q = 0; q < nej; q++) {
+    const int pos = lane < 36 ? et[4 * q + la] : -1;
+    if (pos >= 0) {
+      const int b = eb[q];
+      const double w = ew[q], wnn = ewn[q];
+      const double* ga = eg + 12 * q + 3 * la;
+      const double* gb = eg + 12 * q + 3 * b;
+      const double* Gq = eG + 12 * q;
       const double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
       // Ke_ab[c][d] = wm * (B^T M B)_{3a+c, 3b+d} + wn * G_{3a+c} G_{3b+d}
       double K[3][3];
@@ -377,20 +430,19 @@ __global__ __launch_bounds__(256) void k_node(int md, int nn, ifea_step_params P
           double dl = c == d ? 1.0 : 0.0;
           double bmb = 2.0 / 3.0 * ga[c] * gb[d] * (dl - 1.0 / 3.0) +
                        (dl * (gab - 2.0 * ga[c] * gb[c]) + ga[d] * gb[c]) / 3.0;
-          K[c][d] = w * bmb + wnn * Ge[3 * la + c] * Ge[3 * b + d];
+          K[c][d] = w * bmb + wnn * Gq[3 * la + c] * Gq[3 * b + d];
         }
-      const double* Ri = ax + 9 * (long long)i;
+      const double* Ri = axn + 9 * pos;
       // blk[r][c'] = sum_c sum_d Ri[r][c] K[c][d] Rj[c'][d]
       double sum = 0.0;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        double kr = K[c][0] * Rj[3 * lc] + K[c][1] * Rj[3 * lc + 1] + K[c][2] * Rj[3 * lc + 2];
+        double kr = K[c][0] * rl0 + K[c][1] * rl1 + K[c][2] * rl2;
         sum += Ri[3 * lr + c] * kr;
       }
-      const int pos = bsearch_lds(nbr, deg, i);
-      if (pos >= 0) acc[lc * d3 + 3 * pos + lr] += sum;
+      acc[lc * d3 + 3 * pos + lr] += sum;
     }
-    if (matrix) wave_sync_lds();
+    wave_sync_lds();
   }
   // friction on the free axes of contact nodes (:1175-1214)
   const int nf = nfix[j];
@@ -686,6 +738,14 @@ StepDevice::StepDevice(int nn, int ne, const int* tets, const int* colPtr, const
     for (int f = 0; f < nf; f++) for (int a = 0; a < 3; a++) nfl[fill[faces[3 * f + a]]++] = f;
   }
   tets_.upload(tets, 4 * (size_t)ne, s_);
+  for (int n = 0; n < nn; n++) max_elem_ = std::max(max_elem_, nts[n + 1] - nts[n]);
+  {
+    // the tangent assembly keeps a node's columns, its neighbours' frames and its elements in LDS
+    const size_t lds = node_tan_lds_bytes(max_deg_, max_elem_);
+    if (lds > 160 * 1024) throw Error(-1, "a node has too many neighbours and elements for the assembly kernel");
+    if (lds > 64 * 1024)
+      IFEA_HIP(hipFuncSetAttribute((const void*)k_node, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   nt_start_.upload(nts, s_);
   nt_list_.upload(ntl, s_);
   colPtr_.upload(colPtr, ndof_ + 1, s_);
@@ -777,10 +837,11 @@ double StepDevice::assemble(bool matrix, bool newton, const double* d_v, int mod
                      el_rate_.p, el_dev_.p);
   if (mode == 1) IFEA_HIP(hipMemsetAsync(maxdiag_.p, 0, sizeof(unsigned long long), s_));
   if (matrix)
-    hipLaunchKernelGGL(k_node, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), node_lds_bytes(max_deg_), s_, max_deg_, nn_,
-                       P_, 1, newton ? 1 : 0, mode, tets_.p, nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p,
-                       nbr_start_.p, nbr_.p, ax_.p, nfix_.p, contact_.p, afric_.p, kfric_.p, kvol_.p, d_v, grad_.p,
-                       e_wm_.p, e_wn_.p, e_g_.p, r_.p, val_.p, n_energy_.p, n_power_.p, maxdiag_.p);
+    hipLaunchKernelGGL(k_node, dim3(grid_for(nn_, TAN_WAVES)), dim3(64 * TAN_WAVES),
+                       node_tan_lds_bytes(max_deg_, max_elem_), s_, max_deg_, max_elem_, nn_, P_, 1, newton ? 1 : 0,
+                       mode, tets_.p, nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p, nbr_start_.p, nbr_.p, ax_.p,
+                       nfix_.p, contact_.p, afric_.p, kfric_.p, kvol_.p, d_v, grad_.p, e_wm_.p, e_wn_.p, e_g_.p, r_.p,
+                       val_.p, n_energy_.p, n_power_.p, maxdiag_.p);
   else
     hipLaunchKernelGGL(k_node_res, dim3(grid_for(nn_, NODE_WAVES)), dim3(256), 0, s_, nn_, P_, mode, tets_.p,
                        nt_start_.p, nt_list_.p, colPtr_.p, rowIdx_.p, ax_.p, nfix_.p, contact_.p, afric_.p, kfric_.p,
