@@ -64,8 +64,8 @@ class SpdDevice {
   struct Profile {
     long long syrk_launches = 0, solve_launches = 0, factorizations = 0, solves = 0;
     long long outer_launches = 0, f32_factorizations = 0, f32_solves = 0;
-    double syrk_ms = 0, syrk_flops = 0;     // inner (64-column) updates; algorithmic flops 2 * width per entry
-    double outer_ms = 0, outer_flops = 0;   // outer (256-column) updates, k_syrk_big
+    double syrk_ms = 0, syrk_flops = 0;     // inner (64-column) and mid (256-column) updates; 2 * width per entry
+    double outer_ms = 0, outer_flops = 0;   // outer (512-column) updates, k_syrk_big
     long long outer_launches_f32 = 0;       // the part of the outer updates done on an FP32 factor
     double outer_ms_f32 = 0, outer_flops_f32 = 0;
     double solve_ms = 0, solve_bytes = 0;   // one forward + backward sweep reads every panel once
@@ -104,7 +104,9 @@ class SpdDevice {
     std::vector<LaunchList> panel_steps, syrk_steps, potrf_steps;
     long long sn_begin = 0, sn_end = 0;  // into level_sn_
     std::vector<double> syrk_flops;      // algorithmic flops per syrk step
-    std::vector<LaunchList> outer_steps; // outer (256-column) updates, launched after the matching panel step
+    std::vector<LaunchList> mid_steps;   // mid updates (second inner block of an outer block), after the panel step
+    std::vector<double> mid_flops;
+    std::vector<LaunchList> outer_steps; // outer (512-column) updates, launched after the matching panel step
     std::vector<double> outer_flops;
     // triangular sweeps: narrow fronts one workgroup each; wide fronts blocked over several launches
     long long small_begin = 0, small_end = 0, big_begin = 0, big_end = 0;  // into sweep_sn_

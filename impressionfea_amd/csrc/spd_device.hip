@@ -18,7 +18,9 @@
 namespace ifea {
 
 constexpr int NB = 64;          // panel width
-constexpr int OB = 256;         // outer block: the trailing matrix beyond it is updated once per OB columns
+constexpr int OB = 256;         // inner block: the 64-column panel updates stay inside it
+constexpr int OB2 = 512;        // outer block: the trailing matrix beyond it is updated once per OB2 columns; inside
+                                // it, a mid update of OB columns brings its second half up to date
 constexpr int EXT_TARGET = 8192;
 constexpr long long ZERO_TARGET = 1 << 16;  // update-block entries zeroed per workgroup
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
@@ -285,22 +287,20 @@ __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ task
       }
 }
 
-// Trailing update C -= X X^T of a front, 64x64 output tile per workgroup, MFMA (FP64 16x16x4 or FP32 16x16x4).
+// Inner trailing update C -= X X^T of a front, 64x64 output tile per workgroup, MFMA (FP64 16x16x4 or FP32
+// 16x16x4): X = the 64 columns [kb, kb+w) just factored; C = columns [kb+w, min(k, inner_end)) of the same OB-column
+// inner block, all rows below (delayed right-looking update inside the inner block).
 // task: (supernode s, row tile I, column tile J, 0), I >= J.
-// mode 0 (inner): X = the 64 columns [kb, kb+w) just factored; C = columns [kb+w, min(k, outer_end)) of the same
-//                 outer block, all rows below (delayed right-looking update inside a 256-column outer block).
-// mode 1 (outer): X = the outer block [kb, kb+min(OB, k-kb)); C = every column to its right, including the update
-//                 block U, so the large trailing matrix is read and written once per 256 columns.
 template <typename T>
-__global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, int kb, int mode, SymDev sd,
+__global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, int kb, SymDev sd,
                                               T* __restrict__ L, T* __restrict__ U) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
-  const int kw = mode == 0 ? min(NB, k - kb) : min(OB, k - kb);
-  const long long rb = kb + kw;
-  const long long col_end = mode == 0 ? min((long long)k, (long long)(kb / OB) * OB + OB) : m;
+  const int w = min(NB, k - kb);
+  const long long rb = kb + w;
+  const long long col_end = min((long long)k, (long long)(kb / OB) * OB + OB);
   const T* P = L + sd.Loff[s];
   __shared__ T Xi[NB][64 + 4];
   __shared__ T Xj[NB][64 + 4];
@@ -310,27 +310,23 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
   const int l16 = lane & 15;
   typedef typename Mfma<T>::v4 v4;
   v4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
-  for (int kc = 0; kc < kw; kc += NB) {
-    const int w = min(NB, kw - kc);
-    if (kc) __syncthreads();
-    for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
-      int q = idx >> 6, rr = idx & 63;
-      long long ri = i0 + rr, rj = j0 + rr;
-      const T* cp = P + (long long)(kb + kc + q) * m;
-      Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
-      Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
-    }
-    __syncthreads();
-    const int kq = (w + 3) & ~3;
-    for (int kk = 0; kk < kq; kk += 4) {
-      const int q = kk + (lane >> 4);
-      T a0 = Xj[q][jb + l16], a1 = Xj[q][jb + 16 + l16];
-      T b0 = Xi[q][ib + l16], b1 = Xi[q][ib + 16 + l16];
-      acc00 = Mfma<T>::op(a0, b0, acc00);
-      acc01 = Mfma<T>::op(a0, b1, acc01);
-      acc10 = Mfma<T>::op(a1, b0, acc10);
-      acc11 = Mfma<T>::op(a1, b1, acc11);
-    }
+  for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
+    int q = idx >> 6, rr = idx & 63;
+    long long ri = i0 + rr, rj = j0 + rr;
+    const T* cp = P + (long long)(kb + q) * m;
+    Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
+    Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
+  }
+  __syncthreads();
+  const int kq = (w + 3) & ~3;
+  for (int kk = 0; kk < kq; kk += 4) {
+    const int q = kk + (lane >> 4);
+    T a0 = Xj[q][jb + l16], a1 = Xj[q][jb + 16 + l16];
+    T b0 = Xi[q][ib + l16], b1 = Xi[q][ib + 16 + l16];
+    acc00 = Mfma<T>::op(a0, b0, acc00);
+    acc01 = Mfma<T>::op(a0, b1, acc01);
+    acc10 = Mfma<T>::op(a1, b0, acc10);
+    acc11 = Mfma<T>::op(a1, b1, acc11);
   }
   // D[a][b] = sum_q Xj[a][q] Xi[b][q] = contribution to C(i = i0+ib+16bi+b, j = j0+jb+16ai+a)
   v4 accs[2][2] = {{acc00, acc01}, {acc10, acc11}};
@@ -353,7 +349,7 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 }
 
 
-// Outer trailing update with 128x128 output tiles (mode 1 of k_syrk): NT/64 waves, two along the rows (64 each) and
+// Outer and mid trailing updates with 128x128 output tiles: NT/64 waves, two along the rows (64 each) and
 // NT/128 along the columns, each wave a block of 16x16 MFMA tiles (64x32 with 8 waves). X is streamed in 16-column
 // chunks through a double-buffered LDS tile; the global loads of the next chunk are in flight while the current
 // chunk feeds the MFMAs, one barrier per chunk. Register use is kept at or below 128 VGPRs (one chunk of loads in
@@ -361,7 +357,10 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 // workgroup's MFMAs run while the other waits at its barrier; the tile is subtracted by atomic adds in L2 (one update
 // per entry per launch, so the result does not depend on their order). task: (supernode s, row tile I, column tile
 // J, 0), I >= J. Fronts are below 2^27 rows (checked at analysis), so chunk offsets fit in 32 bits.
-template <typename T, int NT, int KC>
+// MID = false (outer): X = the outer block [kb, kb + min(OB2, k - kb)), C = every column to its right (L and U).
+// MID = true: X = the first OB columns [kb, kb + OB) of an outer block, C = its second half [kb + OB, min(k, kb +
+// OB2)), all rows below; the panels of the second half are then factored against a complete update.
+template <typename T, int NT, int KC, bool MID>
 __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
                                                  T* __restrict__ L, T* __restrict__ U) {
   constexpr int PER = KC * 128 / NT;   // elements of each operand a thread stages per chunk
@@ -376,8 +375,9 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
   const int s = tk.x;
   const int k = sn_k(sd, s);
   const int m = (int)sn_m(sd, s);
-  const int kw = min(OB, k - kb);
+  const int kw = min(MID ? OB : OB2, k - kb);
   const int rb = kb + kw;
+  const int cend = MID ? min(k, kb + OB2) : m;      // columns updated: [rb, cend)
   const T* P = L + sd.Loff[s] + (long long)kb * m;  // column kb of the front
   __shared__ T Xi[2][KC][LD];
   __shared__ T Xj[2][KC][LD];
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
       for (int r = 0; r < 4; r++) {
         long long j = j0 + jb + 16 * a + Mfma<T>::row(lane, r);
         long long i = i0 + ib + 16 * b + l16;
-        if (i < m && j < m && i >= j) {
+        if (i < m && j < cend && i >= j) {
           // each entry of the trailing triangle belongs to exactly one tile of the launch: a hardware atomic add in
           // L2 gives the same bits as load, subtract and store, without the tile waiting for the load
           T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
@@ -1204,21 +1204,23 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       ll.end = (long long)ext.size();
       if (ll.end > ll.begin) L.ext_rounds.push_back(ll);
     }
-    // factor steps: 64-column panels; inner updates stay inside the 256-column outer block, the outer update
-    // of everything to its right follows the outer block's last panel
+    // factor steps: 64-column panels; inner updates stay inside the OB-column inner block; after the first inner
+    // block of an OB2-column outer block, a mid update brings the second one up to date; the outer update of
+    // everything to its right follows the outer block's last panel
     int maxk = 0;
     for (int s : bylevel[l]) maxk = std::max(maxk, S.sn_first[s + 1] - S.sn_first[s]);
     for (int c0 = 0; c0 < maxk; c0 += NB) {
-      LaunchList lp, ls, lo, lpo;
+      LaunchList lp, ls, lm, lo, lpo;
       lp.begin = (long long)pan.size();
       lpo.begin = (long long)pot.size();
       lpo.c0 = c0;
       ls.begin = (long long)syr.size();
       lp.c0 = ls.c0 = c0;
-      double fl = 0, flo = 0;
+      double fl = 0, flm = 0, flo = 0;
       int nslot = 0;
-      const int ob0 = (c0 / OB) * OB;
-      const bool outer_now = (c0 + NB) % OB == 0 || c0 + NB >= maxk;
+      const int ib0 = (c0 / OB) * OB, ob0 = (c0 / OB2) * OB2;
+      const bool outer_now = (c0 + NB) % OB2 == 0 || c0 + NB >= maxk;
+      const bool mid_now = (c0 + NB) % OB2 == OB;
       for (int s : bylevel[l]) {
         int k = S.sn_first[s + 1] - S.sn_first[s];
         if (k <= c0) continue;
@@ -1230,8 +1232,8 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
         for (long long r0 = rb; r0 < m; r0 += 64)
           pan.push_back(make_int4(s, (int)r0, (int)std::min<long long>(m, r0 + 64), nslot));
         nslot++;
-        // inner update: columns [rb, min(k, ob0 + OB))
-        long long cend = std::min<long long>(k, ob0 + OB);
+        // inner update: columns [rb, min(k, ib0 + OB))
+        long long cend = std::min<long long>(k, ib0 + OB);
         if (cend > rb) {
           long long ti = (nrows + 63) / 64, tj = (cend - rb + 63) / 64;
           for (long long I = 0; I < ti; I++)
@@ -1246,17 +1248,35 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       L.potrf_steps.push_back(lpo);
       L.syrk_steps.push_back(ls);
       L.syrk_flops.push_back(fl);
+      lm.begin = (long long)syr.size();
+      lm.c0 = ob0;
+      if (mid_now) {
+        for (int s : bylevel[l]) {
+          // a front with columns beyond the first inner block has just factored all of its panels in it
+          int k = S.sn_first[s + 1] - S.sn_first[s];
+          if (k <= c0 + NB) continue;
+          long long m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+          long long rb = c0 + NB, cend = std::min<long long>(k, ob0 + OB2);
+          long long ti = (m - rb + 127) / 128, tj = (cend - rb + 127) / 128;
+          for (long long I = 0; I < ti; I++)
+            for (long long J = 0; J <= std::min(I, tj - 1); J++) syr.push_back(make_int4(s, (int)I, (int)J, 0));
+          for (long long j = rb; j < cend; j++) flm += 2.0 * OB * (double)(m - j);
+        }
+      }
+      lm.end = (long long)syr.size();
+      L.mid_steps.push_back(lm);
+      L.mid_flops.push_back(flm);
       lo.begin = (long long)syr.size();
       lo.c0 = ob0;
       if (outer_now) {
         for (int s : bylevel[l]) {
           int k = S.sn_first[s + 1] - S.sn_first[s];
           if (k <= ob0) continue;
-          // the outer update of this front happens once, after its last panel inside [ob0, ob0 + OB)
-          int last_c0 = ((std::min(k, ob0 + OB) - 1) / NB) * NB;
+          // the outer update of this front happens once, after its last panel inside [ob0, ob0 + OB2)
+          int last_c0 = ((std::min(k, ob0 + OB2) - 1) / NB) * NB;
           if (last_c0 > c0) continue;
           long long m = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
-          int kw = std::min(OB, k - ob0);
+          int kw = std::min(OB2, k - ob0);
           long long rb = ob0 + kw;
           long long nrows = m - rb;
           if (nrows <= 0) continue;
@@ -1472,23 +1492,28 @@ int SpdDevice::factorize_t(const double* d_vals) {
       if (p.end > p.begin)
         hipLaunchKernelGGL(k_trsm_mfma<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
                            panel_tasks_.p + p.begin, p.c0, sd_, L, Linv_.p);
-      for (int mode = 0; mode < 2; mode++) {
-        const LaunchList& q = mode == 0 ? lv.syrk_steps[t] : lv.outer_steps[t];
+      // inner (0), mid (1) and outer (2) updates; the profile counts inner and mid updates together
+      for (int mode = 0; mode < 3; mode++) {
+        const LaunchList& q = mode == 0 ? lv.syrk_steps[t] : mode == 1 ? lv.mid_steps[t] : lv.outer_steps[t];
         if (q.end <= q.begin) continue;
         if (profiling_) IFEA_HIP(hipEventRecord(event(2 * nev), stream_));
         if (mode == 0)
           hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
-                             syrk_tasks_.p + q.begin, q.c0, mode, sd_, L, U);
-        else
-          hipLaunchKernelGGL((k_syrk_big<T, 512, 16>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, stream_,
                              syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+        else if (mode == 1)
+          hipLaunchKernelGGL((k_syrk_big<T, 512, 16, true>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0,
+                             stream_, syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+        else
+          hipLaunchKernelGGL((k_syrk_big<T, 512, 16, false>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0,
+                             stream_, syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
         if (profiling_) {
           IFEA_HIP(hipEventRecord(event(2 * nev + 1), stream_));
           nev++;
-          ev_mode_.push_back((char)mode);
-          if (mode == 0) prof_.syrk_flops += lv.syrk_flops[t];
-          else prof_.outer_flops += lv.outer_flops[t];
-          if (mode == 1 && sizeof(T) == 4) prof_.outer_flops_f32 += lv.outer_flops[t];
+          const bool outer = mode == 2;
+          ev_mode_.push_back((char)outer);
+          if (!outer) prof_.syrk_flops += mode == 0 ? lv.syrk_flops[t] : lv.mid_flops[t];
+          if (outer) prof_.outer_flops += lv.outer_flops[t];
+          if (outer && sizeof(T) == 4) prof_.outer_flops_f32 += lv.outer_flops[t];
         }
       }
     }
