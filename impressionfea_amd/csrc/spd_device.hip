@@ -436,11 +436,14 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
   __syncthreads();
   for (int c = 0; c < nch; c++) {
     const int buf = c & 1;
-    compute(buf);                 // chunk c
+    // stage chunk c + 1 (its buffer was last read before the previous barrier) and put the loads of chunk c + 2 in
+    // flight before the MFMAs of chunk c: the wait for the loads comes right after a barrier, while the other
+    // workgroup on the CU keeps the MFMA pipes busy
     if (c + 1 < nch) {
-      stage(buf ^ 1);             // chunk c + 1 (its buffer was last read before the previous barrier)
+      stage(buf ^ 1);
       if (c + 2 < nch) fetch(c + 2);
     }
+    compute(buf);
     __syncthreads();
   }
   T* Ls = L + sd.Loff[s];
