@@ -21,6 +21,7 @@ constexpr int NB = 64;          // panel width
 constexpr int OB = 256;         // inner block: the 64-column panel updates stay inside it
 constexpr int OB2 = 512;        // outer block: the trailing matrix beyond it is updated once per OB2 columns; inside
                                 // it, a mid update of OB columns brings its second half up to date
+constexpr long long OUTER_GROUP = 8;  // outer-update tiles are listed in groups of this many tile rows and columns
 constexpr int EXT_TARGET = 8192;
 constexpr long long ZERO_TARGET = 1 << 16;  // update-block entries zeroed per workgroup
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
@@ -1283,9 +1284,14 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
           long long rb = ob0 + kw;
           long long nrows = m - rb;
           if (nrows <= 0) continue;
+          // tiles in OUTER_GROUP x OUTER_GROUP groups: an XCD runs a contiguous range of the list, so the tiles
+          // resident on it at once read few distinct row panels, which then stay in its L2
           long long tiles = (nrows + 127) / 128;
-          for (long long I = 0; I < tiles; I++)
-            for (long long J = 0; J <= I; J++) syr.push_back(make_int4(s, (int)I, (int)J, 0));
+          for (long long IB = 0; IB < tiles; IB += OUTER_GROUP)
+            for (long long JB = 0; JB <= IB; JB += OUTER_GROUP)
+              for (long long I = IB; I < std::min(tiles, IB + OUTER_GROUP); I++)
+                for (long long J = JB; J <= std::min(I, JB + OUTER_GROUP - 1); J++)
+                  syr.push_back(make_int4(s, (int)I, (int)J, 0));
           flo += (double)kw * (double)nrows * (double)(nrows + 1);
         }
       }
