@@ -92,15 +92,15 @@ class SpdDevice {
   DBuf<int> amap_src_;
   DBuf<long long> amap_dst_;
   // task arrays (int4: a, b, c, d)
-  DBuf<int4> ext_tasks_, panel_tasks_, syrk_tasks_, zero_tasks_, potrf_tasks_;
+  DBuf<int4> ext_tasks_, panel_tasks_, syrk_tasks_, potrf_tasks_;
   DBuf<int> level_sn_, sweep_sn_;
   DBuf<int4> fwd_tasks_, bwd_tasks_, inv_tasks_;
   DBuf<long long> Xoff_, Goff_, gptr_, gsrc_;
   LaunchList inv_launch_;
   // launch schedule
   struct Level {
-    std::vector<LaunchList> ext_rounds;
-    LaunchList zero;
+    std::vector<LaunchList> ext_rounds;    // children's columns that land in the fronts' own columns (before)
+    std::vector<LaunchList> ext_rounds_u;  // children's columns that land in the fronts' update blocks (after)
     std::vector<LaunchList> panel_steps, syrk_steps, potrf_steps;
     long long sn_begin = 0, sn_end = 0;  // into level_sn_
     std::vector<double> syrk_flops;      // algorithmic flops per syrk step

@@ -23,7 +23,6 @@ constexpr int OB2 = 512;        // outer block: the trailing matrix beyond it is
                                 // it, a mid update of OB columns brings its second half up to date
 constexpr long long OUTER_GROUP = 8;  // outer-update tiles are listed in groups of this many tile rows and columns
 constexpr int EXT_TARGET = 8192;
-constexpr long long ZERO_TARGET = 1 << 16;  // update-block entries zeroed per workgroup
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
 constexpr int SB = 2048;        // diagonal block of the blocked sweeps (explicit inverse per block)
@@ -84,20 +83,6 @@ __global__ void k_scatter(const double* __restrict__ vals, const int* __restrict
                           long long cnt, T* __restrict__ L) {
   long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (k < cnt) L[dst[k]] = (T)vals[src[k]];
-}
-
-// task: (supernode s, first column, end column, 0): zero the lower triangle (rows >= column) of those columns of the
-// supernode's update block before its level starts; nothing reads the strict upper triangle of an update block.
-template <typename T>
-__global__ __launch_bounds__(256) void k_zero_u(const int4* __restrict__ tasks, SymDev sd, T* __restrict__ U) {
-  const int4 tk = tasks[blockIdx.x];
-  const int s = tk.x;
-  const long long mu = sn_m(sd, s) - sn_k(sd, s);
-  T* u = U + sd.Uoff[s];
-  for (int j = tk.y; j < tk.z; j++) {
-    T* col = u + (long long)j * mu;
-    for (long long i = j + threadIdx.x; i < mu; i += blockDim.x) col[i] = T(0);
-  }
 }
 
 // task: (child c, first column, end column, parent p) of the child's update block
@@ -460,9 +445,14 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
         long long i = i0 + ib + 16 * b + l16;
         if (i < m && j < cend && i >= j) {
           // each entry of the trailing triangle belongs to exactly one tile of the launch: a hardware atomic add in
-          // L2 gives the same bits as load, subtract and store, without the tile waiting for the load
-          T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
-          unsafeAtomicAdd(cp, -acc[a][b][r]);
+          // L2 gives the same bits as load, subtract and store, without the tile waiting for the load. The first
+          // outer update of a front (kb = 0) is the first write to its update block: a store.
+          if (!MID && kb == 0 && j >= k) {
+            Us[(j - k) * mu + (i - k)] = -acc[a][b][r];
+          } else {
+            T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
+            unsafeAtomicAdd(cp, -acc[a][b][r]);
+          }
         }
       }
 }
@@ -1125,7 +1115,7 @@ hipEvent_t SpdDevice::event(size_t i) {
 
 size_t SpdDevice::device_bytes() const {
   return L_.bytes() + U_.bytes() + V_.bytes() + W_.bytes() + Pr_.bytes() + amap_dst_.bytes() + amap_src_.bytes() +
-         sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() + zero_tasks_.bytes() +
+         sn_rows_.bytes() + ext_tasks_.bytes() + panel_tasks_.bytes() + syrk_tasks_.bytes() +
          Linv_.bytes() + X_.bytes() + XT_.bytes() + X32_.bytes() + XT32_.bytes() + gptr_.bytes() + gsrc_.bytes() +
          fwd_tasks_.bytes() + bwd_tasks_.bytes();
 }
@@ -1162,7 +1152,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   std::vector<std::vector<int>> bylevel(nl);
   for (int s = 0; s < ns; s++) bylevel[S.sn_level[s]].push_back(s);
   std::vector<int> lvl_sn;
-  std::vector<int4> ext, pan, syr, zer, pot;
+  std::vector<int4> ext, pan, syr, pot;
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
     L.sn_begin = (long long)lvl_sn.size();
@@ -1173,41 +1163,36 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
     L.sn_end = (long long)lvl_sn.size();
     for (int s : bylevel[l])
       L.solve_bytes += (double)(S.sn_rowptr[s + 1] - S.sn_rowptr[s]) * (S.sn_first[s + 1] - S.sn_first[s]);
-    L.zero.begin = (long long)zer.size();
-    for (int s : bylevel[l]) {
-      long long mu = (S.sn_rowptr[s + 1] - S.sn_rowptr[s]) - (S.sn_first[s + 1] - S.sn_first[s]);
-      long long j = 0;
-      while (j < mu) {
-        long long j1 = j, work = 0;
-        while (j1 < mu && work < ZERO_TARGET) work += mu - j1++;
-        zer.push_back(make_int4(s, (int)j, (int)j1, 0));
-        j = j1;
-      }
-    }
-    L.zero.end = (long long)zer.size();
-    // extend-add rounds: round r = r-th child of every parent at this level
+    // extend-add rounds: round r = r-th child of every parent at this level. A child's update columns that land in
+    // the parent's own columns are added before the parent is factored; those that land in the parent's update block
+    // are added after it: the parent's first outer update stores (not adds) into every entry of its update block, so
+    // the block is never cleared
     int maxc = 0;
     for (int s : bylevel[l]) maxc = std::max(maxc, cptr[s + 1] - cptr[s]);
-    for (int r = 0; r < maxc; r++) {
-      LaunchList ll;
-      ll.begin = (long long)ext.size();
-      for (int s : bylevel[l]) {
-        if (cptr[s + 1] - cptr[s] <= r) continue;
-        int c = clist[cptr[s] + r];
-        int kc = S.sn_first[c + 1] - S.sn_first[c];
-        int mu = (int)(S.sn_rowptr[c + 1] - S.sn_rowptr[c] - kc);
-        int j = 0;
-        while (j < mu) {
-          int j1 = j;
-          long long work = 0;
-          while (j1 < mu && work < EXT_TARGET) { work += mu - j1; j1++; }
-          ext.push_back(make_int4(c, j, j1, s));
-          j = j1;
+    for (int part = 0; part < 2; part++)
+      for (int r = 0; r < maxc; r++) {
+        LaunchList ll;
+        ll.begin = (long long)ext.size();
+        for (int s : bylevel[l]) {
+          if (cptr[s + 1] - cptr[s] <= r) continue;
+          int c = clist[cptr[s] + r];
+          int kc = S.sn_first[c + 1] - S.sn_first[c];
+          int mu = (int)(S.sn_rowptr[c + 1] - S.sn_rowptr[c] - kc);
+          // the child's update rows are sorted; those below the parent's last column are its own columns
+          const int* ur = S.sn_rows.data() + S.sn_rowptr[c] + kc;
+          int js = (int)(std::lower_bound(ur, ur + mu, S.sn_first[s + 1]) - ur);
+          int j = part == 0 ? 0 : js, jend = part == 0 ? js : mu;
+          while (j < jend) {
+            int j1 = j;
+            long long work = 0;
+            while (j1 < jend && work < EXT_TARGET) { work += mu - j1; j1++; }
+            ext.push_back(make_int4(c, j, j1, s));
+            j = j1;
+          }
         }
+        ll.end = (long long)ext.size();
+        if (ll.end > ll.begin) (part == 0 ? L.ext_rounds : L.ext_rounds_u).push_back(ll);
       }
-      ll.end = (long long)ext.size();
-      if (ll.end > ll.begin) L.ext_rounds.push_back(ll);
-    }
     // factor steps: 64-column panels; inner updates stay inside the OB-column inner block; after the first inner
     // block of an OB2-column outer block, a mid update brings the second one up to date; the outer update of
     // everything to its right follows the outer block's last panel
@@ -1452,7 +1437,6 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   if (ext.empty()) ext_tasks_.alloc(1); else ext_tasks_.upload(ext, stream_);
   if (pan.empty()) panel_tasks_.alloc(1); else panel_tasks_.upload(pan, stream_);
   if (syr.empty()) syrk_tasks_.alloc(1); else syrk_tasks_.upload(syr, stream_);
-  if (zer.empty()) zero_tasks_.alloc(1); else zero_tasks_.upload(zer, stream_);
   if (pot.empty()) potrf_tasks_.alloc(1); else potrf_tasks_.upload(pot, stream_);
   level_sn_.upload(lvl_sn, stream_);
   V_.alloc(std::max<long long>(1, S.Vsize));
@@ -1486,9 +1470,6 @@ int SpdDevice::factorize_t(const double* d_vals) {
                              amap_dst_.p, na, L);
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
-    if (lv.zero.end > lv.zero.begin)
-      hipLaunchKernelGGL(k_zero_u<T>, dim3((unsigned)(lv.zero.end - lv.zero.begin)), dim3(256), 0, stream_,
-                         zero_tasks_.p + lv.zero.begin, sd_, U);
     for (const LaunchList& r : lv.ext_rounds)
       hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, stream_,
                          ext_tasks_.p + r.begin, sd_, L, U);
@@ -1526,6 +1507,9 @@ int SpdDevice::factorize_t(const double* d_vals) {
         }
       }
     }
+    for (const LaunchList& r : lv.ext_rounds_u)
+      hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, stream_,
+                         ext_tasks_.p + r.begin, sd_, L, U);
   }
   if (inv_launch_.end > inv_launch_.begin)
     hipLaunchKernelGGL(k_inv_diag<T>, dim3((unsigned)(inv_launch_.end - inv_launch_.begin)), dim3(256), 0, stream_,
