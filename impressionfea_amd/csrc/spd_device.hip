@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ task
 // task: (supernode s, row tile I, column tile J, 0), I >= J.
 template <typename T>
 __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, int kb, SymDev sd,
-                                              T* __restrict__ L, T* __restrict__ U) {
+                                              T* __restrict__ L) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
@@ -296,6 +296,20 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
   const int l16 = lane & 15;
   typedef typename Mfma<T>::v4 v4;
   v4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+  // the lane's 16 output entries (all in the front's own columns: col_end <= k) are loaded before the MFMAs, so
+  // their latency overlaps the product; each entry belongs to one tile of the launch
+  T* Ls = L + sd.Loff[s];
+  T cv[2][2][4];
+#pragma unroll
+  for (int ai = 0; ai < 2; ai++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long long j = j0 + jb + 16 * ai + Mfma<T>::row(lane, r);
+        const long long i = i0 + ib + 16 * bi + l16;
+        cv[ai][bi][r] = (i < m && j < col_end && i >= j) ? Ls[j * m + i] : T(0);
+      }
   for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
     int q = idx >> 6, rr = idx & 63;
     long long ri = i0 + rr, rj = j0 + rr;
@@ -316,21 +330,15 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
   }
   // D[a][b] = sum_q Xj[a][q] Xi[b][q] = contribution to C(i = i0+ib+16bi+b, j = j0+jb+16ai+a)
   v4 accs[2][2] = {{acc00, acc01}, {acc10, acc11}};
-  T* Ls = L + sd.Loff[s];
-  T* Us = U + sd.Uoff[s];
-  const long long mu = m - k;
 #pragma unroll
   for (int ai = 0; ai < 2; ai++)
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        long long j = j0 + jb + 16 * ai + Mfma<T>::row(lane, r);
-        long long i = i0 + ib + 16 * bi + l16;
-        if (i < m && j < col_end && i >= j) {
-          T* cp = (j < k) ? (Ls + j * m + i) : (Us + (j - k) * mu + (i - k));
-          *cp -= accs[ai][bi][r];
-        }
+        const long long j = j0 + jb + 16 * ai + Mfma<T>::row(lane, r);
+        const long long i = i0 + ib + 16 * bi + l16;
+        if (i < m && j < col_end && i >= j) Ls[j * m + i] = cv[ai][bi][r] - accs[ai][bi][r];
       }
 }
 
@@ -1489,7 +1497,7 @@ int SpdDevice::factorize_t(const double* d_vals) {
         if (profiling_) IFEA_HIP(hipEventRecord(event(2 * nev), stream_));
         if (mode == 0)
           hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
-                             syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+                             syrk_tasks_.p + q.begin, q.c0, sd_, L);
         else if (mode == 1)
           hipLaunchKernelGGL((k_syrk_big<T, 512, 16, true>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0,
                              stream_, syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
