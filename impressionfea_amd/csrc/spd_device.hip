@@ -23,6 +23,7 @@ constexpr int OB2 = 512;        // outer block: the trailing matrix beyond it is
                                 // it, a mid update of OB columns brings its second half up to date
 constexpr long long OUTER_GROUP = 8;  // outer-update tiles are listed in groups of this many tile rows and columns
 constexpr int EXT_TARGET = 8192;
+constexpr int STAGE_PER = NB * 64 / 256;  // 64x64-tile elements each of 256 threads stages (TRSM, inner update)
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
 constexpr int SB = 2048;        // diagonal block of the blocked sweeps (explicit inverse per block)
@@ -239,11 +240,22 @@ __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ task
   const long long r0 = tk.y;
   const int nr = tk.z - tk.y;
   const double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
-  for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
-    int p = idx >> 6, rr = idx & 63;
-    sA[p][rr] = (p < w && rr < nr) ? (double)P[(long long)(c0 + p) * m + r0 + rr] : 0.0;
-    // Lt is stored [c][r] = Linv[r][c]; sL[p][q] = Linv[q][p] = Lt[p][q]
-    sL[p][rr] = (p < w && rr < w) ? Lt[p * NB + rr] : 0.0;
+  {
+    // all of a thread's loads in flight before its LDS writes (256 threads, STAGE_PER elements of each tile)
+    double va[STAGE_PER], vl[STAGE_PER];
+#pragma unroll
+    for (int t = 0; t < STAGE_PER; t++) {
+      const int idx = threadIdx.x + 256 * t, p = idx >> 6, rr = idx & 63;
+      va[t] = (p < w && rr < nr) ? (double)P[(long long)(c0 + p) * m + r0 + rr] : 0.0;
+      // Lt is stored [c][r] = Linv[r][c]; sL[p][q] = Linv[q][p] = Lt[p][q]
+      vl[t] = (p < w && rr < w) ? Lt[p * NB + rr] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_PER; t++) {
+      const int idx = threadIdx.x + 256 * t, p = idx >> 6, rr = idx & 63;
+      sA[p][rr] = va[t];
+      sL[p][rr] = vl[t];
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -310,12 +322,23 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
         const long long i = i0 + ib + 16 * bi + l16;
         cv[ai][bi][r] = (i < m && j < col_end && i >= j) ? Ls[j * m + i] : T(0);
       }
-  for (int idx = threadIdx.x; idx < NB * 64; idx += blockDim.x) {
-    int q = idx >> 6, rr = idx & 63;
-    long long ri = i0 + rr, rj = j0 + rr;
-    const T* cp = P + (long long)(kb + q) * m;
-    Xi[q][rr] = (q < w && ri < m) ? cp[ri] : T(0);
-    Xj[q][rr] = (q < w && rj < m) ? cp[rj] : T(0);
+  {
+    // all of a thread's loads in flight before its LDS writes (256 threads, STAGE_PER elements of each operand)
+    T xi[STAGE_PER], xj[STAGE_PER];
+#pragma unroll
+    for (int t = 0; t < STAGE_PER; t++) {
+      const int idx = threadIdx.x + 256 * t, q = idx >> 6, rr = idx & 63;
+      const long long ri = i0 + rr, rj = j0 + rr;
+      const T* cp = P + (long long)(kb + q) * m;
+      xi[t] = (q < w && ri < m) ? cp[ri] : T(0);
+      xj[t] = (q < w && rj < m) ? cp[rj] : T(0);
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_PER; t++) {
+      const int idx = threadIdx.x + 256 * t, q = idx >> 6, rr = idx & 63;
+      Xi[q][rr] = xi[t];
+      Xj[q][rr] = xj[t];
+    }
   }
   __syncthreads();
   const int kq = (w + 3) & ~3;
