@@ -144,6 +144,9 @@ int ifea_spd_factorize(ifea_spd_t h, const double* values) {
   } catch (const ifea::Error& e) {
     set_last_error(e.what());
     return e.code;
+  } catch (const std::exception& e) {  // host allocation failures and the like never cross the C boundary
+    set_last_error(e.what());
+    return IFEA_ERR_DEVICE;
   }
 }
 
@@ -207,6 +210,9 @@ int ifea_spd_solve(ifea_spd_t h, const double* b, double* x) {
   } catch (const ifea::Error& e) {
     set_last_error(e.what());
     return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return IFEA_ERR_DEVICE;
   }
 }
 
@@ -340,6 +346,16 @@ int ifea_step_final(ifea_step_t h, const int* die_axis, double* power, double* r
 int ifea_step_advance(ifea_step_t h, double dt, double* eps, double* sigma, double* rate) {
   if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
   IFEA_GUARD(h->dev->advance(dt, eps, sigma, rate))
+}
+
+int ifea_step_element_results(ifea_step_t h, double* mean, double* damage, double* dev6) {
+  if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->element_results(mean, damage, dev6))
+}
+
+int ifea_step_set_damage(ifea_step_t h, const double* damage) {
+  if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->set_damage(damage))
 }
 
 int ifea_step_get_v(ifea_step_t h, double* v) {

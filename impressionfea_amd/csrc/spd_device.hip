@@ -1575,6 +1575,8 @@ int SpdDevice::factorize_t(const double* d_vals) {
 
 int SpdDevice::factorize_device(const double* d_vals, int precision) {
   if (!analyzed_) throw Error(-5, "Analyze must be called first.");
+  // the factor storage is overwritten from here on: a factorization that throws or fails leaves no usable factor
+  factored_ = false;
   hipEvent_t e0, e1;
   IFEA_HIP(hipEventCreate(&e0));
   IFEA_HIP(hipEventCreate(&e1));

@@ -28,6 +28,9 @@ class StepDevice {
   void solve_velocity(int warm, ifea_step_stats* st);
   void final_eval(const int* die_axis, double* power, double* reaction, double* vg, double* r, unsigned char* release);
   void advance(double dt, double* eps, double* sigma, double* rate);
+  // element mean stress of the last FinalEvaluation, accumulated damage, deviatoric stress of the last assembly
+  void element_results(double* mean, double* damage, double* dev6);
+  void set_damage(const double* damage);
   void get_v(double* v);
   void set_v(const double* v);
   // index of the first tet whose quality at the host positions x is <= qmin, or -1; per-tet values into q if given
@@ -62,7 +65,7 @@ class StepDevice {
   DBuf<double> x_, ax_, fixed_val_, grad_, vol_, vnode_, gcoef_, kvol_, afric_, kfric_;
   DBuf<int> nfix_, contact_, bad_;
   // state
-  DBuf<double> eps_, el_sig_, el_rate_, el_dev_, el_mean_;
+  DBuf<double> eps_, el_sig_, el_rate_, el_dev_, el_mean_, dmg_, nodal_;
   // iteration
   DBuf<double> v_, r_, dv_, vtrial_, val_, e_wm_, e_wn_, e_g_, e_energy_, e_power_, n_energy_, n_power_;
   DBuf<double> pr_, pz_, pp_, pq_, tmp_;

@@ -663,6 +663,59 @@ __global__ void k_advance(int ne, double dt, const double* __restrict__ rate, do
   if (e < ne) eps[e] += rate[e] * dt;
 }
 
+// FinalEvaluation's element mean stress (:1440-1450): the nodal (ANP) volumetric rate kPen g_n . vg from the global
+// velocities of the converged field, then the mean of an element's four nodes
+__global__ void k_nodal_pressure(int nn, double kpen, const int* __restrict__ ps, const int* __restrict__ pn,
+                                 const double* __restrict__ gcoef, const double* __restrict__ vg,
+                                 double* __restrict__ nodal) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double ev = 0.0;
+  for (int q = ps[n]; q < ps[n + 1]; q++) {
+    const int m = pn[q];
+    const double* c = gcoef + 3 * (long long)q;
+    ev += c[0] * vg[3 * m] + c[1] * vg[3 * m + 1] + c[2] * vg[3 * m + 2];
+  }
+  nodal[n] = kpen * ev;
+}
+
+__global__ void k_el_mean(int ne, const int* __restrict__ tets, const double* __restrict__ nodal,
+                          double* __restrict__ mean) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int* t = tets + 4 * (long long)e;
+  mean[e] = 0.25 * (nodal[t[0]] + nodal[t[1]] + nodal[t[2]] + nodal[t[3]]);
+}
+
+// Largest eigenvalue of a symmetric 3x3 tensor by the trigonometric solution of its characteristic cubic
+// (Principal.Max3, Solver/Principal.cs:7-19)
+__device__ double principal_max3(double xx, double yy, double zz, double xy, double yz, double zx) {
+  const double p1 = xy * xy + yz * yz + zx * zx;
+  const double q = (xx + yy + zz) / 3;
+  if (p1 < 1e-30 * (xx * xx + yy * yy + zz * zz + 1e-300)) return fmax(xx, fmax(yy, zz));
+  const double p2 = (xx - q) * (xx - q) + (yy - q) * (yy - q) + (zz - q) * (zz - q) + 2 * p1;
+  const double p = sqrt(p2 / 6);
+  const double bxx = (xx - q) / p, byy = (yy - q) / p, bzz = (zz - q) / p, bxy = xy / p, byz = yz / p, bzx = zx / p;
+  const double det = bxx * (byy * bzz - byz * byz) - bxy * (bxy * bzz - byz * bzx) + bzx * (bxy * byz - byy * bzx);
+  const double r = fmin(1.0, fmax(-1.0, det / 2));
+  return q + 2 * p * cos(acos(r) / 3);
+}
+
+// AccumulateDamage (:1844-1855): normalised Cockcroft-Latham D += max(sigma_1, 0) / sigma_bar * rate * dt with
+// sigma = s + sigma_m I from the converged deviatoric stress and the element mean stress; tension only
+__global__ void k_damage(int ne, double dt, double rate_cut, const double* __restrict__ sig,
+                         const double* __restrict__ rate, const double* __restrict__ dev, const double* __restrict__ mean,
+                         double* __restrict__ dmg) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const double s = sig[e], re = rate[e];
+  if (!(s > 0) || re <= rate_cut) return;
+  const double* d = dev + 6 * (long long)e;
+  const double m = mean[e];
+  const double s1 = principal_max3(d[0] + m, d[1] + m, d[2] + m, d[3], d[4], d[5]);
+  if (s1 > 0) dmg[e] += s1 / s * re * dt;
+}
+
 // ------------------------------------------------------------------------------- host
 StepDevice::StepDevice(int nn, int ne, const int* tets, const int* colPtr, const int* rowIdx, int nf, const int* faces,
                        const int* face_owner)
@@ -764,7 +817,11 @@ StepDevice::StepDevice(int nn, int ne, const int* tets, const int* colPtr, const
   kvol_.alloc(nnz_); afric_.alloc(nn); kfric_.alloc(nn);
   nfix_.alloc(nn); contact_.alloc(nn); bad_.alloc(1);
   eps_.alloc(ne); el_sig_.alloc(ne); el_rate_.alloc(ne); el_dev_.alloc(6 * (size_t)ne); el_mean_.alloc(ne);
+  dmg_.alloc(ne); nodal_.alloc(nn);
   IFEA_HIP(hipMemsetAsync(eps_.p, 0, ne * sizeof(double), s_));
+  IFEA_HIP(hipMemsetAsync(el_dev_.p, 0, 6 * (size_t)ne * sizeof(double), s_));
+  IFEA_HIP(hipMemsetAsync(el_mean_.p, 0, ne * sizeof(double), s_));
+  IFEA_HIP(hipMemsetAsync(dmg_.p, 0, ne * sizeof(double), s_));
   IFEA_HIP(hipMemsetAsync(el_sig_.p, 0, ne * sizeof(double), s_));
   IFEA_HIP(hipMemsetAsync(el_rate_.p, 0, ne * sizeof(double), s_));
   v_.alloc(ndof_); r_.alloc(ndof_); dv_.alloc(ndof_); vtrial_.alloc(ndof_); val_.alloc(nnz_);
@@ -879,6 +936,8 @@ int StepDevice::refactor(int precision, ifea_step_stats* st) {
     spd_->analyze(ndof_, h_colPtr_.data(), h_rowIdx_.data(), user_perm_.empty() ? nullptr : user_perm_.data(), opt);
     need_analyze_ = false;
   }
+  // a factorization that throws (out of memory, a device error) must not leave PCG with a half-written factor
+  have_factor_ = false;
   int rc = spd_->factorize_device(val_.p, precision);
   if (st) {
     st->factorizations++;
@@ -1037,6 +1096,10 @@ void StepDevice::final_eval(const int* die_axis, double* power, double* reaction
   IFEA_HIP(hipMemsetAsync(maxdiag_.p, 0, 2 * sizeof(unsigned long long), s_));
   hipLaunchKernelGGL(k_final_nodes, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, ax_.p, v_.p, r_.p, contact_.p,
                      da.p, vgd.p, rz.p, comp.p, maxdiag_.p);
+  hipLaunchKernelGGL(k_nodal_pressure, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, P_.kpen, p_start_.p,
+                     p_node_.p, gcoef_.p, vgd.p, nodal_.p);
+  hipLaunchKernelGGL(k_el_mean, dim3(grid_for(ne_, 256)), dim3(256), 0, s_, ne_, tets_.p, nodal_.p, el_mean_.p);
+  IFEA_HIP(hipGetLastError());
   vec_->dot(rz.p, nullptr, nn_, S_REACTION);
   double rc;
   vec_->read(S_REACTION, 1, &rc);
@@ -1073,9 +1136,25 @@ void StepDevice::final_eval(const int* die_axis, double* power, double* reaction
 
 void StepDevice::advance(double dt, double* eps, double* sigma, double* rate) {
   hipLaunchKernelGGL(k_advance, dim3(grid_for(ne_, 256)), dim3(256), 0, s_, ne_, dt, el_rate_.p, eps_.p);
+  hipLaunchKernelGGL(k_damage, dim3(grid_for(ne_, 256)), dim3(256), 0, s_, ne_, dt, P_.rate_cut, el_sig_.p,
+                     el_rate_.p, el_dev_.p, el_mean_.p, dmg_.p);
+  IFEA_HIP(hipGetLastError());
   if (eps) IFEA_HIP(hipMemcpyAsync(eps, eps_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
   if (sigma) IFEA_HIP(hipMemcpyAsync(sigma, el_sig_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
   if (rate) IFEA_HIP(hipMemcpyAsync(rate, el_rate_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+void StepDevice::element_results(double* mean, double* damage, double* dev6) {
+  if (mean) IFEA_HIP(hipMemcpyAsync(mean, el_mean_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (damage) IFEA_HIP(hipMemcpyAsync(damage, dmg_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (dev6) IFEA_HIP(hipMemcpyAsync(dev6, el_dev_.p, 6 * (size_t)ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+void StepDevice::set_damage(const double* damage) {
+  if (damage) IFEA_HIP(hipMemcpyAsync(dmg_.p, damage, ne_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  else IFEA_HIP(hipMemsetAsync(dmg_.p, 0, ne_ * sizeof(double), s_));
   IFEA_HIP(hipStreamSynchronize(s_));
 }
 

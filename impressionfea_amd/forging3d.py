@@ -442,6 +442,18 @@ class Forging3DSolver:
         val[dg] += 1e-10 * val[dg].max()
         return val, -np.where(fixed, 0.0, r)
 
+    def element_results(self):
+        """Element mean stress of the last FinalEvaluation (:1440-1450), accumulated Cockcroft-Latham damage
+        (AccumulateDamage :1844-1855) and the deviatoric stress [ne, 6] of the last assembly, from the device."""
+        mean, dmg, dev = np.zeros(self.ne), np.zeros(self.ne), np.zeros((self.ne, 6))
+        self._check(N.lib().ifea_step_element_results(self._h, N.ptr(mean, C.c_double), N.ptr(dmg, C.c_double),
+                                                      N.ptr(dev, C.c_double)), "element_results")
+        return dict(elMean=mean, dmg=dmg, elDev=dev)
+
+    def set_damage(self, dmg=None):
+        d = N.f64(dmg) if dmg is not None else None
+        self._check(N.lib().ifea_step_set_damage(self._h, N.ptr(d, C.c_double)), "set_damage")
+
     def get_v(self):
         v = np.zeros(self.ndof)
         self._check(N.lib().ifea_step_get_v(self._h, N.ptr(v, C.c_double)), "get_v")

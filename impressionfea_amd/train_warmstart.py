@@ -72,8 +72,8 @@ def build_dataset(seeds, increments=20, log=print):
     return np.concatenate(X), np.concatenate(Y)
 
 
-def train(seeds=TRAIN_SEEDS, epochs=60, batch=4096, lr=2e-3, seed=0, out=DEFAULT_WEIGHTS, log=print):
-    X, Y = build_dataset(seeds, log=log)
+def train(seeds=TRAIN_SEEDS, epochs=60, batch=4096, lr=2e-3, seed=0, out=DEFAULT_WEIGHTS, log=print, increments=20):
+    X, Y = build_dataset(seeds, increments=increments, log=log)
     # targets are O(1e-3) of the ram speed: train on them in units of their RMS so the BF16 network sees O(1) values
     scale = float(np.sqrt(np.mean(Y.astype(np.float64) ** 2))) or 1.0
     Y = (Y / scale).astype(np.float32)

@@ -87,8 +87,18 @@ int ifea_step_solve_velocity(ifea_step_t h, int warm, ifea_step_stats* stats);
 int ifea_step_final(ifea_step_t h, const int* die_axis, double* power, double* reaction, double* vg, double* r,
                     unsigned char* release);
 
-/* eps += rate * dt (Forging3DSolver.cs:626) on the device; copies out eps, sigma, rate when not NULL. */
+/* eps += rate * dt (Forging3DSolver.cs:626) and the damage update (AccumulateDamage, :1844-1855) on the device;
+ * copies out eps, sigma, rate when not NULL. */
 int ifea_step_advance(ifea_step_t h, double dt, double* eps, double* sigma, double* rate);
+
+/* Element results (FinalEvaluation :1440-1450, AccumulateDamage :1844-1855): mean[ne] the element mean stress
+ * 1/4 sum kPen g_n . vg of the last ifea_step_final; damage[ne] the Cockcroft-Latham damage accumulated by
+ * ifea_step_advance (D += max(sigma_1, 0) / sigma_bar * rate * dt, Principal.Max3 for sigma_1); dev6[6ne] the
+ * deviatoric stress (xx, yy, zz, xy, yz, zx) of the last assembly. Any pointer may be NULL. */
+int ifea_step_element_results(ifea_step_t h, double* mean, double* damage, double* dev6);
+
+/* Damage state damage[ne] (a restart), or zero when NULL. */
+int ifea_step_set_damage(ifea_step_t h, const double* damage);
 
 /* Current local-frame velocity v[3nn] (out) / set it (in). */
 int ifea_step_get_v(ifea_step_t h, double* v);

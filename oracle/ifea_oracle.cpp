@@ -1311,6 +1311,27 @@ void orc_get_element_state(void* h, double* elSig, double* elRate, double* elDev
   if (dmg) std::copy(o->dmg.begin(), o->dmg.end(), dmg);
 }
 
+// Overwrite the converged element state that AccumulateDamage reads (any pointer may be NULL).
+void orc_set_element_state(void* h, const double* elSig, const double* elRate, const double* elDev,
+                           const double* elMean) {
+  auto* o = (Oracle*)h;
+  if (elSig) std::copy(elSig, elSig + o->ne, o->elSig.begin());
+  if (elRate) std::copy(elRate, elRate + o->ne, o->elRate.begin());
+  if (elDev) std::copy(elDev, elDev + 6 * (size_t)o->ne, o->elDev.begin());
+  if (elMean) std::copy(elMean, elMean + o->ne, o->elMean.begin());
+}
+
+// AccumulateDamage (Forging3DSolver.cs:1844-1855) on the current element state.
+void orc_accumulate_damage(void* h, double dt) { ((Oracle*)h)->AccumulateDamage(dt); }
+
+// Principal.Max3 (Solver/Principal.cs:7-19) of n tensors t6[6n] = (xx, yy, zz, xy, yz, zx).
+void orc_max3(int n, const double* t6, double* out) {
+  for (int k = 0; k < n; k++) {
+    const double* t = t6 + 6 * (size_t)k;
+    out[k] = max3(t[0], t[1], t[2], t[3], t[4], t[5]);
+  }
+}
+
 // stats: [factorizations, pcg iterations, pcg calls, solves, assemblies]
 int orc_solve_velocity(void* h, int* converged, double* lastRel, int* stats) {
   auto* o = (Oracle*)h;

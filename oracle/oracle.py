@@ -63,6 +63,9 @@ def lib():
         L.orc_assemble.restype = C.c_double
         L.orc_assemble.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp]
         L.orc_get_element_state.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.orc_set_element_state.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_accumulate_damage.argtypes = [vp, C.c_double]
+        L.orc_max3.argtypes = [C.c_int, dp, dp]
         L.orc_solve_velocity.argtypes = [vp, ip, dp, ip]
         L.orc_final.restype = C.c_double
         L.orc_final.argtypes = [vp, dp]
@@ -232,6 +235,13 @@ class Oracle:
         lib().orc_get_element_state(self.h, _p(s, C.c_double), _p(ra, C.c_double), _p(dv, C.c_double), _p(mn, C.c_double), _p(dm, C.c_double))
         return dict(elSig=s, elRate=ra, elDev=dv, elMean=mn, dmg=dm)
 
+    def set_element_state(self, elSig=None, elRate=None, elDev=None, elMean=None):
+        k = [F64(a) if a is not None else None for a in (elSig, elRate, elDev, elMean)]
+        lib().orc_set_element_state(self.h, *(_p(a, C.c_double) for a in k))
+
+    def accumulate_damage(self, dt):
+        lib().orc_accumulate_damage(self.h, float(dt))
+
     def solve_velocity(self):
         cv = C.c_int(0); rel = C.c_double(0); st = (C.c_int * 5)()
         it = lib().orc_solve_velocity(self.h, C.byref(cv), C.byref(rel), st)
@@ -307,6 +317,14 @@ def quality(x, tets):
     q = np.zeros(t.shape[0])
     lib().orc_quality(t.shape[0], _p(t, C.c_int), _p(xs, C.c_double), _p(q, C.c_double))
     return q
+
+
+def max3(t6):
+    """Principal.Max3 (Solver/Principal.cs:7-19) of tensors t6[..., 6] = (xx, yy, zz, xy, yz, zx) (oracle)."""
+    t = F64(np.asarray(t6, dtype=np.float64).reshape(-1, 6))
+    out = np.zeros(t.shape[0])
+    lib().orc_max3(t.shape[0], _p(t, C.c_double), _p(out, C.c_double))
+    return out
 
 
 def brute_symbolic(n, colPtr, rowIdx, perm):

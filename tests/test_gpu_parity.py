@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
+from impressionfea_amd import _native as N
 from impressionfea_amd import spd
 from impressionfea_amd.config import upset_small
 from impressionfea_amd.errors import MatrixNotPositiveDefiniteError
@@ -96,28 +97,53 @@ def test_spd_caller_permutation_is_kept(c1_system):
         assert np.linalg.norm(As @ x - b) <= 1e-8 * np.linalg.norm(b)
 
 
-def test_spd_mixed_precision_refines(c1_system):
-    o, cp, ri, A, b = c1_system
+def _upset_system(penalty_factor):
+    import oracle as O
+    wl = upset_small(penalty_factor=penalty_factor)
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=0)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    o.prepare()
+    return (o,) + _bc_system(o, o.frames(), o.v())
+
+
+@pytest.mark.parametrize("penalty_factor", [1e4, 1e7])
+def test_spd_mixed_precision_refines(oracle_lib, penalty_factor):
+    """The slot in mixed precision on the first Newton system of the configured upset. At the reference's default
+    penalty factor 1e4 the FP32 factor is positive definite and FP64 refinement around it reaches 1e-10 (no
+    fallback); at 1e7 the FP32 factor is not positive definite and Factorize falls back to the FP64 factor (one
+    fallback, counted before any solve); the solve then refines with the factor's FP32 copy."""
+    o, cp, ri, A, b = _upset_system(penalty_factor)
     n = o.ndof
     As = sp.csc_matrix((A, ri, cp), shape=(n, n))
     with spd.HipSpdSolver(precision=1, refine_tol=1e-10) as h:
         h.Analyze(n, cp, ri)
         h.Factorize(A)
+        fb = h.stats()["fp64_fallbacks"]
         x = h.Solve(b)
         st = h.stats()
-        # FP64 refinement reaches the FP64 factor's own backward error (~1e-9 on this penalty system)
+        print(f"penalty {penalty_factor:.0e}: fallbacks after Factorize {fb}, after Solve {st['fp64_fallbacks']}, "
+              f"refinement iterations {st['refine_iterations']}")
+        # FP64 refinement reaches the FP64 factor's own backward error (~1e-9 on these penalty systems)
         assert np.linalg.norm(As @ x - b) <= 1e-8 * np.linalg.norm(b)
-        assert st["refine_iterations"] >= 1 or st["fp64_fallbacks"] >= 1
+        assert st["refine_iterations"] >= 1
+        if penalty_factor == 1e4:
+            assert fb == 0 and st["fp64_fallbacks"] == 0, st
+        else:
+            assert fb == 1 and st["fp64_fallbacks"] == 1, st
 
 
-@pytest.mark.parametrize("precision", [0, 2])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("nxy,nz,min_tail", [(14, 20, 2048), (20, 28, 4096)])
 def test_spd_wide_fronts(precision, nxy, nz, min_tail):
     """Two-ring pattern of a Kuhn cylinder under the geometric ordering: 14x14x20 cells (14,175 DOF, dense trailing
     triangle of 2,160 columns: two 2048-column sweep steps) and 20x20x28 cells (38,367 DOF, 4,284 columns: three
     steps, the last one 188 wide), with update-vector rectangles on the fronts below; the blocked multi-launch sweep
     path. Random diagonally dominant SPD values; the solution must reach a backward error of 1e-13 with the FP64
-    factor, or after refinement with the FP32 copy of the factor, and match scipy's sparse LU on the smaller case."""
+    factor, or after refinement with the FP32 factor (precision 1: FP32 MFMA updates over several 512-column outer
+    blocks, mid updates included) or the FP32 copy of the factor (precision 2), and match scipy's sparse LU on the
+    smaller case. The FP32 factor of these well-conditioned matrices must neither fall back nor skip refinement, and
+    its unrefined solution must already be within 1e-5 of the refined one."""
     import scipy.sparse.linalg as spla
     x, t, ox, oy = kuhn_cylinder(40, 60, nxy, nz, seed=1)
     m = TetMesh(x.shape[0], t, ox, oy)
@@ -147,14 +173,20 @@ def test_spd_wide_fronts(precision, nxy, nz, min_tail):
             assert np.linalg.norm(A @ xs - b) <= 1e-13 * np.linalg.norm(b)
             if xr is not None:
                 assert np.linalg.norm(xs - xr) <= 1e-12 * np.linalg.norm(xr)
-        if precision == 2:
-            assert h.stats()["refine_iterations"] >= 1
+        if precision != 0:
+            st = h.stats()
+            assert st["refine_iterations"] >= 1 and st["fp64_fallbacks"] == 0, st
         # deterministic: the same values factor and solve to the same bits
         h.Factorize(A.data)
         assert np.array_equal(h.Solve(b), xs)
+        if precision == 1:
+            h.set_option(N.IFEA_OPT_REFINE_TOL, 1.0)   # accept the first solve: the FP32 factor alone
+            x32 = h.Solve(b)
+            assert h.stats()["refine_iterations"] == 0
+            assert 0 < np.linalg.norm(x32 - xs) <= 1e-5 * np.linalg.norm(xs)
 
 
-@pytest.mark.parametrize("precision", [0, 2])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_spd_split_sweep_tasks(precision):
     """Dense blocks [A B 0; B' B C; 0 C' C] of 2,100 columns each under the identity ordering: two supernodes, the
     first with 2,100 columns and 2,100 update-vector rows, so its forward update-vector products run as two column
@@ -183,9 +215,15 @@ def test_spd_split_sweep_tasks(precision):
             xs = h.Solve(b)
             assert np.linalg.norm(A @ xs - b) <= 1e-13 * np.linalg.norm(b)
             assert np.linalg.norm(xs - xr) <= 1e-12 * np.linalg.norm(xr)
+        if precision != 0:
+            st = h.stats()
+            assert st["refine_iterations"] >= 1 and st["fp64_fallbacks"] == 0, st
 
 
 def test_spd_not_positive_definite(c1_system):
+    """A matrix that is not positive definite raises MatrixNotPositiveDefiniteError (MklPardisoSolver.cs:95-97), and
+    the failed Factorize leaves no factor behind: the next Solve is a call-order error, not a solve with the previous
+    (partly overwritten) factor."""
     o, cp, ri, A, b = c1_system
     bad = A.copy()
     col = np.repeat(np.arange(o.ndof), np.diff(cp))
@@ -193,8 +231,14 @@ def test_spd_not_positive_definite(c1_system):
     bad[d[100]] = -abs(bad[d[100]])
     with spd.HipSpdSolver() as h:
         h.Analyze(o.ndof, cp, ri)
+        h.Factorize(A)
+        h.Solve(b)
         with pytest.raises(MatrixNotPositiveDefiniteError):
             h.Factorize(bad)
+        with pytest.raises(RuntimeError, match="Factorize must be called first"):
+            h.Solve(b)
+        h.Factorize(A)      # a good matrix factors again on the same analysis
+        h.Solve(b)
 
 
 def test_spd_call_order():

@@ -1,0 +1,92 @@
+"""GPU parity of FinalEvaluation's element mean stress, AccumulateDamage and the converged velocities at the reference's
+default penalty, against the FP64 CPU oracle, through the C ABI.
+
+The element mean stress kPen * g_n . v is a product of a large penalty and a small divergence, so it is compared on
+identical inputs (the same velocity field on both sides) to rounding, and along a stepped run at the reference's
+default penalty factor 1e4 (Forging3DSolver.cs:27), where the FP64 solutions agree to ~1e-12.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from impressionfea_amd import _native as N
+from impressionfea_amd.config import upset_small
+from impressionfea_amd.forging3d import Forging3DSolver
+from impressionfea_amd.mesh import kuhn_cylinder
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(wl, seed=0):
+    import oracle as O
+    x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=seed, quarter=wl.quarter)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.initial(0.0)
+    s = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0)
+    s.initial(0.0)
+    return o, s
+
+
+def test_mean_stress_and_damage_same_state(oracle_lib):
+    """One FinalEvaluation and one damage update on the same (perturbed, compressible) velocity field on both sides:
+    the element mean stress to 1e-12 of kPen |g| |v|, the deviatoric stress to 1e-12 of its largest entry, and the
+    damage to 1e-9 of its largest value (the trigonometric largest-eigenvalue formula loses digits near double roots,
+    and the device's acos/cos differ from the host's in the last bits)."""
+    wl = upset_small(nxy=6, nz=8, quarter=True)
+    o, s = _pair(wl, seed=4)
+    o.prepare()
+    s.prepare()
+    rng = np.random.default_rng(9)
+    v = o.v() * (1 + 0.2 * rng.standard_normal(o.ndof)) + 0.02 * rng.standard_normal(o.ndof)
+    o.set_v(v)
+    o.final()
+    s.set_v(v)
+    da = N.i32(s.die_axis)
+    pw, rc = C.c_double(0), C.c_double(0)
+    assert N.lib().ifea_step_final(s._h, N.ptr(da, C.c_int), C.byref(pw), C.byref(rc), None, None, None) == 0
+    eo = o.element_state()
+    eg = s.element_results()
+    kpen = o.constants()["kPen"]
+    gscale = kpen * np.abs(o.geometry()["grad"]).max() * np.abs(v).max()
+    assert np.abs(eg["elMean"] - eo["elMean"]).max() <= 1e-12 * gscale
+    assert np.abs(eo["elMean"]).max() > 1e-6 * gscale          # a compressible field: the mean stress is not noise
+    assert np.abs(eg["elDev"] - eo["elDev"]).max() <= 1e-12 * np.abs(eo["elDev"]).max()
+    dt = 0.05
+    o.accumulate_damage(dt)
+    assert N.lib().ifea_step_advance(s._h, dt, None, None, None) == 0
+    d_o, d_g = o.element_state()["dmg"], s.element_results()["dmg"]
+    assert (d_o > 0).sum() > 0.1 * o.ne
+    assert np.array_equal(d_o > 0, d_g > 0)
+    assert np.abs(d_g - d_o).max() <= 1e-9 * d_o.max()
+    # a restart value replaces the state; NULL clears it
+    s.set_damage(np.full(o.ne, 0.25))
+    assert np.all(s.element_results()["dmg"] == 0.25)
+    s.set_damage(None)
+    assert np.all(s.element_results()["dmg"] == 0.0)
+    s.close()
+
+
+def test_velocities_mean_stress_damage_default_penalty(oracle_lib):
+    """Three increments of the small upset at the reference's default penalty factor 1e4 and a tight nonlinear
+    tolerance: the converged velocities agree with the oracle's to 1e-9 relative (the plain-FP64 velocity bar), the
+    ram load to 1e-6, and the element mean stress and accumulated damage of the stepped run to 1e-6 of their
+    largest values."""
+    wl = upset_small(nxy=8, nz=8, tolerance=1e-12, penalty_factor=1e4)
+    o, s = _pair(wl)
+    rows = []
+    for k in range(3):
+        ro, rs = o.step(), s.step()
+        assert rs.converged
+        vg = o.state()["vg"]
+        dv = np.linalg.norm(s.vg - vg) / np.linalg.norm(vg)
+        eo, eg = o.element_state(), s.element_results()
+        dm = np.abs(eg["elMean"] - eo["elMean"]).max() / np.abs(eo["elMean"]).max()
+        dd = np.abs(eg["dmg"] - eo["dmg"]).max() / max(eo["dmg"].max(), 1e-300)
+        rows.append((k, rs.load / ro["load"] - 1, dv, dm, dd))
+        print(f"increment {k}: load rel {rows[-1][1]:.2e} |dv|/|v| {dv:.2e} mean stress {dm:.2e} damage {dd:.2e}")
+        assert rs.load == pytest.approx(ro["load"], rel=1e-6), rows
+        assert dv <= 1e-9, rows
+        assert dm <= 1e-6 and dd <= 1e-6, rows
+        assert eo["dmg"].max() > 0
+    s.close()
