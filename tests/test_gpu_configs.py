@@ -58,8 +58,8 @@ def _device_used():
 def test_conrod_full_size_properties():
     """The connecting-rod die at full size (324,433 nodes, 1,843,200 tets, 973,299 DOF): the first Picard system as
     SolveVelocity forms it, assembled on the GPU, factored and solved through the ISpdSolver slot. Properties that need
-    no CPU factorization of this size: the backward error |A x - b| / |b| (scipy's SpMV on the host) at the level of
-    an FP64 factorization of a penalty system, the slot's nnz(L) and column counts equal to the host-only symbolic
+    no CPU factorization of this size: the normwise backward error |A x - b| / (|A| |x| + |b|) (scipy's SpMV on the
+    host) of a backward-stable FP64 factorization, the slot's nnz(L) and column counts equal to the host-only symbolic
     analysis of the same ordering, and the device memory of the process below the MI355X's 288 GB."""
     t0 = time.time()
     wl = conrod_production()
@@ -81,9 +81,13 @@ def test_conrod_full_size_properties():
         print(f"  factored and solved ({time.time() - t0:.1f} s); device memory {used / 2**30:.1f} GiB of "
               f"{total / 2**30:.1f} GiB", flush=True)
     A = sp.csc_matrix((val, s.rowIdx, s.colPtr), shape=(n, n))
-    berr = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
-    print(f"  backward error {berr:.2e}", flush=True)
-    assert np.isfinite(x).all() and berr <= 1e-8
+    res = np.linalg.norm(A @ x - b)
+    anorm = abs(A).sum(axis=0).max()        # ||A||_1
+    eta = res / (anorm * np.linalg.norm(x) + np.linalg.norm(b))
+    print(f"  relative residual {res / np.linalg.norm(b):.2e}, normwise backward error {eta:.2e}", flush=True)
+    # a backward-stable FP64 factorization gives a normwise backward error of a small multiple of n eps |A|; the
+    # relative residual |r| / |b| is larger by the condition of the penalty system (~1e10 at this penalty)
+    assert np.isfinite(x).all() and eta <= 1e-14 and res <= 1e-6 * np.linalg.norm(b)
     # the host-only analysis of the slot's ordering gives the same structure
     po, par2, cc2 = (np.zeros(n, np.int32) for _ in range(3))
     nz = C.c_longlong(0)

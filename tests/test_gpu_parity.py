@@ -112,10 +112,16 @@ def test_spd_mixed_precision_refines(oracle_lib, penalty_factor):
     """The slot in mixed precision on the first Newton system of the configured upset. At the reference's default
     penalty factor 1e4 the FP32 factor is positive definite and FP64 refinement around it reaches 1e-10 (no
     fallback); at 1e7 the FP32 factor is not positive definite and Factorize falls back to the FP64 factor (one
-    fallback, counted before any solve); the solve then refines with the factor's FP32 copy."""
+    fallback, counted before any solve), and refinement with the FP64 factor's FP32 copy stalls too, so the solve
+    falls back to the FP64 sweeps (the second fallback). Either way the result has the backward error of a plain FP64
+    factorization of the same system (within 10x)."""
     o, cp, ri, A, b = _upset_system(penalty_factor)
     n = o.ndof
     As = sp.csc_matrix((A, ri, cp), shape=(n, n))
+    with spd.HipSpdSolver(precision=0) as h:
+        h.Analyze(n, cp, ri)
+        h.Factorize(A)
+        res64 = np.linalg.norm(As @ h.Solve(b) - b) / np.linalg.norm(b)
     with spd.HipSpdSolver(precision=1, refine_tol=1e-10) as h:
         h.Analyze(n, cp, ri)
         h.Factorize(A)
@@ -124,13 +130,14 @@ def test_spd_mixed_precision_refines(oracle_lib, penalty_factor):
         st = h.stats()
         print(f"penalty {penalty_factor:.0e}: fallbacks after Factorize {fb}, after Solve {st['fp64_fallbacks']}, "
               f"refinement iterations {st['refine_iterations']}")
-        # FP64 refinement reaches the FP64 factor's own backward error (~1e-9 on these penalty systems)
-        assert np.linalg.norm(As @ x - b) <= 1e-8 * np.linalg.norm(b)
+        res = np.linalg.norm(As @ x - b) / np.linalg.norm(b)
+        print(f"  relative residual {res:.2e}, plain FP64 factor {res64:.2e}")
+        assert res <= max(1e-10, 10 * res64)
         assert st["refine_iterations"] >= 1
         if penalty_factor == 1e4:
             assert fb == 0 and st["fp64_fallbacks"] == 0, st
         else:
-            assert fb == 1 and st["fp64_fallbacks"] == 1, st
+            assert fb == 1 and st["fp64_fallbacks"] == 2, st
 
 
 @pytest.mark.parametrize("precision", [0, 1, 2])
