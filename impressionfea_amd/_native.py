@@ -26,7 +26,8 @@ MLP_SYMBOLS = ["ifea_mlp_create", "ifea_mlp_num_params", "ifea_mlp_set_params", 
 STEP_SYMBOLS = ["ifea_step_create", "ifea_step_set_params", "ifea_step_set_perm", "ifea_step_prepare",
                 "ifea_step_set_element_state", "ifea_step_assemble", "ifea_step_solve_velocity", "ifea_step_final",
                 "ifea_step_advance", "ifea_step_element_results", "ifea_step_set_damage", "ifea_step_get_v",
-                "ifea_step_set_v", "ifea_step_quality", "ifea_step_info", "ifea_step_destroy"]
+                "ifea_step_set_v", "ifea_step_quality", "ifea_step_quality_noalloc", "ifea_step_info",
+                "ifea_step_destroy"]
 
 IFEA_OK = 0
 IFEA_ERR_ARG = -1
@@ -110,6 +111,7 @@ def lib():
     L.ifea_step_get_v.argtypes = [vp, dp]
     L.ifea_step_set_v.argtypes = [vp, dp]
     L.ifea_step_quality.argtypes = [vp, dp, C.c_double, ip, dp]
+    L.ifea_step_quality_noalloc.argtypes = [C.c_int, C.c_int, ip, dp, C.c_double, ip, dp]
     L.ifea_step_info.argtypes = [vp, C.c_int, dp]
     L.ifea_step_destroy.argtypes = [vp]
     L.ifea_step_destroy.restype = None

@@ -373,6 +373,14 @@ int ifea_step_quality(ifea_step_t h, const double* x, double qmin, int* first_ba
   IFEA_GUARD(*first_bad = h->dev->quality(x, qmin, quality))
 }
 
+int ifea_step_quality_noalloc(int nn, int ne, const int* tets, const double* x, double qmin, int* first_bad,
+                              double* quality) {
+  if (!tets || !x || !first_bad) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  int rc = ensure_device();
+  if (rc) return rc;
+  IFEA_GUARD(*first_bad = quality_noalloc(nn, ne, tets, x, qmin, quality))
+}
+
 int ifea_step_info(ifea_step_t h, int reset, double* out24) {
   if (!h || !out24) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
   IFEA_GUARD(h->dev->info(reset, out24))

@@ -12,6 +12,9 @@
 
 namespace ifea {
 
+// TryUpdate's collapse check on host arrays through the library's static scratch array: no device allocation.
+int quality_noalloc(int nn, int ne, const int* tets, const double* x, double qmin, double* q);
+
 class StepDevice {
  public:
   StepDevice(int nn, int ne, const int* tets, const int* colPtr, const int* rowIdx, int nf, const int* faces,

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "step_device.h"
 
@@ -1172,6 +1173,75 @@ int StepDevice::quality(const double* x, double qmin, double* q) {
   if (q) IFEA_HIP(hipMemcpyAsync(q, qual_.p, ne_ * sizeof(double), hipMemcpyDeviceToHost, s_));
   IFEA_HIP(hipStreamSynchronize(s_));
   return first == big ? -1 : first;
+}
+
+// ------------------------------------------------------------------------------- quality check without allocation
+// The collapse check of TryUpdate for callers that cannot allocate device memory: the caller's tets and positions are
+// staged, the first-index counter kept and the per-tet values returned through one file-scope device array. Each GPU
+// holds its own copy of the array (NOALLOC_SCRATCH_BYTES) from module load until the process ends; every call on a
+// GPU shares it, so calls that overlap on one GPU race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int MAX_DEVICES = 64;
+unsigned char* g_scratch_ptr[MAX_DEVICES] = {nullptr};
+
+// Device address of this GPU's scratch array, looked up once per device with that device current. The array's name
+// is only a host-side stand-in: host code uses this address, never the name.
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  IFEA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) throw Error(-6, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    IFEA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(-6, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+int quality_noalloc(int nn, int ne, const int* tets, const double* x, double qmin, double* q) {
+  if (nn <= 0 || ne <= 0 || !tets || !x) throw Error(-1, "empty mesh");
+  // layout: [0, 256) the counter; tets (4 ne ints); positions (3 nn doubles); per-tet values (ne doubles, if asked)
+  const size_t o_t = 256;
+  const size_t o_x = o_t + align256(4 * (size_t)ne * sizeof(int));
+  const size_t o_q = o_x + align256(3 * (size_t)nn * sizeof(double));
+  const size_t end = o_q + (q ? align256((size_t)ne * sizeof(double)) : 0);
+  if (end > NOALLOC_SCRATCH_BYTES)
+    throw Error(-1, "mesh of " + std::to_string(nn) + " nodes and " + std::to_string(ne) + " tets needs " +
+                        std::to_string(end) + " bytes of scratch; the allocation-free quality check holds at most " +
+                        std::to_string(NOALLOC_SCRATCH_BYTES));
+  for (size_t k = 0; k < 4 * (size_t)ne; k++)
+    if (tets[k] < 0 || tets[k] >= nn) throw Error(-1, "tet node index out of range");
+  unsigned char* base = noalloc_scratch();
+  int* first = (int*)base;
+  int* d_tets = (int*)(base + o_t);
+  double* d_x = (double*)(base + o_x);
+  double* d_q = q ? (double*)(base + o_q) : nullptr;
+  const int big = 0x7fffffff;
+  hipStream_t s = 0;
+  // the counter starts at "none" for every call (a fill kernel, so the sequence can also be captured)
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, first, big);
+  IFEA_HIP(hipMemcpyAsync(d_tets, tets, 4 * (size_t)ne * sizeof(int), hipMemcpyHostToDevice, s));
+  IFEA_HIP(hipMemcpyAsync(d_x, x, 3 * (size_t)nn * sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_quality, dim3(grid_for(ne, 256)), dim3(256), 0, s, ne, d_tets, d_x, qmin, d_q, first);
+  IFEA_HIP(hipGetLastError());
+  int h_first = big;
+  IFEA_HIP(hipMemcpyAsync(&h_first, first, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (q) IFEA_HIP(hipMemcpyAsync(q, d_q, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost, s));
+  IFEA_HIP(hipStreamSynchronize(s));
+  return h_first == big ? -1 : h_first;
 }
 
 void StepDevice::get_v(double* v) {

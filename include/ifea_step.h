@@ -109,6 +109,15 @@ int ifea_step_set_v(ifea_step_t h, const double* v);
  * receives every tet's value when not NULL. */
 int ifea_step_quality(ifea_step_t h, const double* x, double qmin, int* first_bad, double* quality);
 
+/* ifea_step_quality without a handle and without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). tets[4ne] and x[3nn] are host arrays, staged with the first-index counter and the per-tet values
+ * in one static 64 MiB device array per GPU (held from library load until the process ends): a mesh needing more
+ * (16 ne + 24 nn + 8 ne bytes with quality, rounded up to 256-byte blocks, plus 256) returns IFEA_ERR_ARG. Calls
+ * that overlap on one GPU share that array and race. Results as ifea_step_quality. */
+int ifea_step_quality_noalloc(int nn, int ne, const int* tets, const double* x, double qmin, int* first_bad,
+                              double* quality);
+
 /* Device and factor statistics (24 doubles): [0] nnz(L), [1] factor flops, [2] supernodes, [3] device bytes,
  * [4] last factor ms, [5] L words, [6] update-pool words, [7] levels; with params.profile set, accumulated since the
  * last call with reset: [8] inner (64-column) MFMA update launches, [9] their ms, [10] their algorithmic flops,
