@@ -2,16 +2,22 @@
 
 Workload (default): the closed-die hex-head upset, billet D 30 x H 45 mm into a hexagonal cavity (across-flats
 38 mm, 2 mm flash land), 42x42x64 Kuhn cells = 120,185 nodes / 677,376 tets / 360,555 DOF, synthetic seeded mesh.
-Two arms run from the same initial state:
-  fp64   plain FP64 supernodal factor, previous-increment warm start (the reference's default)
-  mixed  FP32 MFMA supernode updates + FP64 POTRF/residual/PCG refinement (FP64 fallback on stall), with the learned
-         warm start when a trained network is available, else the previous-increment start
+Four arms run from the same initial state:
+  fp64            plain FP64 supernodal factor, previous-increment start, PCG on the cached factor at every solve: the
+                  reference's algorithm (Forging3DSolver.cs:854-1019) on the GPU
+  fp64_trend      the same with the trend start 2 v_n - v_{n-1} (the reference's ExtrapolateStart option)
+  mixed_previous  mixed precision (FP32 MFMA factor, or at a penalty beyond FP32 the FP64 factor with an FP32 copy for
+                  the triangular sweeps; FP64 residual and PCG refinement) with the previous-increment start
+  mixed           the full method: mixed precision, the learned start judged by the solver's energy against the
+                  previous and trend starts, and the adaptive first solve (an increment's first solve refactors
+                  directly once PCG on the previous increment's factor has missed its budget twice in a row)
 A "step" is one stroke increment: PrepareStep (frames on the host, geometry/ANP/friction on the GPU) + SolveVelocity +
-FinalEvaluation + strain update, followed by the host geometry update (TryUpdate), which is identical in both arms.
+FinalEvaluation + strain update, followed by the host geometry update (TryUpdate), which is identical in every arm.
 `value` is the mixed arm's increments per second of wall clock over the timed steps (TryUpdate included; the rate
-without it is reported too); the fp64 arm, the load error between the arms, the roofline of the dominant kernel (the
-outer MFMA supernode update, HIP events around every launch on the library's stream) and a host CPU baseline (the
-FP64 oracle with MKL PARDISO on the host cores, on a bounded sample) are reported beside it.
+without it is reported too); the other arms, every arm's load error against fp64, the precision each arm actually
+computed in, the roofline of the dominant kernel (the outer MFMA supernode update, HIP events around every launch on
+the library's stream) and a host CPU baseline (the FP64 oracle with MKL PARDISO on the host cores, timed piecewise on a
+bounded sample and scaled by the fp64 arm's per-increment counts: a model, labelled as such) are reported beside it.
 """
 from __future__ import annotations
 
@@ -34,9 +40,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
+# The arms: (factor precision, start, adaptive first-solve policy). "fp64" is the reference's algorithm on the GPU;
+# the two middle arms separate the effect of the start from that of the precision; "mixed" is the full method.
+ARMS = {
+    "fp64": dict(precision=0, start="previous", first_solve=False),
+    "fp64_trend": dict(precision=0, start="trend", first_solve=False),
+    "mixed_previous": dict(precision=1, start="previous", first_solve=False),
+    "mixed": dict(precision=1, start="learned", first_solve=True),
+}
+
+
+def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True, start="previous", first_solve=False):
+    from dataclasses import replace
     from impressionfea_amd.forging3d import Forging3DSolver
-    s = Forging3DSolver.from_workload(wl, precision=precision, warm_start=warm_start)
+    if start == "trend":
+        wl = replace(wl, cfg=replace(wl.cfg, extrapolate_start=True))
+    s = Forging3DSolver.from_workload(wl, precision=precision, warm_start=warm_start if start == "learned" else None,
+                                      first_solve_policy=first_solve)
     s.set_params(profile=profile)
     loads, ms_solve, ms_update, stats, all_loads = [], [], [], [], []
     for k in range(warmup):
@@ -55,7 +75,8 @@ def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True):
         stats.append(r.stats)
         log(f"  step {k}: load {r.load:.6e} N it {r.iterations} fact {r.stats['factorizations']} "
             f"pcg {r.stats['pcg_iterations']} solves {r.stats['solves']} fb {r.stats['fp64_fallbacks']} "
-            f"start {r.stats.get('start')} {r.ms_solve:.0f} ms (+{r.ms_update:.0f} ms update)")
+            f"skip {r.stats['first_solve_skips']} start {r.stats.get('start')} {r.ms_solve:.0f} ms "
+            f"(+{r.ms_update:.0f} ms update)")
     wall = time.perf_counter() - t0  # every step returns after its device work (results are read back)
     info = s.info(reset=True)
     used = device_memory_used()
@@ -140,6 +161,9 @@ def cpu_baseline(wl, gpu_stats, threads, nnz_gpu=0):
     except (OSError, StopIteration):
         pass
     return dict(value=1.0 / per_inc, unit="increments/s", cores=threads, kind="port", cpu=model,
+                method=("modelled: the host's piecewise times of one increment's parts, multiplied by the fp64 arm's "
+                        "per-increment counts (an end-to-end oracle increment with MKL PARDISO takes longer: "
+                        "tests/golden/make_hexdie_loads.py records whole increments)"),
                 sample=(f"{wl.name} {3 * x.shape[0]:,} DOF, one increment's work measured piecewise on the host: PrepareStep "
                         f"{t_prep:.2f}s, tangent assembly {t_asm:.2f}s, residual assembly {t_res:.2f}s, MKL PARDISO "
                         f"analyse {t_an:.1f}s (not per increment) / factor {t_fac:.2f}s ({mkl_counts}) / solve "
@@ -205,7 +229,9 @@ def main():
                     help="hexdie: the 360k-DOF hex-head closed die (the headline line); conrod: the ~1M-DOF "
                          "connecting-rod closed die (peak device memory against the 288 GB of one MI355X); upset: "
                          "the small upset")
-    ap.add_argument("--arms", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--arms", default="all",
+                    help="comma-separated arms out of " + ", ".join(ARMS) + "; 'all' (default) runs every arm, "
+                         "'both' only fp64 and mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-graph", action="store_true",
@@ -231,22 +257,36 @@ def main():
     wl = {"hexdie": hexdie_production, "conrod": conrod_production, "upset": upset_small}[args.workload]()
     ws = warmstart.load_default()
     log(f"workload {wl.name}: {wl.nxy}x{wl.nxy}x{wl.nz} cells; learned warm start: {'yes' if ws else 'no'}")
+    arms = list(ARMS) if args.arms == "all" else (["fp64", "mixed"] if args.arms == "both" else args.arms.split(","))
+    for a in arms:
+        if a not in ARMS:
+            sys.exit(f"unknown arm {a!r}; choose from {', '.join(ARMS)}")
     res = {}
-    if args.arms in ("both", "fp64"):
-        log("arm fp64")
-        res["fp64"] = run_arm(wl, 0, args.steps, args.warmup)
-    if args.arms in ("both", "mixed"):
-        log("arm mixed")
-        res["mixed"] = run_arm(wl, 1, args.steps, args.warmup, warm_start=ws)
-    head = res.get("mixed") or res["fp64"]
+    for a in arms:
+        spec = ARMS[a]
+        log(f"arm {a}: precision {spec['precision']}, {spec['start']} start, adaptive first solve "
+            f"{'on' if spec['first_solve'] else 'off'}")
+        res[a] = run_arm(wl, spec["precision"], args.steps, args.warmup, warm_start=ws, start=spec["start"],
+                         first_solve=spec["first_solve"])
+    head = res.get("mixed") or res[arms[-1]]
     ips = {k: len(v["ms_solve"]) / v["wall"] for k, v in res.items()}                  # whole increments, wall clock
     ips_solve = {k: len(v["ms_solve"]) / (sum(v["ms_solve"]) / 1e3) for k, v in res.items()}  # without TryUpdate
-    value = ips.get("mixed", ips.get("fp64"))
+    value = ips["mixed"] if "mixed" in ips else ips[arms[-1]]
     ms_step = 1e3 / value
-    load_err = None
-    if "mixed" in res and "fp64" in res:
-        a, b = np.array(res["fp64"]["loads"]), np.array(res["mixed"]["loads"])
-        load_err = float(np.max(np.abs(b - a) / np.abs(a)))
+    load_err = {}
+    if "fp64" in res:
+        a = np.array(res["fp64"]["loads"])
+        for k, v in res.items():
+            if k != "fp64":
+                load_err[k] = float(np.max(np.abs(np.array(v["loads"]) - a) / np.abs(a)))
+    # the arithmetic the timed arms actually used: factorizations and triangular sweeps by precision
+    precision_used = {k: dict(fp32_factorizations=v["info"]["fp32_factorizations"],
+                              fp64_factorizations=v["info"]["factorizations"] - v["info"]["fp32_factorizations"],
+                              fp32_sweeps=v["info"]["fp32_sweeps"],
+                              fp64_sweeps=v["info"]["sweeps"] - v["info"]["fp32_sweeps"]) for k, v in res.items()}
+    hp = precision_used["mixed" if "mixed" in res else arms[-1]]
+    dtype = ("fp32 factor + fp64" if hp["fp32_factorizations"] else "fp64") + (
+        " (fp32 sweeps)" if hp["fp32_sweeps"] else "")
     # roofline of the dominant kernel: the outer supernode update k_syrk_big (MFMA), from HIP events around every
     # launch on the library's stream; split by the precision the factor was actually computed in
     info = head["info"]
@@ -308,14 +348,22 @@ def main():
         "metric": metric,
         "value": round(value, 4), "unit": "increments/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_step, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32+fp64" if "mixed" in res else "fp64", "data": "synthetic",
+        "dtype": dtype, "data": "synthetic",
         "config": {"workload": desc, "dof": ndof,
                    "arms": {k: round(v, 4) for k, v in ips.items()},
+                   "arm_definitions": {k: ARMS[k] for k in res},
                    "arms_without_host_update": {k: round(v, 4) for k, v in ips_solve.items()},
-                   "speedup_mixed_vs_fp64": round(ips["mixed"] / ips["fp64"], 3) if len(ips) == 2 else None,
+                   "speedup_vs_fp64": ({k: round(v / ips["fp64"], 3) for k, v in ips.items() if k != "fp64"}
+                                       if "fp64" in ips else None),
                    "newton_iterations": {k: [st["iterations"] for st in v["stats"]] for k, v in res.items()},
-                   "starts": {k: [st.get("start") for st in v["stats"]] for k, v in res.items()},
-                   "load_rel_err_mixed_vs_fp64": load_err,
+                   "per_increment_means": {k: {q: round(float(np.mean([st[q] for st in v["stats"]])), 2)
+                                               for q in ("factorizations", "solves", "pcg_iterations",
+                                                         "first_solve_skips", "fp64_fallbacks")}
+                                           for k, v in res.items()},
+                   "starts": {k: {nm: sum(st.get("start") == nm for st in v["stats"])
+                                  for nm in ("previous", "trend", "learned")} for k, v in res.items()},
+                   "precision_used": precision_used,
+                   "load_rel_err_vs_fp64": load_err,
                    "learned_warm_start": bool(ws),
                    "nnz_L": info["nnz_l"], "factor_flops": info["flops"],
                    "ms_update_host": round(float(np.mean(head["ms_update"])), 1),

@@ -23,8 +23,8 @@ SPD_SYMBOLS = ["ifea_spd_create", "ifea_spd_name", "ifea_spd_analyze", "ifea_spd
 MLP_SYMBOLS = ["ifea_mlp_create", "ifea_mlp_num_params", "ifea_mlp_set_params", "ifea_mlp_get_params",
                "ifea_mlp_forward", "ifea_mlp_loss_grad", "ifea_mlp_set_dataset", "ifea_mlp_train_epoch",
                "ifea_mlp_destroy"]
-STEP_SYMBOLS = ["ifea_step_create", "ifea_step_set_params", "ifea_step_set_perm", "ifea_step_prepare",
-                "ifea_step_set_element_state", "ifea_step_assemble", "ifea_step_solve_velocity", "ifea_step_final",
+STEP_SYMBOLS = ["ifea_step_create", "ifea_step_set_params", "ifea_step_set_option", "ifea_step_set_perm",
+                "ifea_step_prepare", "ifea_step_set_element_state", "ifea_step_assemble", "ifea_step_solve_velocity", "ifea_step_final",
                 "ifea_step_advance", "ifea_step_element_results", "ifea_step_set_damage", "ifea_step_get_v",
                 "ifea_step_set_v", "ifea_step_quality", "ifea_step_quality_noalloc", "ifea_step_info",
                 "ifea_step_destroy"]
@@ -42,6 +42,8 @@ IFEA_OPT_PRECISION = 1
 IFEA_OPT_REFINE_TOL = 2
 IFEA_OPT_REFINE_MAXIT = 3
 IFEA_OPT_LEAF_BLOCKS = 4
+
+IFEA_STEP_OPT_FIRST_SOLVE = 1
 
 
 class StepParams(C.Structure):
@@ -62,6 +64,7 @@ class StepStats(C.Structure):
         ("pcg_iterations", C.c_int), ("solves", C.c_int), ("assemblies", C.c_int),
         ("last_rel", C.c_double), ("last_rnorm", C.c_double),
         ("ms_assembly", C.c_double), ("ms_factor", C.c_double), ("ms_solve", C.c_double), ("ms_total", C.c_double),
+        ("first_solve_skips", C.c_int),
     ]
 
     def as_dict(self):
@@ -99,6 +102,7 @@ def lib():
     L.ifea_spd_set_coordinates.argtypes = [vp, C.c_int, dp]
     L.ifea_step_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, ip, ip, ip, C.c_int, ip, ip]
     L.ifea_step_set_params.argtypes = [vp, C.POINTER(StepParams)]
+    L.ifea_step_set_option.argtypes = [vp, C.c_int, C.c_double]
     L.ifea_step_set_perm.argtypes = [vp, ip]
     L.ifea_step_prepare.argtypes = [vp, dp, dp, ip, dp, ip, dp]
     L.ifea_step_set_element_state.argtypes = [vp, dp, dp]

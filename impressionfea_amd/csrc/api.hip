@@ -310,6 +310,11 @@ int ifea_step_set_params(ifea_step_t h, const ifea_step_params* p) {
   IFEA_GUARD(h->dev->set_params(*p))
 }
 
+int ifea_step_set_option(ifea_step_t h, int key, double value) {
+  if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->set_option(key, value))
+}
+
 int ifea_step_set_perm(ifea_step_t h, const int* perm) {
   if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
   IFEA_GUARD(h->dev->set_perm(perm))

@@ -22,6 +22,7 @@ class StepDevice {
   ~StepDevice();
   void set_params(const ifea_step_params& p) { P_ = p; have_params_ = true; spd_->set_profiling(p.profile != 0); }
   void set_perm(const int* perm);
+  void set_option(int key, double value);
   void prepare(const double* x, const double* ax, const int* nfix, const double* fixed_val, const int* contact,
                const double* vguess);
   void set_element_state(const double* eps, const double* sigma);
@@ -61,6 +62,10 @@ class StepDevice {
   int pcg_failures_ = 0, pcg_skip_ = 0;
   bool fp32_unstable_ = false;  // an FP32 factorisation was not positive definite: factor in FP64 from now on
   bool fp32_sweeps_unstable_ = false;  // PCG on a fresh FP64 factor failed with FP32 sweeps: FP64 sweeps from now on
+  int opt_first_solve_ = 0;            // IFEA_STEP_OPT_FIRST_SOLVE
+  bool first_solve_ = false;           // the next solve_linear is the first of an increment
+  int first_fail_streak_ = 0;          // increments in a row whose first solve missed the PCG budget
+  int first_skips_ = 0;                // first solves refactored directly since the last PCG probe
   // topology (device)
   DBuf<int> tets_, nt_start_, nt_list_, colPtr_, rowIdx_, nbr_start_, nbr_, p_start_, p_node_, diagpos_;
   DBuf<int> nf_start_, nf_list_, faces_, face_owner_;

@@ -840,6 +840,11 @@ StepDevice::~StepDevice() {
   if (s_) (void)hipStreamDestroy(s_);
 }
 
+void StepDevice::set_option(int key, double value) {
+  if (key == IFEA_STEP_OPT_FIRST_SOLVE) { opt_first_solve_ = value != 0 ? 1 : 0; return; }
+  throw Error(-1, "unknown step option key " + std::to_string(key));
+}
+
 void StepDevice::set_perm(const int* perm) {
   if (perm) user_perm_.assign(perm, perm + ndof_);
   else user_perm_.clear();
@@ -992,10 +997,23 @@ int StepDevice::pcg(const double* b, double* x, double tol, int maxIt, ifea_step
 
 // SolveLinear (:965-1019) with the mixed-precision refinement slot
 int StepDevice::solve_linear(double* b, double* x, ifea_step_stats* st) {
-  if (have_factor_ && pcg_skip_ > 0) {
+  const bool first = first_solve_;
+  first_solve_ = false;
+  // IFEA_STEP_OPT_FIRST_SOLVE: the first solve of an increment meets the previous increment's factor, which the
+  // changed contact set and frames make a poor preconditioner; after it failed the PCG budget at the first solves of
+  // two increments in a row, the first solve refactors directly, probing PCG again after 8 skipped increments.
+  const bool skip_first = first && opt_first_solve_ && have_factor_ && first_fail_streak_ >= 2 && first_skips_ < 8;
+  if (skip_first) {
+    first_skips_++;
+    if (st) st->first_solve_skips++;
+  } else if (have_factor_ && pcg_skip_ > 0) {
     pcg_skip_--;
   } else if (have_factor_) {
     int its = pcg(b, x, P_.pcg_tol, P_.pcg_budget, st);
+    if (first) {
+      first_fail_streak_ = its >= 0 ? 0 : first_fail_streak_ + 1;
+      first_skips_ = 0;
+    }
     if (its >= 0) { pcg_failures_ = 0; return 0; }
     if (++pcg_failures_ >= 2) pcg_skip_ = std::min(64, 1 << (pcg_failures_ - 1));
   }
@@ -1039,6 +1057,7 @@ void StepDevice::solve_velocity(int warm, ifea_step_stats* st) {
   int halved = 0;
   double last_rel = INFINITY;
   int it;
+  first_solve_ = true;
   const int nlim = P_.newton_limit > 0 ? P_.newton_limit : 20;
   for (it = 1; it <= P_.max_iterations; it++) {
     bool newton = (warm || it > 2) && it <= nlim && !(P_.adaptive_newton && halved >= 2);

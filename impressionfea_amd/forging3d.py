@@ -123,7 +123,7 @@ class Forging3DSolver:
     """One billet on a fixed mesh, stepped by increments on the GPU (Forging3DSolver.Step without remeshing)."""
 
     def __init__(self, cfg: StepConfig, x, tets, onX0, onY0, gap0: float, precision: int = 0, perm=None,
-                 warm_start=None):
+                 warm_start=None, first_solve_policy: bool = False):
         self.cfg = cfg
         self.x = np.array(x, dtype=np.float64)
         self.mesh = TetMesh(self.x.shape[0], tets, np.asarray(onX0, bool), np.asarray(onY0, bool))
@@ -170,6 +170,8 @@ class Forging3DSolver:
         if perm is not None:
             pm = N.i32(perm)
             self._check(L.ifea_step_set_perm(self._h, N.ptr(pm, C.c_int)), "set_perm")
+        if first_solve_policy:
+            self._check(L.ifea_step_set_option(self._h, N.IFEA_STEP_OPT_FIRST_SOLVE, 1.0), "set_option")
         # state
         self.contact = np.zeros(self.nn, dtype=np.int32)
         self.eps = np.zeros(self.ne)
@@ -382,7 +384,8 @@ class Forging3DSolver:
         else:
             self.prepare()
         st = self.solve_velocity()
-        st["start"] = judged or "previous"
+        trend = cfg.extrapolate_start and self.warm and self.vg_prev is not None
+        st["start"] = judged or ("trend" if trend else "previous")
         if not st["converged"] and st["last_rel"] >= 1e-2:
             raise SolverDivergenceError(f"Velocity solution did not converge at step {self.step_index}: "
                                         f"|dv|/|v| = {st['last_rel']:.2E}.")

@@ -50,7 +50,15 @@ typedef struct ifea_step_stats {
   int factorizations, fp64_fallbacks, pcg_calls, pcg_iterations, solves, assemblies;
   double last_rel, last_rnorm;
   double ms_assembly, ms_factor, ms_solve, ms_total;
+  int first_solve_skips;     /* first solves that refactored without trying PCG (IFEA_STEP_OPT_FIRST_SOLVE) */
 } ifea_step_stats;
+
+/* Option keys for ifea_step_set_option. */
+#define IFEA_STEP_OPT_FIRST_SOLVE 1  /* 0 (default): every solve with a cached factor tries PCG first, as
+                                        SolveLinear does (Forging3DSolver.cs:965-1019). 1: after the first solve of
+                                        two increments in a row missed the PCG budget on the previous increment's
+                                        factor, an increment's first solve refactors directly (PCG is probed again
+                                        after 8 such increments); later solves keep the reference policy. */
 
 /* Topology and pattern, once per mesh (Allocate/BuildPattern, Forging3DSolver.cs:403-521, TetMesh.cs:9-64).
  * tets[4*ne]; colPtr/rowIdx: the full symmetric CSC two-ring pattern (3x3 node blocks, sorted);
@@ -58,6 +66,9 @@ typedef struct ifea_step_stats {
 int ifea_step_create(ifea_step_t* h, int nn, int ne, const int* tets, const int* colPtr, const int* rowIdx,
                      int nf, const int* faces, const int* face_owner);
 int ifea_step_set_params(ifea_step_t h, const ifea_step_params* p);
+
+/* Solver policy knobs that are not part of the reference's parameter set (IFEA_STEP_OPT_*). */
+int ifea_step_set_option(ifea_step_t h, int key, double value);
 
 /* Optional caller ordering for the factor (see ifea_spd_analyze_perm). */
 int ifea_step_set_perm(ifea_step_t h, const int* perm);
