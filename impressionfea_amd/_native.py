@@ -24,7 +24,8 @@ MLP_SYMBOLS = ["ifea_mlp_create", "ifea_mlp_num_params", "ifea_mlp_set_params", 
                "ifea_mlp_forward", "ifea_mlp_loss_grad", "ifea_mlp_set_dataset", "ifea_mlp_train_epoch",
                "ifea_mlp_destroy"]
 STEP_SYMBOLS = ["ifea_step_create", "ifea_step_set_params", "ifea_step_set_option", "ifea_step_set_perm",
-                "ifea_step_prepare", "ifea_step_set_element_state", "ifea_step_assemble", "ifea_step_solve_velocity", "ifea_step_final",
+                "ifea_step_prepare", "ifea_step_set_geometry", "ifea_step_prepare_device", "ifea_step_load_guess",
+                "ifea_step_get_frames", "ifea_step_set_element_state", "ifea_step_assemble", "ifea_step_solve_velocity", "ifea_step_final",
                 "ifea_step_advance", "ifea_step_element_results", "ifea_step_set_damage", "ifea_step_get_v",
                 "ifea_step_set_v", "ifea_step_quality", "ifea_step_quality_noalloc", "ifea_step_info",
                 "ifea_step_destroy"]
@@ -105,6 +106,10 @@ def lib():
     L.ifea_step_set_option.argtypes = [vp, C.c_int, C.c_double]
     L.ifea_step_set_perm.argtypes = [vp, ip]
     L.ifea_step_prepare.argtypes = [vp, dp, dp, ip, dp, ip, dp]
+    L.ifea_step_set_geometry.argtypes = [vp, u8p, u8p, C.c_int, dp, C.c_int, dp]
+    L.ifea_step_prepare_device.argtypes = [vp, dp, ip, dp]
+    L.ifea_step_load_guess.argtypes = [vp, dp]
+    L.ifea_step_get_frames.argtypes = [vp, dp, ip, ip, dp]
     L.ifea_step_set_element_state.argtypes = [vp, dp, dp]
     L.ifea_step_assemble.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, dp, dp]
     L.ifea_step_solve_velocity.argtypes = [vp, C.c_int, C.POINTER(StepStats)]

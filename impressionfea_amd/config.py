@@ -43,8 +43,10 @@ class DieSpec:
             t, db_, ds_, df_, dw_, w_, wall, land = self.p
             xb, rb, xs, rs, g = self.CONROD
             fl = 0.25 * w_
-            db = np.hypot(x - xb, y) - rb
-            ds = np.hypot(x - xs, y) - rs
+            # distances as sqrt of the sum of squares (not hypot): the same IEEE operations in the oracle and on the
+            # device, so the three evaluations of the die agree bit for bit
+            db = np.sqrt((x - xb) * (x - xb) + y * y) - rb
+            ds = np.sqrt((x - xs) * (x - xs) + y * y) - rs
             dsh = np.maximum(np.maximum(xb - x, x - xs), np.abs(y) - 0.5 * w_)
             shank = dw_ + (df_ - dw_) * smooth((np.abs(y) - (0.5 * w_ - fl) + 0.5 * wall) / wall)
             depth = np.maximum(np.maximum(db_ * smooth((-db + 0.5 * wall) / wall),
@@ -69,7 +71,7 @@ class DieSpec:
         hx = (self.height(x + dd, y) - self.height(x - dd, y)) / (2 * dd)
         hy = (self.height(x, y + dd) - self.height(x, y - dd)) / (2 * dd)
         up = np.stack([-hx, -hy, np.ones_like(hx)], axis=-1)
-        up = up / np.linalg.norm(up, axis=-1, keepdims=True)
+        up = up / np.sqrt(up[..., 0] * up[..., 0] + up[..., 1] * up[..., 1] + up[..., 2] * up[..., 2])[..., None]
         nrm = up if self.side == 0 else -up
         p = np.stack([x, y, self.height(x, y) + off], axis=-1)
         return p, nrm

@@ -326,6 +326,27 @@ int ifea_step_prepare(ifea_step_t h, const double* x, const double* ax, const in
   IFEA_GUARD(h->dev->prepare(x, ax, nfix, fixed_val, contact, vguess))
 }
 
+int ifea_step_set_geometry(ifea_step_t h, const unsigned char* on_x0, const unsigned char* on_y0, int bottom_kind,
+                           const double* bottom_p, int top_kind, const double* top_p) {
+  if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->set_geometry(on_x0, on_y0, bottom_kind, bottom_p, top_kind, top_p))
+}
+
+int ifea_step_prepare_device(ifea_step_t h, const double* x, const int* contact, const double* vguess) {
+  if (!h || !x || !contact || !vguess) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->prepare_device(x, contact, vguess))
+}
+
+int ifea_step_load_guess(ifea_step_t h, const double* vguess) {
+  if (!h || !vguess) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->load_guess(vguess))
+}
+
+int ifea_step_get_frames(ifea_step_t h, double* ax, int* nfix, int* die_axis, double* fixed_val) {
+  if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
+  IFEA_GUARD(h->dev->get_frames(ax, nfix, die_axis, fixed_val))
+}
+
 int ifea_step_set_element_state(ifea_step_t h, const double* eps, const double* sigma) {
   if (!h) { set_last_error("invalid argument"); return IFEA_ERR_ARG; }
   IFEA_GUARD(h->dev->set_element_state(eps, sigma))

@@ -12,6 +12,12 @@
 
 namespace ifea {
 
+// An analytic rigid die as the device evaluates it (kind 0 flat, 1 hex cavity, 2 con-rod half; side 0 bottom, 1 top)
+struct DieDev {
+  int kind = 0, side = 0;
+  double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
 // TryUpdate's collapse check on host arrays through the library's static scratch array: no device allocation.
 int quality_noalloc(int nn, int ne, const int* tets, const double* x, double qmin, double* q);
 
@@ -26,6 +32,13 @@ class StepDevice {
   void prepare(const double* x, const double* ax, const int* nfix, const double* fixed_val, const int* contact,
                const double* vguess);
   void set_element_state(const double* eps, const double* sigma);
+  // PrepareStep with the nodal frames computed on the device from the symmetry flags and dies of set_geometry
+  void set_geometry(const unsigned char* on_x0, const unsigned char* on_y0, int bottom_kind, const double* bottom_p,
+                    int top_kind, const double* top_p);
+  void prepare_device(const double* x, const int* contact, const double* vguess);
+  // LoadVelocityGuess with the current frames: v = R vg, fixed DOFs overwritten
+  void load_guess(const double* vguess);
+  void get_frames(double* ax, int* nfix, int* die_axis, double* fixed_val);
   // mode: 0 raw residual, 1 BC + negated residual (linear system), 2 energy only
   double assemble(bool matrix, bool newton, const double* d_v, int mode, double* power);
   void assemble_host(int matrix, int newton, const double* v, double* energy, double* power, double* r, double* val);
@@ -46,6 +59,7 @@ class StepDevice {
   int pcg(const double* b, double* x, double tol, int maxIt, ifea_step_stats* st);
   void lin_solve(const double* b, double* x, ifea_step_stats* st);
   int refactor(int precision, ifea_step_stats* st);
+  void prepare_rest(const double* x, const double* vguess);
 
   hipStream_t s_ = nullptr;
   std::unique_ptr<Vec> vec_;
@@ -71,7 +85,10 @@ class StepDevice {
   DBuf<int> nf_start_, nf_list_, faces_, face_owner_;
   // per increment
   DBuf<double> x_, ax_, fixed_val_, grad_, vol_, vnode_, gcoef_, kvol_, afric_, kfric_;
-  DBuf<int> nfix_, contact_, bad_;
+  DBuf<int> nfix_, contact_, bad_, die_axis_;
+  DBuf<unsigned char> on_x0_, on_y0_;
+  DieDev bottom_, top_;
+  bool have_geometry_ = false, device_frames_ = false;
   // state
   DBuf<double> eps_, el_sig_, el_rate_, el_dev_, el_mean_, dmg_, nodal_;
   // iteration

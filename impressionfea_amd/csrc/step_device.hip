@@ -188,6 +188,157 @@ __global__ void k_fric(int nn, const int* __restrict__ nfs, const int* __restric
   kfric[n] = cnt > 0 ? ks / cnt : 0.0;
 }
 
+// ------------------------------------------------------------------------------- PrepareStep frames (:669-727)
+// Analytic rigid dies as height fields (the host's DieSpec and the oracle's Die::height): kind 0 a flat plane at
+// z = p[0], kind 1 a top die with a hexagonal cavity, kind 2 one half of a connecting-rod closed die. The frames are
+// evaluated with the same IEEE operations in the same order as on the host (no FMA contraction in these functions),
+// so they agree bit for bit.
+__device__ double die_smooth(double u) {
+#pragma clang fp contract(off)
+  u = fmin(1.0, fmax(0.0, u));
+  return u * u * (3 - 2 * u);
+}
+
+__device__ double die_height(const DieDev& D, double x, double y) {
+#pragma clang fp contract(off)
+  const double* p = D.p;
+  if (D.kind == 0) return p[0];
+  if (D.kind == 2) {
+    const double xb = -42.0, rb = 20.0, xs = 49.5, rs = 12.5, g = 5.0;
+    const double t = p[0], wall = p[6], land = p[7], w = p[5], fl = 0.25 * w;
+    const double db = sqrt((x - xb) * (x - xb) + y * y) - rb, ds = sqrt((x - xs) * (x - xs) + y * y) - rs;
+    const double dsh = fmax(fmax(xb - x, x - xs), fabs(y) - 0.5 * w);
+    const double shank = p[4] + (p[3] - p[4]) * die_smooth((fabs(y) - (0.5 * w - fl) + 0.5 * wall) / wall);
+    const double depth = fmax(fmax(p[1] * die_smooth((-db + 0.5 * wall) / wall), p[2] * die_smooth((-ds + 0.5 * wall) / wall)),
+                              shank * die_smooth((-dsh + 0.5 * wall) / wall));
+    const double h = 0.5 * t + depth + g * die_smooth((fmin(fmin(db, ds), dsh) - land) / wall);
+    return D.side == 1 ? h : -h;
+  }
+  const double s3 = sqrt(3.0) / 2;
+  const double d = fmax(fabs(x), fmax(fabs(0.5 * x + s3 * y), fabs(-0.5 * x + s3 * y))) - 0.5 * p[2];
+  double h = p[0] + p[1] * die_smooth((-d + 0.5 * p[3]) / p[3]);
+  h += p[5] * die_smooth((d - p[4]) / p[3]);
+  return h;
+}
+
+// normal of the die surface below (x, y), pointing out of the die: central differences of the height
+// (HeightFieldDie.Project's normal, HeightFieldDie.cs:157-191, for an analytic surface)
+__device__ void die_normal(const DieDev& D, double x, double y, double n[3]) {
+#pragma clang fp contract(off)
+  const double dd = 1e-5;
+  const double hx = (die_height(D, x + dd, y) - die_height(D, x - dd, y)) / (2 * dd);
+  const double hy = (die_height(D, x, y + dd) - die_height(D, x, y - dd)) / (2 * dd);
+  double u0 = -hx, u1 = -hy, u2 = 1.0;
+  const double l = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+  if (l > 0) { u0 = u0 / l; u1 = u1 / l; u2 = u2 / l; }
+  n[0] = D.side == 0 ? u0 : -u0;
+  n[1] = D.side == 0 ? u1 : -u1;
+  n[2] = D.side == 0 ? u2 : -u2;
+}
+
+__device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
+#pragma clang fp contract(off)
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ void normalize3(double a[3]) {
+#pragma clang fp contract(off)
+  const double l = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  if (l > 0) { a[0] = a[0] / l; a[1] = a[1] / l; a[2] = a[2] / l; }
+}
+
+// Per node: constraint directions (symmetry planes x = 0 and y = 0, the die normal of a contact node), Gram-Schmidt
+// with the 0.2 drop threshold, completion to an orthonormal frame; the first nfix local DOFs are fixed to the
+// normal die velocity (:669-727).
+__global__ void k_frames(int nn, DieDev bottom, DieDev top, double ram, const double* __restrict__ x,
+                         const unsigned char* __restrict__ on_x0, const unsigned char* __restrict__ on_y0,
+                         const int* __restrict__ contact, double* __restrict__ ax, int* __restrict__ nfix,
+                         int* __restrict__ die_axis, double* __restrict__ fixed_val) {
+#pragma clang fp contract(off)
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double dirs[3][3], vals[3], e[3][3], ev[3];
+  int k = 0, die_idx = -1;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    vals[q] = 0.0;
+    ev[q] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) { dirs[q][c] = 0.0; e[q][c] = 0.0; }
+  }
+  if (on_x0[n]) { dirs[k][0] = 1.0; k++; }
+  if (on_y0[n]) {
+    if (k == 0) dirs[0][1] = 1.0; else dirs[1][1] = 1.0;
+    k++;
+  }
+  const int ct = contact[n];
+  if (ct != 0) {
+    double nrm[3];
+    die_normal(ct == 2 ? top : bottom, x[3 * n], x[3 * n + 1], nrm);
+    // DieVelocity . nrm: the top die moves with (0, 0, -ram), the bottom die rests
+    const double v = ct == 2 ? 0.0 * nrm[0] + 0.0 * nrm[1] + (-ram) * nrm[2] : 0.0 * nrm[0] + 0.0 * nrm[1] + 0.0 * nrm[2];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q == k) { dirs[q][0] = nrm[0]; dirs[q][1] = nrm[1]; dirs[q][2] = nrm[2]; vals[q] = v; }
+    die_idx = k;
+    k++;
+  }
+  int m = 0, da = -1;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    if (q >= k) continue;
+    double d[3] = {dirs[q][0], dirs[q][1], dirs[q][2]};
+    double vv = vals[q];
+#pragma unroll
+    for (int s2 = 0; s2 < 3; s2++) {
+      if (s2 >= m) continue;
+      const double proj = dot3(d, e[s2]);
+      d[0] = d[0] - proj * e[s2][0];
+      d[1] = d[1] - proj * e[s2][1];
+      d[2] = d[2] - proj * e[s2][2];
+      vv -= proj * ev[s2];
+    }
+    const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (len < 0.2) continue;
+    const double il = 1 / len;
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+      if (t == m) { e[t][0] = il * d[0]; e[t][1] = il * d[1]; e[t][2] = il * d[2]; ev[t] = vv / len; }
+    if (q == die_idx) da = m;
+    m++;
+  }
+  if (m == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) e[q][c] = q == c ? 1.0 : 0.0;
+  } else if (m == 1) {
+    const double helper[3] = {fabs(e[0][0]) < 0.8 ? 1.0 : 0.0, fabs(e[0][0]) < 0.8 ? 0.0 : 1.0, 0.0};
+    cross3(e[0], helper, e[1]);
+    normalize3(e[1]);
+    cross3(e[0], e[1], e[2]);
+  } else if (m == 2) {
+    cross3(e[0], e[1], e[2]);
+    normalize3(e[2]);
+  }
+  double* R = ax + 9 * (long long)n;
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) R[3 * q + c] = e[q][c];
+  nfix[n] = m;
+  die_axis[n] = da;
+#pragma unroll
+  for (int q = 0; q < 3; q++) fixed_val[3 * n + q] = q < m ? ev[q] : 0.0;
+}
+
 // LoadVelocityGuess (:854-869)
 __global__ void k_vguess(int nn, const double* __restrict__ ax, const int* __restrict__ nfix,
                          const double* __restrict__ fixed_val, const double* __restrict__ vg, double* __restrict__ v) {
@@ -855,6 +1006,17 @@ void StepDevice::set_perm(const int* perm) {
 void StepDevice::prepare(const double* x, const double* ax, const int* nfix, const double* fixed_val,
                          const int* contact, const double* vguess) {
   if (!have_params_) throw Error(-5, "parameters must be set before PrepareStep");
+  ax_.upload(ax, 9 * (size_t)nn_, s_);
+  nfix_.upload(nfix, nn_, s_);
+  fixed_val_.upload(fixed_val, ndof_, s_);
+  contact_.upload(contact, nn_, s_);
+  device_frames_ = false;
+  prepare_rest(x, vguess);
+}
+
+// PrepareStep after the frames: element geometry, ANP, friction, warm start
+void StepDevice::prepare_rest(const double* x, const double* vguess) {
+  if (!have_params_) throw Error(-5, "parameters must be set before PrepareStep");
   x_.upload(x, ndof_, s_);
   if (need_analyze_) {
     h_coords_.resize(3 * (size_t)ndof_);
@@ -862,10 +1024,6 @@ void StepDevice::prepare(const double* x, const double* ax, const int* nfix, con
       for (int c = 0; c < 3; c++)
         for (int a = 0; a < 3; a++) h_coords_[3 * (3 * (size_t)n + c) + a] = x[3 * n + a];
   }
-  ax_.upload(ax, 9 * (size_t)nn_, s_);
-  nfix_.upload(nfix, nn_, s_);
-  fixed_val_.upload(fixed_val, ndof_, s_);
-  contact_.upload(contact, nn_, s_);
   IFEA_HIP(hipMemcpyAsync(tmp_.p, vguess, ndof_ * sizeof(double), hipMemcpyHostToDevice, s_));
   int big = 0x7fffffff;
   IFEA_HIP(hipMemcpyAsync(bad_.p, &big, sizeof(int), hipMemcpyHostToDevice, s_));
@@ -886,6 +1044,55 @@ void StepDevice::prepare(const double* x, const double* ax, const int* nfix, con
                      v_.p);
   IFEA_HIP(hipGetLastError());
   prepared_ = true;
+}
+
+
+void StepDevice::set_geometry(const unsigned char* on_x0, const unsigned char* on_y0, int bottom_kind,
+                              const double* bottom_p, int top_kind, const double* top_p) {
+  if (!on_x0 || !on_y0 || !bottom_p || !top_p) throw Error(-1, "invalid geometry arguments");
+  if (bottom_kind < 0 || bottom_kind > 2 || top_kind < 0 || top_kind > 2) throw Error(-1, "unknown die kind");
+  on_x0_.upload(on_x0, nn_, s_);
+  on_y0_.upload(on_y0, nn_, s_);
+  bottom_.kind = bottom_kind;
+  bottom_.side = 0;
+  top_.kind = top_kind;
+  top_.side = 1;
+  for (int k = 0; k < 8; k++) { bottom_.p[k] = bottom_p[k]; top_.p[k] = top_p[k]; }
+  die_axis_.alloc(nn_);
+  IFEA_HIP(hipStreamSynchronize(s_));
+  have_geometry_ = true;
+}
+
+void StepDevice::prepare_device(const double* x, const int* contact, const double* vguess) {
+  if (!have_geometry_) throw Error(-5, "the symmetry flags and dies must be set before PrepareStep on the device");
+  contact_.upload(contact, nn_, s_);
+  xq_.upload(x, ndof_, s_);
+  hipLaunchKernelGGL(k_frames, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, bottom_, top_, P_.ram, xq_.p,
+                     on_x0_.p, on_y0_.p, contact_.p, ax_.p, nfix_.p, die_axis_.p, fixed_val_.p);
+  IFEA_HIP(hipGetLastError());
+  device_frames_ = true;
+  prepare_rest(x, vguess);
+}
+
+void StepDevice::load_guess(const double* vguess) {
+  if (!prepared_) throw Error(-5, "PrepareStep must be called first");
+  IFEA_HIP(hipMemcpyAsync(tmp_.p, vguess, ndof_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  hipLaunchKernelGGL(k_vguess, dim3(grid_for(nn_, 256)), dim3(256), 0, s_, nn_, ax_.p, nfix_.p, fixed_val_.p, tmp_.p,
+                     v_.p);
+  IFEA_HIP(hipGetLastError());
+  IFEA_HIP(hipStreamSynchronize(s_));
+}
+
+void StepDevice::get_frames(double* ax, int* nfix, int* die_axis, double* fixed_val) {
+  if (!prepared_) throw Error(-5, "PrepareStep must be called first");
+  if (ax) IFEA_HIP(hipMemcpyAsync(ax, ax_.p, 9 * (size_t)nn_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  if (nfix) IFEA_HIP(hipMemcpyAsync(nfix, nfix_.p, nn_ * sizeof(int), hipMemcpyDeviceToHost, s_));
+  if (die_axis) {
+    if (!device_frames_) throw Error(-5, "the die axes are the caller's when the frames come from the host");
+    IFEA_HIP(hipMemcpyAsync(die_axis, die_axis_.p, nn_ * sizeof(int), hipMemcpyDeviceToHost, s_));
+  }
+  if (fixed_val) IFEA_HIP(hipMemcpyAsync(fixed_val, fixed_val_.p, ndof_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  IFEA_HIP(hipStreamSynchronize(s_));
 }
 
 void StepDevice::set_element_state(const double* eps, const double* sigma) {

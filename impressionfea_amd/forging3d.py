@@ -24,8 +24,18 @@ from .mesh import TetMesh, build_pattern
 FREE, BOTTOM, TOP = 0, 1, 2
 
 
+def _dot3(a, b):
+    """Row-wise a . b as (a0 b0 + a1 b1) + a2 b2: the reference's Vec3.Dot order, never a fused or reassociated sum."""
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def _normalized(a):
+    return a / np.sqrt(_dot3(a, a))[..., None]
+
+
 def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
-    """PrepareStep's nodal frames (Forging3DSolver.cs:669-727), vectorised over nodes.
+    """PrepareStep's nodal frames (Forging3DSolver.cs:669-727), vectorised over nodes; the arithmetic of every node is
+    the reference's (and the device kernel's) in the same order.
 
     Returns ax [nn,3,3] (rows ax0, ax1, ax2), nfix [nn], dieAxis [nn], fixedVal [3nn].
     """
@@ -49,7 +59,7 @@ def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
         idx = ar[m]
         dirs[idx, k[m]] = nrm
         dvel = np.array([0.0, 0.0, -cfg.ram]) if side == TOP else np.zeros(3)
-        vals[idx, k[m]] = nrm @ dvel
+        vals[idx, k[m]] = _dot3(nrm, dvel)
         die_idx[idx] = k[m]
         k[m] += 1
     # nodes without constraints keep the identity frame: the Gram-Schmidt below runs on the constrained ones only
@@ -69,10 +79,10 @@ def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
         v = vals[:, q].copy()
         for s in range(3):
             sm = active & (s < mc)
-            proj = np.einsum("ij,ij->i", d, e[:, s])
+            proj = _dot3(d, e[:, s])
             d[sm] -= proj[sm, None] * e[sm, s]
             v[sm] -= proj[sm] * ev[sm, s]
-        ln = np.sqrt(np.einsum("ij,ij->i", d, d))
+        ln = np.sqrt(_dot3(d, d))
         keep = active & (ln >= 0.2)
         ki = ar[keep]
         e[ki, mc[keep]] = (1.0 / ln[keep])[:, None] * d[keep]
@@ -88,13 +98,13 @@ def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
         e0 = e[m1, 0]
         helper = np.where((np.abs(e0[:, 0]) < 0.8)[:, None], np.array([1.0, 0, 0]), np.array([0, 1.0, 0]))
         e1 = np.cross(e0, helper)
-        e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+        e1 = _normalized(e1)
         e[m1, 1] = e1
         e[m1, 2] = np.cross(e0, e1)
     m2 = mc == 2
     if m2.any():
         e2 = np.cross(e[m2, 0], e[m2, 1])
-        e[m2, 2] = e2 / np.linalg.norm(e2, axis=1, keepdims=True)
+        e[m2, 2] = _normalized(e2)
     full_e[sub], full_ev[sub], full_mc[sub], full_da[sub] = e, ev, mc, die_axis
     e, ev, mc, die_axis, nn = full_e, full_ev, full_mc, full_da, nn_all
     fixed_val = np.zeros(3 * nn)
@@ -123,7 +133,7 @@ class Forging3DSolver:
     """One billet on a fixed mesh, stepped by increments on the GPU (Forging3DSolver.Step without remeshing)."""
 
     def __init__(self, cfg: StepConfig, x, tets, onX0, onY0, gap0: float, precision: int = 0, perm=None,
-                 warm_start=None, first_solve_policy: bool = False):
+                 warm_start=None, first_solve_policy: bool = False, device_frames: bool = True):
         self.cfg = cfg
         self.x = np.array(x, dtype=np.float64)
         self.mesh = TetMesh(self.x.shape[0], tets, np.asarray(onX0, bool), np.asarray(onY0, bool))
@@ -172,6 +182,15 @@ class Forging3DSolver:
             self._check(L.ifea_step_set_perm(self._h, N.ptr(pm, C.c_int)), "set_perm")
         if first_solve_policy:
             self._check(L.ifea_step_set_option(self._h, N.IFEA_STEP_OPT_FIRST_SOLVE, 1.0), "set_option")
+        # PrepareStep's frames on the device (the host's compute_frames is the same arithmetic, kept for checks)
+        self.device_frames = bool(device_frames)
+        self._sym_flags = [np.ascontiguousarray(m.onX0, np.uint8), np.ascontiguousarray(m.onY0, np.uint8)]
+        bp, tp = N.f64(np.asarray(cfg.bottom.p, float)), N.f64(np.asarray(cfg.top.p, float))
+        self._check(L.ifea_step_set_geometry(self._h, N.ptr(self._sym_flags[0], C.c_ubyte),
+                                             N.ptr(self._sym_flags[1], C.c_ubyte),
+                                             int(cfg.bottom.kind), N.ptr(bp, C.c_double), int(cfg.top.kind),
+                                             N.ptr(tp, C.c_double)), "set_geometry")
+        self._host_frames = None
         # state
         self.contact = np.zeros(self.nn, dtype=np.int32)
         self.eps = np.zeros(self.ne)
@@ -258,10 +277,8 @@ class Forging3DSolver:
         return max(0.1 * float(np.min(zt[ok] - zb[ok])), self.cfg.stroke_increment / 32)
 
     def prepare(self, vguess=None):
-        """PrepareStep: host frames, then the device part (geometry, ANP, friction, warm start)."""
-        ax, nfix, die_axis, fixed_val = compute_frames(self.x, self.mesh.onX0, self.mesh.onY0, self.contact, self.gap,
-                                                       self.cfg)
-        self.ax, self.nfix, self.die_axis, self.fixed_val = ax, nfix, die_axis, fixed_val
+        """PrepareStep: the nodal frames (on the device, or by compute_frames on the host), then the device part
+        (geometry, ANP, friction, warm start)."""
         if vguess is None:
             trend = self.cfg.extrapolate_start and self.warm and self.vg_prev is not None
             vguess = 2 * self.vg - self.vg_prev if trend else self.vg
@@ -270,10 +287,50 @@ class Forging3DSolver:
             e, s = N.f64(self.eps), N.f64(self.el_sig)
             self._check(L.ifea_step_set_element_state(self._h, N.ptr(e, C.c_double), N.ptr(s, C.c_double)), "state")
             self._state_dirty = False
-        xs, axs, nf, fv, ct, vgs = N.f64(self.x), N.f64(ax), N.i32(nfix), N.f64(fixed_val), N.i32(self.contact), N.f64(vguess)
+        xs, ct, vgs = N.f64(self.x), N.i32(self.contact), N.f64(vguess)
+        if self.device_frames:
+            rc = L.ifea_step_prepare_device(self._h, N.ptr(xs, C.c_double), N.ptr(ct, C.c_int), N.ptr(vgs, C.c_double))
+            self._check(rc, "PrepareStep")
+            self._host_frames = None
+            da = np.zeros(self.nn, np.int32)
+            self._check(L.ifea_step_get_frames(self._h, None, None, N.ptr(da, C.c_int), None), "frames")
+            self.die_axis = da
+            return
+        ax, nfix, die_axis, fixed_val = compute_frames(self.x, self.mesh.onX0, self.mesh.onY0, self.contact, self.gap,
+                                                       self.cfg)
+        self._host_frames = (ax, nfix, fixed_val)
+        self.die_axis = die_axis
+        axs, nf, fv = N.f64(ax), N.i32(nfix), N.f64(fixed_val)
         rc = L.ifea_step_prepare(self._h, N.ptr(xs, C.c_double), N.ptr(axs, C.c_double), N.ptr(nf, C.c_int),
                                  N.ptr(fv, C.c_double), N.ptr(ct, C.c_int), N.ptr(vgs, C.c_double))
         self._check(rc, "PrepareStep")
+
+    def frames(self):
+        """The current increment's frames (ax [nn,3,3], nfix [nn], fixed values [3nn]), read from the device after a
+        device PrepareStep."""
+        if self._host_frames is None:
+            ax, nf, fv = np.zeros((self.nn, 3, 3)), np.zeros(self.nn, np.int32), np.zeros(self.ndof)
+            self._check(N.lib().ifea_step_get_frames(self._h, N.ptr(ax, C.c_double), N.ptr(nf, C.c_int), None,
+                                                     N.ptr(fv, C.c_double)), "frames")
+            self._host_frames = (ax, nf, fv)
+        return self._host_frames
+
+    @property
+    def ax(self):
+        return self.frames()[0]
+
+    @property
+    def nfix(self):
+        return self.frames()[1]
+
+    @property
+    def fixed_val(self):
+        return self.frames()[2]
+
+    def load_guess(self, vg):
+        """LoadVelocityGuess (:854-869) of a global nodal velocity field with the current frames, on the device."""
+        v = N.f64(vg)
+        self._check(N.lib().ifea_step_load_guess(self._h, N.ptr(v, C.c_double)), "load_guess")
 
     def _learned_start(self) -> str:
         """PrepareStep with the learned guess, judged against the reference's own starts (the previous increment's
@@ -283,18 +340,14 @@ class Forging3DSolver:
         from .mlp import trend_start
         guess = self.warm_start(self)
         self.prepare(guess)
-        cands = {"learned": self.get_v()}
-        fixed = (np.arange(3)[None, :] < self.nfix[:, None]).ravel()
-        for name, vg in (("previous", self.vg), ("trend", trend_start(self))):
-            v = np.einsum("nij,nj->ni", self.ax, vg).ravel()
-            v[fixed] = self.fixed_val[fixed]
-            cands[name] = v
+        cands = {"learned": guess, "previous": self.vg, "trend": trend_start(self)}
         best, best_e = None, np.inf
-        for name, v in cands.items():
-            e = self.assemble(v, matrix=False, newton=False)[0]
+        for name, vg in cands.items():
+            self.load_guess(vg)
+            e = self.assemble(None, matrix=False, newton=False)[0]
             if e < best_e:
                 best, best_e = name, e
-        self.set_v(cands[best])
+        self.load_guess(cands[best])
         return best
 
     def first_collapsed(self, p, qmin: float, quality=None) -> int:

@@ -80,6 +80,25 @@ int ifea_step_set_perm(ifea_step_t h, const int* perm);
 int ifea_step_prepare(ifea_step_t h, const double* x, const double* ax, const int* nfix, const double* fixed_val,
                       const int* contact, const double* vguess);
 
+/* PrepareStep with the nodal frames on the device (Forging3DSolver.cs:669-727): once per mesh, the symmetry-plane
+ * flags on_x0[nn] / on_y0[nn] and the two analytic rigid dies (kind 0: flat plane at z = p[0]; kind 1: top die with a
+ * hexagonal cavity; kind 2: one half of a connecting-rod closed die; p[8] their closed-position parameters, as the
+ * Python host's DieSpec); then per increment ifea_step_prepare_device(x[3nn], contact[nn], vguess[3nn]) computes the
+ * constraint directions (symmetry normals, the die normal of every contact node by central differences of the die
+ * height, HeightFieldDie.Project's normal), their Gram-Schmidt with the 0.2 drop threshold, the completed orthonormal
+ * frames and the fixed DOFs (normal die velocity, the top die moving with -ram), and continues as ifea_step_prepare.
+ * The frames equal the reference's arithmetic bit for bit (no FMA contraction). */
+int ifea_step_set_geometry(ifea_step_t h, const unsigned char* on_x0, const unsigned char* on_y0, int bottom_kind,
+                           const double* bottom_p, int top_kind, const double* top_p);
+int ifea_step_prepare_device(ifea_step_t h, const double* x, const int* contact, const double* vguess);
+
+/* LoadVelocityGuess (:854-869) with the current frames: v = R vg with the fixed DOFs overwritten (a start candidate). */
+int ifea_step_load_guess(ifea_step_t h, const double* vguess);
+
+/* The current frames ax[9nn], nfix[nn], die axes die_axis[nn] (only after ifea_step_prepare_device) and fixed values
+ * fixed_val[3nn]; any pointer may be NULL. */
+int ifea_step_get_frames(ifea_step_t h, double* ax, int* nfix, int* die_axis, double* fixed_val);
+
 /* Element state: accumulated strain eps[ne] and flow stress sigma[ne] (used by BuildFriction). Either may be NULL. */
 int ifea_step_set_element_state(ifea_step_t h, const double* eps, const double* sigma);
 

@@ -83,7 +83,7 @@ struct Die {
     if (kind == 2) {
       const double xb = -42.0, rb = 20.0, xs = 49.5, rs = 12.5, g = 5.0;
       const double t = p[0], wall = p[6], land = p[7], w = p[5], fl = 0.25 * w;
-      double db = std::hypot(x - xb, y) - rb, ds = std::hypot(x - xs, y) - rs;
+      double db = std::sqrt((x - xb) * (x - xb) + y * y) - rb, ds = std::sqrt((x - xs) * (x - xs) + y * y) - rs;
       double dsh = std::max(std::max(xb - x, x - xs), std::fabs(y) - 0.5 * w);
       double shank = p[4] + (p[3] - p[4]) * smooth((std::fabs(y) - (0.5 * w - fl) + 0.5 * wall) / wall);
       double depth = std::max(std::max(p[1] * smooth((-db + 0.5 * wall) / wall), p[2] * smooth((-ds + 0.5 * wall) / wall)),

@@ -91,3 +91,43 @@ def test_velocities_mean_stress_damage_default_penalty(oracle_lib):
         assert dm <= 1e-6 and dd <= 1e-6, rows
         assert eo["dmg"].max() > 0
     s.close()
+
+
+@pytest.mark.parametrize("workload", ["upset_quarter", "hexdie", "conrod"])
+def test_device_frames_bit_equal(oracle_lib, workload):
+    """PrepareStep's frames on the device (ifea_step_prepare_device) equal the oracle's PrepareStep (:669-727) and the
+    host's compute_frames bit for bit: the quarter upset (symmetry planes with both flat dies), the hex die and the
+    con-rod die (curved die surfaces). Every surface node is put in contact (upper half with the top die, lower half
+    with the bottom die) and, for the hex die, the billet is widened to reach the cavity walls, so that the die normals
+    of the curved parts are exercised; the die axes and fixed velocities are compared too."""
+    import oracle as O
+    from impressionfea_amd.config import conrod_production, hexdie_production
+    from impressionfea_amd.forging3d import compute_frames
+    from impressionfea_amd.mesh import TetMesh
+    wl = {"upset_quarter": lambda: upset_small(quarter=True, nxy=5, nz=6),
+          "hexdie": lambda: hexdie_production(nxy=12, nz=18),
+          "conrod": lambda: conrod_production(nxy=8, nz=38)}[workload]()
+    x, t, ox, oy = wl.mesh()
+    if workload == "hexdie":
+        x = x * np.array([1.3, 1.3, 1.0])
+    surf = TetMesh(x.shape[0], t, ox, oy).is_surface_node
+    zmid = 0.5 * (x[:, 2].min() + x[:, 2].max())
+    contact = np.where(surf, np.where(x[:, 2] > zmid, 2, 1), 0).astype(np.int32)
+    o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0)
+    o.set_state(x=x, contact=contact, gap=wl.gap0)
+    o.prepare()
+    fr = o.frames()
+    s = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0)
+    s.x, s.contact, s.gap = x.copy(), contact.copy(), wl.gap0
+    s.prepare(np.zeros((s.nn, 3)))
+    ax, nfix, fv = s.frames()
+    assert np.array_equal(ax, fr["ax"])
+    assert np.array_equal(nfix, fr["nFix"]) and np.array_equal(s.die_axis, fr["dieAxis"])
+    assert np.array_equal(fv, fr["fixedVal"])
+    hx = compute_frames(x, ox, oy, contact, wl.gap0, wl.cfg)
+    assert np.array_equal(ax, hx[0]) and np.array_equal(fv, hx[3])
+    if workload != "upset_quarter":
+        # curved parts of the dies give tilted normals (the first local axis of a contact node is its die normal
+        # when no symmetry plane comes first)
+        assert np.abs(ax[contact > 0, 0, :2]).max() > 1e-2
+    s.close()
