@@ -510,6 +510,39 @@ class Forging3DSolver:
         d = N.f64(dmg) if dmg is not None else None
         self._check(N.lib().ifea_step_set_damage(self._h, N.ptr(d, C.c_double)), "set_damage")
 
+    def checkpoint(self):
+        """Forging3DSolver.Checkpoint (:1891-1893): the state at the end of the last increment (see formats)."""
+        from .formats import Forging3DCheckpoint
+        er = self.element_results()
+        m = self.mesh
+        return Forging3DCheckpoint(
+            tets=m.tets.copy(), on_x0=np.asarray(m.onX0, bool).copy(), on_y0=np.asarray(m.onY0, bool).copy(),
+            x=self.x.copy(), vg=self.vg.copy(), vg_prev=None if self.vg_prev is None else self.vg_prev.copy(),
+            contact=self.contact.copy(), eps=self.eps.copy(), damage=er["dmg"], sigma=self.el_sig.copy(),
+            rate=self.el_rate.copy(), mean=er["elMean"], gap=float(self.gap), ds=float(self.ds), time=float(self.time),
+            h=float(self.cfg.h), step=int(self.step_index))
+
+    def restore_from(self, cp, warm: bool = False):
+        """RestoreFrom (:1899-1921) on this solver's own mesh: positions, velocities, contact, element history, gap,
+        increment and time. Like the reference the next increment starts cold (Picard first, no trend history) unless
+        `warm` keeps the checkpoint's trend history, which replays a run's continuation. A checkpoint of another mesh
+        needs the reference's remesh and remap, which this solver does not do."""
+        m = self.mesh
+        if not (np.array_equal(cp.tets, m.tets) and np.array_equal(cp.on_x0, m.onX0) and np.array_equal(cp.on_y0, m.onY0)):
+            raise NotImplementedError("the checkpoint holds another mesh: restoring it needs a remesh and remap")
+        self.x = np.array(cp.x, dtype=np.float64)
+        self.vg = np.array(cp.vg, dtype=np.float64)
+        self.contact = np.array(cp.contact, dtype=np.int32)
+        self.set_element_state(cp.eps, cp.sigma)
+        self.el_rate = np.array(cp.rate, dtype=np.float64)
+        self.set_damage(cp.damage)
+        self.release = np.zeros(self.nn, dtype=bool)
+        self.gap, self.ds, self.time, self.step_index = float(cp.gap), float(cp.ds), float(cp.time), int(cp.step)
+        if warm and cp.vg_prev is not None:
+            self.vg_prev, self.warm, self.n_converged = np.array(cp.vg_prev, dtype=np.float64), True, 2
+        else:
+            self.vg_prev, self.warm, self.n_converged = None, bool(warm), 1
+
     def get_v(self):
         v = np.zeros(self.ndof)
         self._check(N.lib().ifea_step_get_v(self._h, N.ptr(v, C.c_double)), "get_v")

@@ -131,3 +131,40 @@ def test_device_frames_bit_equal(oracle_lib, workload):
         # when no symmetry plane comes first)
         assert np.abs(ax[contact > 0, 0, :2]).max() > 1e-2
     s.close()
+
+
+def test_checkpoint_restart_continues_the_run(tmp_path):
+    """Forging3DSolver.Checkpoint / RestoreFrom through the checkpoint file: a fresh solver restored from the state
+    after two increments (with the trend history kept) steps the third increment like the uninterrupted run. The two
+    take different Newton paths (the restarted solver has no cached factor), and the velocities converge to the
+    configured 1e-6 relative change, so the bars are: ram load 1e-6, positions 1e-6 of their scale, strain and damage
+    (which integrate the strain rate) 1e-5 of their largest values. The run's telemetry rows go to a CSV in the
+    reference's format and read back."""
+    from impressionfea_amd import formats as F
+    wl = upset_small(nxy=6, nz=8)
+    a = Forging3DSolver.from_workload(wl)
+    v0 = float(F.tet_volumes(a.x, a.mesh.tets).sum())
+    rows, cp_path = [], str(tmp_path / "after2.npz")
+    for k in range(3):
+        r = a.step()
+        rows.append(F.telemetry_row(a, r, samples=F.cavity_samples(wl.cfg), v0=v0))
+        if k == 1:
+            F.save_checkpoint(cp_path, a.checkpoint())
+    ref = dict(load=r.load, x=a.x.copy(), eps=a.eps.copy(), dmg=a.element_results()["dmg"])
+    b = Forging3DSolver.from_workload(wl)
+    b.restore_from(F.load_checkpoint(cp_path), warm=True)
+    rb = b.step()
+    print(f"restart: load {rb.load:.9e} vs {ref['load']:.9e} N; Newton {rb.iterations} vs {r.iterations}")
+    assert rb.step == 3 and rb.load == pytest.approx(ref["load"], rel=1e-6)
+    assert np.abs(b.x - ref["x"]).max() <= 1e-6 * np.abs(ref["x"]).max()
+    assert np.abs(b.eps - ref["eps"]).max() <= 1e-5 * ref["eps"].max()
+    d = b.element_results()["dmg"]
+    assert ref["dmg"].max() > 0 and np.abs(d - ref["dmg"]).max() <= 1e-5 * ref["dmg"].max()
+    csv = str(tmp_path / "run.csv")
+    F.write_telemetry_csv(csv, rows, comments=[wl.name])
+    back, _ = F.read_telemetry_csv(csv)
+    assert [q.step for q in back] == [1, 2, 3]
+    assert back[-1].load_tonnes == pytest.approx(r.load / F.TONNE_N, abs=1e-3)
+    assert back[-1].stroke == pytest.approx(wl.gap0 - a.gap, abs=1e-4) and abs(back[-1].volume_change_percent) < 1.0
+    a.close()
+    b.close()
