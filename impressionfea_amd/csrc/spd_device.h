@@ -82,6 +82,10 @@ class SpdDevice {
   template <typename T> void sweeps(const T* L);  // forward + backward sweeps on y_ (captured into a HIP graph)
 
   hipStream_t stream_;
+  // the factorization's second stream (the panel chains of the outer blocks) and the events that order the two
+  hipStream_t chain_ = nullptr, outer_ = nullptr;
+  hipEvent_t ev_main_ = nullptr, ev_chain_ = nullptr;
+  hipStream_t chain_stream();
   Symbolic S_;
   bool analyzed_ = false, factored_ = false;
   int prec_ = 0;
@@ -106,8 +110,11 @@ class SpdDevice {
     std::vector<double> syrk_flops;      // algorithmic flops per syrk step
     std::vector<LaunchList> mid_steps;   // mid updates (second inner block of an outer block), after the panel step
     std::vector<double> mid_flops;
-    std::vector<LaunchList> outer_steps; // outer (512-column) updates, launched after the matching panel step
-    std::vector<double> outer_flops;
+    // outer (512-column) updates, launched after the matching panel step in two parts: outer_steps the tile columns
+    // of the next outer block (its panel chain waits for them), outer_b_steps the rest of the trailing matrix, which
+    // runs while the next block's panels are factored on the second stream
+    std::vector<LaunchList> outer_steps, outer_b_steps;
+    std::vector<double> outer_flops, outer_b_flops;
     // triangular sweeps: narrow fronts one workgroup each; wide fronts blocked over several launches
     long long small_begin = 0, small_end = 0, big_begin = 0, big_end = 0;  // into sweep_sn_
     long long small_max_m = 0;

@@ -22,6 +22,7 @@ constexpr int OB = 256;         // inner block: the 64-column panel updates stay
 constexpr int OB2 = 512;        // outer block: the trailing matrix beyond it is updated once per OB2 columns; inside
                                 // it, a mid update of OB columns brings its second half up to date
 constexpr long long OUTER_GROUP = 8;  // outer-update tiles are listed in groups of this many tile rows and columns
+constexpr long long OUTER_NEXT_TILES = OB2 / 128;  // tile columns of an outer update that the next block's panels read
 constexpr int EXT_TARGET = 8192;
 constexpr int STAGE_PER = NB * 64 / 256;  // 64x64-tile elements each of 256 threads stages (TRSM, inner update)
 constexpr int BIG_K = 256;      // supernodes wider than this, or with panels larger than BIG_PANEL entries, use the
@@ -123,105 +124,182 @@ __global__ __launch_bounds__(256) void k_extend(const int4* __restrict__ tasks, 
   }
 }
 
-// Diagonal block of one 64-column step: FP64 Cholesky in one wave, lane i owning row i of the block (in LDS, and in
-// registers for the 16-column panel being factored). Then the inverse of the factor (lane c solves L x = e_c with the
-// factor broadcast from LDS). The factor goes into the panel; the inverse is kept (Lt[c][r] = Linv[r][c]) for the TRSM
-// launch that follows and for the triangular sweeps. Columns beyond the block width are the identity.
+// Diagonal block of one 64-column step, FP64, one 256-thread workgroup: a blocked Cholesky in four 16-column panels
+// (the panel's 16x16 diagonal block factored and inverted by one wave, lane i owning row i, pivots and column entries
+// taken by lane reads; the rows below by forward substitution, a row per thread; the trailing triangle updated by all
+// threads),
+// then the blocked inverse of the factor from the panels' diagonal inverses (Linv_IJ = -Linv_II sum_K L_IK Linv_KJ,
+// one column block per wave). The factor goes into the panel; the inverse is kept (Lt[c][r] = Linv[r][c]) for the
+// TRSM launch that follows and for the triangular sweeps. Columns beyond the block width are the identity.
 // task: (supernode s, 0, 0, 0)
+constexpr int PB = 16;  // panel width of the diagonal-block Cholesky
 template <typename T>
-__global__ __launch_bounds__(64) void k_potrf_inv(const int4* __restrict__ tasks, int c0, SymDev sd,
-                                                  T* __restrict__ L, double* __restrict__ Linv,
-                                                  int* __restrict__ err) {
+__global__ __launch_bounds__(256) void k_potrf_inv(const int4* __restrict__ tasks, int c0, SymDev sd,
+                                                   T* __restrict__ L, double* __restrict__ Linv,
+                                                   int* __restrict__ err) {
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
   const long long m = sn_m(sd, s);
   const int w = min(NB, k - c0);
   T* P = L + sd.Loff[s];
-  __shared__ double D[NB][NB + 1];
-  __shared__ double rdiag[NB];  // 1 / L[j][j]
-  const int i = threadIdx.x;
-  const T* prow = P + (long long)c0 * m + c0 + i;
-  // the 64 column loads of the block are independent: all of them in flight at once
+  __shared__ double D[NB][NB + 1];   // the block, then its factor (lower)
+  __shared__ double Li[NB][NB + 1];  // the inverse of the factor (lower)
+  __shared__ double Tb[3][PB][PB + 1];
+  __shared__ double rdg[NB];         // 1 / L[j][j]
+  __shared__ int bad_any;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) bad_any = 0;
+  // load: thread (wave, lane) reads row `lane` of columns wave, wave + 4, ... (16 loads in flight, coalesced rows)
   {
-    double v[NB];
+    double v[NB / 4];
 #pragma unroll
-    for (int j = 0; j < NB; j++) v[j] = (i < w && j < w && i >= j) ? (double)prow[(long long)j * m] : 0.0;
+    for (int t = 0; t < NB / 4; t++) {
+      const int j = wv + 4 * t, i = lane;
+      v[t] = (i < w && j < w && i >= j) ? (double)P[(long long)(c0 + j) * m + c0 + i] : 0.0;
+    }
 #pragma unroll
-    for (int j = 0; j < NB; j++) D[i][j] = (i < w && j < w) ? v[j] : ((i == j) ? 1.0 : 0.0);
+    for (int t = 0; t < NB / 4; t++) {
+      const int j = wv + 4 * t, i = lane;
+      D[i][j] = (i < w && j < w) ? v[t] : ((i == j) ? 1.0 : 0.0);
+      Li[i][j] = 0.0;
+    }
   }
-  wave_sync_lds();
+  __syncthreads();
   bool bad = false;
-  double* row = &D[i][0];
-  // 16-column panels: lane i keeps its row of the panel in registers and factors the panel's columns with the pivots
-  // and column entries taken by lane reads (no LDS round trip inside the panel); then every column right of the panel
-  // takes one 16-term update from the panel's rows in LDS. Entries right of the diagonal are cleared afterwards.
-  for (int pc = 0; pc < NB; pc += 16) {
-    double r[16];
+#pragma unroll 1
+  for (int jp = 0; jp < NB; jp += PB) {
+    // (a) the panel's diagonal block and its inverse, in wave 0
+    if (wv == 0) {
+      const int i = lane;
+      double r[PB];
 #pragma unroll
-    for (int jj = 0; jj < 16; jj++) r[jj] = row[pc + jj];
+      for (int jj = 0; jj < PB; jj++) r[jj] = (i < PB && jj <= i) ? D[jp + i][jp + jj] : 0.0;
 #pragma unroll
-    for (int jj = 0; jj < 16; jj++) {
-      const int j = pc + jj;
-      const double piv = lane_read(r[jj], j);
-      bad |= !(piv > 0.0) || !isfinite(piv);
-      const double d = sqrt(piv > 0.0 ? piv : 1.0);
-      const double inv = 1.0 / d;
-      const double l = (i > j) ? r[jj] * inv : ((i == j) ? d : 0.0);
-      if (i == j) rdiag[j] = inv;
-      r[jj] = l;
+      for (int jj = 0; jj < PB; jj++) {
+        const double piv = lane_read(r[jj], jj);
+        bad |= !(piv > 0.0) || !isfinite(piv);
+        const double d = sqrt(piv > 0.0 ? piv : 1.0);
+        const double inv = 1.0 / d;
+        const double l = (i > jj) ? r[jj] * inv : ((i == jj) ? d : 0.0);
+        if (i == jj) rdg[jp + jj] = inv;
+        r[jj] = l;
 #pragma unroll
-      for (int kk = jj + 1; kk < 16; kk++) r[kk] -= l * lane_read(l, pc + kk);
-    }
-#pragma unroll
-    for (int jj = 0; jj < 16; jj++) row[pc + jj] = r[jj];
-    wave_sync_lds();
-#pragma unroll 4
-    for (int q = pc + 16; q < NB; q++) {
-      const double* Dq = &D[q][pc];
-      double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-      for (int jj = 0; jj < 16; jj += 2) {
-        a0 += r[jj] * Dq[jj];
-        a1 += r[jj + 1] * Dq[jj + 1];
+        for (int kk = jj + 1; kk < PB; kk++) r[kk] -= l * lane_read(l, kk);
       }
-      row[q] -= a0 + a1;
+      if (i < PB) {
+#pragma unroll
+        for (int jj = 0; jj < PB; jj++) D[jp + i][jp + jj] = jj <= i ? r[jj] : 0.0;
+      }
+      wave_sync_lds();
+      if (i < PB) {
+        // lane c: x = L_pp^-1 e_c, x[q] = (delta_qc - sum_{p<q} L[q][p] x[p]) / L[q][q]
+        const int c = i;
+        double x[PB];
+#pragma unroll
+        for (int q = 0; q < PB; q++) {
+          double a0 = (q == c) ? 1.0 : 0.0, a1 = 0.0;
+#pragma unroll
+          for (int p2 = 0; p2 + 1 < q; p2 += 2) {
+            a0 -= D[jp + q][jp + p2] * x[p2];
+            a1 -= D[jp + q][jp + p2 + 1] * x[p2 + 1];
+          }
+          if (q & 1) a0 -= D[jp + q][jp + q - 1] * x[q - 1];
+          x[q] = (a0 + a1) / D[jp + q][jp + q];
+        }
+#pragma unroll
+        for (int q = 0; q < PB; q++) Li[jp + q][jp + c] = x[q];
+      }
     }
-    wave_sync_lds();
-  }
-  for (int q = i + 1; q < NB; q++) row[q] = 0.0;
-  wave_sync_lds();
-  // factor into the panel (column q, row i)
-#pragma unroll 16
-  for (int q = 0; q < NB; q++)
-    if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)row[q];
-  if (bad && i == 0) atomicOr(err, 1);
-  // lane c: x = L^{-1} e_c, x[r] = (delta_rc - sum_{p<r} L[r][p] x[p]) / L[r][r]; four partial sums keep a row's
-  // dependent chain a quarter as long, and the reciprocal of the diagonal replaces the division
-  const int c = i;
-  double x[NB];
+    __syncthreads();
+    const int below = NB - jp - PB;  // rows under the panel's diagonal block
+    if (below > 0) {
+      // (b) the rows below the panel's diagonal block by forward substitution, one row per thread (the accumulation
+      // order of a right-looking column step): X[i][c] = (A[i][c] - sum_{q<c} X[i][q] L[c][q]) / L[c][c]
+      if (tid < below) {
+        const int i = jp + PB + tid;
+        double x[PB];
 #pragma unroll
-  for (int r = 0; r < NB; r++) {
-    double a0 = (r == c) ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int c = 0; c < PB; c++) {
+          double a = D[i][jp + c];
 #pragma unroll
-    for (int p = 0; p + 3 < r; p += 4) {
-      a0 -= D[r][p] * x[p];
-      a1 -= D[r][p + 1] * x[p + 1];
-      a2 -= D[r][p + 2] * x[p + 2];
-      a3 -= D[r][p + 3] * x[p + 3];
+          for (int q = 0; q < c; q++) a -= x[q] * D[jp + c][jp + q];
+          x[c] = a * rdg[jp + c];
+        }
+#pragma unroll
+        for (int c = 0; c < PB; c++) D[i][jp + c] = x[c];
+      }
+      __syncthreads();
+      // (c) trailing triangle: D[i][j] -= sum_q X[i][q] X[j][q], jp + 16 <= j <= i < 64
+      const int nt = below * (below + 1) / 2;
+      for (int e = tid; e < nt; e += 256) {
+        // e -> (a, b) with 0 <= b <= a < below, row-major over the lower triangle
+        int a = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while ((a + 1) * (a + 2) / 2 <= e) a++;
+        while (a * (a + 1) / 2 > e) a--;
+        const int b2 = e - a * (a + 1) / 2;
+        const int i = jp + PB + a, j = jp + PB + b2;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < PB; q += 2) {
+          s0 += D[i][jp + q] * D[j][jp + q];
+          s1 += D[i][jp + q + 1] * D[j][jp + q + 1];
+        }
+        D[i][j] -= s0 + s1;
+      }
+      __syncthreads();
     }
-#pragma unroll
-    for (int p = r & ~3; p < r; p++) a0 -= D[r][p] * x[p];
-    x[r] = ((a0 + a1) + (a2 + a3)) * rdiag[r];
   }
-  wave_sync_lds();
-  // transpose through LDS so that the stores of Lt[c][r] = Linv[r][c] are contiguous over the lanes
+  if (bad) atomicOr(&bad_any, 1);
+  // blocked inverse: wave J fills column block J below the diagonal, one block row at a time
+  if (wv < 3) {
+    const int J = wv;
+#pragma unroll 1
+    for (int I = J + 1; I < NB / PB; I++) {
+      double tv[4];
 #pragma unroll
-  for (int r = 0; r < NB; r++) D[c][r] = x[r];
-  wave_sync_lds();
+      for (int u = 0; u < 4; u++) {
+        const int e = lane + 64 * u, r = e / PB, c = e % PB;
+        double a0 = 0.0, a1 = 0.0;
+        for (int K = J; K < I; K++) {
+#pragma unroll
+          for (int q = 0; q < PB; q += 2) {
+            a0 += D[PB * I + r][PB * K + q] * Li[PB * K + q][PB * J + c];
+            a1 += D[PB * I + r][PB * K + q + 1] * Li[PB * K + q + 1][PB * J + c];
+          }
+        }
+        tv[u] = a0 + a1;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = lane + 64 * u;
+        Tb[J][e / PB][e % PB] = tv[u];
+      }
+      wave_sync_lds();
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = lane + 64 * u, r = e / PB, c = e % PB;
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < PB; q += 2) {
+          if (q <= r) a0 += Li[PB * I + r][PB * I + q] * Tb[J][q][c];
+          if (q + 1 <= r) a1 += Li[PB * I + r][PB * I + q + 1] * Tb[J][q + 1][c];
+        }
+        Li[PB * I + r][PB * J + c] = -(a0 + a1);
+      }
+      wave_sync_lds();
+    }
+  }
+  __syncthreads();
+  if (tid == 0 && bad_any) atomicOr(err, 1);
+  // factor into the panel (column q, row i) and the inverse transposed, Lt[c][r] = Linv[r][c], both row-coalesced
   double* Lt = Linv + sd.Ioff[s] + (long long)(c0 / NB) * NB * NB;
-#pragma unroll 16
-  for (int t = 0; t < NB; t++) Lt[t * NB + i] = D[t][i];
+#pragma unroll
+  for (int t = 0; t < NB / 4; t++) {
+    const int q = wv + 4 * t, i = lane;
+    if (i < w && q < w && q <= i) P[(long long)(c0 + q) * m + c0 + i] = (T)D[i][q];
+    Lt[q * NB + i] = i >= q ? Li[i][q] : 0.0;
+  }
 }
 
 // Rows below the diagonal block: X = A L11^{-T} as a 64x64x64 FP64 MFMA product per task, written back into the
@@ -1133,6 +1211,14 @@ __global__ void k_perm_out(const double* __restrict__ y, const int* __restrict__
 SpdDevice::~SpdDevice() {
   drop_graph();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+  if (chain_) {
+    (void)hipStreamSynchronize(chain_);
+    (void)hipStreamSynchronize(outer_);
+    (void)hipEventDestroy(ev_main_);
+    (void)hipEventDestroy(ev_chain_);
+    (void)hipStreamDestroy(chain_);
+    (void)hipStreamDestroy(outer_);
+  }
 }
 
 hipEvent_t SpdDevice::event(size_t i) {
@@ -1229,6 +1315,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
     // everything to its right follows the outer block's last panel
     int maxk = 0;
     for (int s : bylevel[l]) maxk = std::max(maxk, S.sn_first[s + 1] - S.sn_first[s]);
+    std::vector<int4> outa, outb;
     for (int c0 = 0; c0 < maxk; c0 += NB) {
       LaunchList lp, ls, lm, lo, lpo;
       lp.begin = (long long)pan.size();
@@ -1236,7 +1323,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       lpo.c0 = c0;
       ls.begin = (long long)syr.size();
       lp.c0 = ls.c0 = c0;
-      double fl = 0, flm = 0, flo = 0;
+      double fl = 0, flm = 0, flo = 0, flo_a = 0;
       int nslot = 0;
       const int ib0 = (c0 / OB) * OB, ob0 = (c0 / OB2) * OB2;
       const bool outer_now = (c0 + NB) % OB2 == 0 || c0 + NB >= maxk;
@@ -1307,13 +1394,25 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
             for (long long JB = 0; JB <= IB; JB += OUTER_GROUP)
               for (long long I = IB; I < std::min(tiles, IB + OUTER_GROUP); I++)
                 for (long long J = JB; J <= std::min(I, JB + OUTER_GROUP - 1); J++)
-                  syr.push_back(make_int4(s, (int)I, (int)J, 0));
+                  (J < OUTER_NEXT_TILES ? outa : outb).push_back(make_int4(s, (int)I, (int)J, 0));
+          // part A: the tile columns of the next outer block, which its panels need; part B: the rest
+          const long long na = std::min(nrows, (long long)OB2);
+          flo_a += (double)kw * (double)na * (double)(2 * nrows - na + 1);
           flo += (double)kw * (double)nrows * (double)(nrows + 1);
         }
       }
+      syr.insert(syr.end(), outa.begin(), outa.end());
       lo.end = (long long)syr.size();
+      LaunchList lob = lo;
+      lob.begin = lo.end;
+      syr.insert(syr.end(), outb.begin(), outb.end());
+      lob.end = (long long)syr.size();
+      outa.clear();
+      outb.clear();
       L.outer_steps.push_back(lo);
-      L.outer_flops.push_back(flo);
+      L.outer_flops.push_back(flo_a);
+      L.outer_b_steps.push_back(lob);
+      L.outer_b_flops.push_back(flo - flo_a);
     }
   }
   // sweep schedule: narrow fronts one workgroup each; wide fronts in SB-column steps (diagonal + panel launches)
@@ -1499,49 +1598,76 @@ int SpdDevice::factorize_t(const double* d_vals) {
   long long na = (long long)amap_src_.n;
   if (na) hipLaunchKernelGGL(k_scatter<T>, dim3(grid_for(na, 256)), dim3(256), 0, stream_, d_vals, amap_src_.p,
                              amap_dst_.p, na, L);
+  // Look-ahead over two streams: the main stream runs the extend-adds and the outer updates, the chain stream the
+  // panel chain of each outer block (POTRF, TRSM, inner and mid updates). An outer update runs in two launches: part A,
+  // the tile columns of the next outer block, then part B, the rest of the trailing matrix; the next block's chain
+  // starts after part A and runs while part B does. Writes never overlap: part B touches only columns beyond the next
+  // block, and every update of an entry (the atomic epilogues included) stays on one stream in program order.
+  hipStream_t Cs = chain_stream(), M = outer_;
+  IFEA_HIP(hipEventRecord(ev_main_, stream_));
+  IFEA_HIP(hipStreamWaitEvent(M, ev_main_, 0));
+  auto launch_update = [&](hipStream_t st, int mode, const LaunchList& q, double flops) {
+    if (q.end <= q.begin) return;
+    if (profiling_) IFEA_HIP(hipEventRecord(event(2 * nev), st));
+    if (mode == 0)
+      hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, st, syrk_tasks_.p + q.begin,
+                         q.c0, sd_, L);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_syrk_big<T, 512, 16, true>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, st,
+                         syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+    else
+      hipLaunchKernelGGL((k_syrk_big<T, 512, 16, false>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0, st,
+                         syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
+    if (profiling_) {
+      IFEA_HIP(hipEventRecord(event(2 * nev + 1), st));
+      nev++;
+      const bool outer = mode == 2;
+      ev_mode_.push_back((char)outer);
+      if (!outer) prof_.syrk_flops += flops;
+      if (outer) prof_.outer_flops += flops;
+      if (outer && sizeof(T) == 4) prof_.outer_flops_f32 += flops;
+    }
+  };
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
     for (const LaunchList& r : lv.ext_rounds)
-      hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, stream_,
-                         ext_tasks_.p + r.begin, sd_, L, U);
-    for (size_t t = 0; t < lv.panel_steps.size(); t++) {
+      hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, M, ext_tasks_.p + r.begin,
+                         sd_, L, U);
+    IFEA_HIP(hipEventRecord(ev_main_, M));
+    IFEA_HIP(hipStreamWaitEvent(Cs, ev_main_, 0));
+    const size_t nsteps = lv.panel_steps.size();
+    for (size_t t = 0; t < nsteps; t++) {
+      const int c0 = lv.potrf_steps[t].c0;
+      // a new outer block's chain needs part A of the previous outer update
+      if (t > 0 && c0 % OB2 == 0) IFEA_HIP(hipStreamWaitEvent(Cs, ev_main_, 0));
       const LaunchList& po = lv.potrf_steps[t];
       if (po.end > po.begin)
-        hipLaunchKernelGGL(k_potrf_inv<T>, dim3((unsigned)(po.end - po.begin)), dim3(64), 0, stream_,
+        hipLaunchKernelGGL(k_potrf_inv<T>, dim3((unsigned)(po.end - po.begin)), dim3(256), 0, Cs,
                            potrf_tasks_.p + po.begin, po.c0, sd_, L, Linv_.p, err_.p);
       const LaunchList& p = lv.panel_steps[t];
       if (p.end > p.begin)
-        hipLaunchKernelGGL(k_trsm_mfma<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, stream_,
+        hipLaunchKernelGGL(k_trsm_mfma<T>, dim3((unsigned)(p.end - p.begin)), dim3(256), 0, Cs,
                            panel_tasks_.p + p.begin, p.c0, sd_, L, Linv_.p);
-      // inner (0), mid (1) and outer (2) updates; the profile counts inner and mid updates together
-      for (int mode = 0; mode < 3; mode++) {
-        const LaunchList& q = mode == 0 ? lv.syrk_steps[t] : mode == 1 ? lv.mid_steps[t] : lv.outer_steps[t];
-        if (q.end <= q.begin) continue;
-        if (profiling_) IFEA_HIP(hipEventRecord(event(2 * nev), stream_));
-        if (mode == 0)
-          hipLaunchKernelGGL(k_syrk<T>, dim3((unsigned)(q.end - q.begin)), dim3(256), 0, stream_,
-                             syrk_tasks_.p + q.begin, q.c0, sd_, L);
-        else if (mode == 1)
-          hipLaunchKernelGGL((k_syrk_big<T, 512, 16, true>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0,
-                             stream_, syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
-        else
-          hipLaunchKernelGGL((k_syrk_big<T, 512, 16, false>), dim3((unsigned)(q.end - q.begin)), dim3(512), 0,
-                             stream_, syrk_tasks_.p + q.begin, q.c0, sd_, L, U);
-        if (profiling_) {
-          IFEA_HIP(hipEventRecord(event(2 * nev + 1), stream_));
-          nev++;
-          const bool outer = mode == 2;
-          ev_mode_.push_back((char)outer);
-          if (!outer) prof_.syrk_flops += mode == 0 ? lv.syrk_flops[t] : lv.mid_flops[t];
-          if (outer) prof_.outer_flops += lv.outer_flops[t];
-          if (outer && sizeof(T) == 4) prof_.outer_flops_f32 += lv.outer_flops[t];
-        }
+      // the profile counts inner and mid updates together
+      launch_update(Cs, 0, lv.syrk_steps[t], lv.syrk_flops[t]);
+      launch_update(Cs, 1, lv.mid_steps[t], lv.mid_flops[t]);
+      const bool outer_now = (c0 + NB) % OB2 == 0 || t + 1 == nsteps;
+      if (outer_now) {
+        IFEA_HIP(hipEventRecord(ev_chain_, Cs));
+        IFEA_HIP(hipStreamWaitEvent(M, ev_chain_, 0));
+        launch_update(M, 2, lv.outer_steps[t], lv.outer_flops[t]);
+        IFEA_HIP(hipEventRecord(ev_main_, M));
+        launch_update(M, 2, lv.outer_b_steps[t], lv.outer_b_flops[t]);
       }
     }
+    IFEA_HIP(hipEventRecord(ev_chain_, Cs));
+    IFEA_HIP(hipStreamWaitEvent(M, ev_chain_, 0));
     for (const LaunchList& r : lv.ext_rounds_u)
-      hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, stream_,
-                         ext_tasks_.p + r.begin, sd_, L, U);
+      hipLaunchKernelGGL(k_extend<T>, dim3((unsigned)(r.end - r.begin)), dim3(256), 0, M, ext_tasks_.p + r.begin,
+                         sd_, L, U);
   }
+  IFEA_HIP(hipEventRecord(ev_main_, M));
+  IFEA_HIP(hipStreamWaitEvent(stream_, ev_main_, 0));
   if (inv_launch_.end > inv_launch_.begin)
     hipLaunchKernelGGL(k_inv_diag<T>, dim3((unsigned)(inv_launch_.end - inv_launch_.begin)), dim3(256), 0, stream_,
                        inv_tasks_.p, sd_, L, Linv_.p, X_.p, XT_.p);
@@ -1662,6 +1788,18 @@ void SpdDevice::sweeps(const T* L) {
       launch(k_bwdb_panel<T>, lv.bwd_panel[t], bwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
   }
+}
+
+// The factorization's two streams: the outer updates and extend-adds on `outer_`, the panel chains on `chain_` (both
+// at the default priority).
+hipStream_t SpdDevice::chain_stream() {
+  if (!chain_) {
+    IFEA_HIP(hipStreamCreateWithFlags(&chain_, hipStreamNonBlocking));
+    IFEA_HIP(hipStreamCreateWithFlags(&outer_, hipStreamNonBlocking));
+    IFEA_HIP(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
+    IFEA_HIP(hipEventCreateWithFlags(&ev_chain_, hipEventDisableTiming));
+  }
+  return chain_;
 }
 
 void SpdDevice::drop_graph() {

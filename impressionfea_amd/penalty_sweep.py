@@ -27,7 +27,9 @@ from .spd import HipSpdSolver
 
 
 def sweep(penalties=(1e4, 1e5, 1e6, 1e7), increments=20, nxy=10, nz=10, refine_tol=1e-10, tolerance=None,
-          log=print):
+          log=print, keep_systems=False):
+    """Every Newton system of both steppers' increments, solved by the slot in FP64 and in mixed precision; with
+    keep_systems each row also holds its system (pattern, values, right-hand side) for an independent check."""
     out = []
     for pf in penalties:
         over = dict(penalty_factor=pf) if tolerance is None else dict(penalty_factor=pf, tolerance=tolerance)
@@ -67,8 +69,10 @@ def sweep(penalties=(1e4, 1e5, 1e6, 1e7), increments=20, nxy=10, nz=10, refine_t
                        pcg_iterations_fp64=ra.stats["pcg_iterations"], pcg_iterations_mixed=rb.stats["pcg_iterations"],
                        same_discrete_state=same_state,
                        contact_differences=int(np.count_nonzero(ref.contact != mix.contact)))
-            rows.append(row)
             log(json.dumps(row))
+            if keep_systems:
+                row["system"] = (ref.colPtr, ref.rowIdx, val, b)
+            rows.append(row)
         for h in (s64, s32):
             h.Dispose()
         ref.close()
