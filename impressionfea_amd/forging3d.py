@@ -115,6 +115,18 @@ def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
     return e, mc.astype(np.int32), die_axis.astype(np.int32), fixed_val
 
 
+def tet_quality(x, tets):
+    """TetMesh.Quality (TetMesh.cs:107-112) of the given tets in the reference's operation order: 6 sqrt(2) V / l2^1.5
+    with l2 the mean squared edge length (1 regular, <= 0 inverted, -1 degenerate)."""
+    a, b, c, d = (x[tets[..., k]] for k in range(4))
+    l2 = (_dot3(b - a, b - a) + _dot3(c - a, c - a) + _dot3(d - a, d - a) + _dot3(c - b, c - b) + _dot3(d - b, d - b)
+          + _dot3(d - c, d - c)) / 6.0
+    vol = _dot3(b - a, np.cross(c - a, d - a)) / 6.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = 6 * np.sqrt(2.0) * vol / np.power(l2, 1.5)
+    return np.where(l2 > 0, q, -1.0)
+
+
 @dataclass
 class IncrementResult:
     step: int
@@ -198,6 +210,7 @@ class Forging3DSolver:
         self.el_rate = np.zeros(self.ne)
         self.vg = np.zeros((self.nn, 3))
         self.release = np.zeros(self.nn, dtype=bool)
+        self.last_repair = (0, 0)          # nodes moved by ImprovePlacement and SmoothInterior in the last TryUpdate
         self._state_dirty = True
 
     @staticmethod
@@ -378,13 +391,172 @@ class Forging3DSolver:
         self.release = rel.astype(bool)
         return pw.value, rc.value
 
+    # ------------------------------------------------------------------ geometry update and placement repair
+    def surface_normals(self, x):
+        """SurfaceNormals (:1540-1562): area-weighted outward unit normals at the surface nodes (zero elsewhere), the
+        faces' contributions added in face order."""
+        m = self.mesh
+        f = m.boundary_faces[~m.face_is_symmetry]
+        an = np.cross(x[f[:, 1]] - x[f[:, 0]], x[f[:, 2]] - x[f[:, 0]])
+        nr = np.zeros((self.nn, 3))
+        np.add.at(nr, f.ravel(), np.repeat(an, 3, axis=0))
+        nr[m.onX0, 0] = 0.0
+        nr[m.onY0, 1] = 0.0
+        l2 = _dot3(nr, nr)
+        ok = l2 > 0
+        nr[ok] = nr[ok] / np.sqrt(l2[ok])[:, None]
+        return nr
+
+    def _tets_of(self, n):
+        m = self.mesh
+        return m.node_tet_list[m.node_tet_start[n]:m.node_tet_start[n + 1]]
+
+    def _worst_around(self, x, n):
+        return float(tet_quality(x, self.mesh.tets[self._tets_of(n)]).min())
+
+    def _push_out(self, die, off, p, outward):
+        """PushOut (:1570-1592): back to the die surface along the inward normal by bisection within 2 h, else the
+        closest point."""
+        if _dot3(outward, outward) == 0 or die.clearance(p, off) >= 0:
+            return die.project(p, off)[0]
+        d = -outward
+        h = self.cfg.h
+        step, lo, hi = 0.1 * h, 0.0, -1.0
+        t = step
+        while t <= 2 * h + 1e-12:
+            if die.clearance(p + t * d, off) >= 0:
+                hi = t
+                break
+            lo = t
+            t += step
+        if hi < 0:
+            return die.project(p, off)[0]
+        for _ in range(25):
+            mid = 0.5 * (lo + hi)
+            if die.clearance(p + mid * d, off) >= 0:
+                hi = mid
+            else:
+                lo = mid
+        return p + hi * d
+
+    def _sym1(self, p, n):
+        m = self.mesh
+        return np.array([0.0 if m.onX0[n] else p[0], 0.0 if m.onY0[n] else p[1], p[2]])
+
+    def _bad_tets(self, x, q_min):
+        """Tets with TetMesh.Quality < q_min at x: the device evaluates every tet, the host re-evaluates those near the
+        threshold with the reference's arithmetic so the decision is the reference's."""
+        q = np.zeros(self.ne)
+        self.first_collapsed(x, q_min, q)
+        cand = np.nonzero(q < q_min + 1e-9)[0]
+        if cand.size == 0:
+            return cand
+        return cand[tet_quality(x, self.mesh.tets[cand]) < q_min]
+
+    def improve_placement(self, x, contact, pred, nrm, gap, q_min=0.05):
+        """ImprovePlacement (:1625-1679) at the normal pass's tolerances: every die-contact node of a tet below q_min
+        tries the closest die point, the exit along the inward normal and, when it barely penetrates, partial
+        pushes, keeping whichever most raises the worst quality of its tets (tets visited in ascending order, each
+        judged at the current positions). Returns the number of nodes moved; x and contact change in place."""
+        import heapq
+        cfg, m, t = self.cfg, self.mesh, self.mesh.tets
+        total = 0
+        for _ in range(4):
+            count = 0
+            heap = [int(e) for e in self._bad_tets(x, q_min)]
+            heapq.heapify(heap)
+            queued = set(heap)
+            while heap:
+                e = heapq.heappop(heap)
+                if float(tet_quality(x, t[e:e + 1])[0]) >= q_min:
+                    continue
+                for a in range(4):
+                    n = int(t[e, a])
+                    if contact[n] == FREE:
+                        continue
+                    die, off = (cfg.top, gap) if contact[n] == TOP else (cfg.bottom, 0.0)
+                    closest = self._sym1(die.project(pred[n], off)[0], n)
+                    cand = [closest]
+                    push = self._sym1(self._push_out(die, off, pred[n], nrm[n]), n)
+                    if np.sqrt(_dot3(push - pred[n], push - pred[n])) <= 0.5 * self.h_local[n]:
+                        cand.append(push)
+                    pen = -float(die.clearance(pred[n], off))
+                    for f in (0.0, 0.25, 0.5, 0.75):
+                        if pen * (1 - f) <= 0.05 * self.h_local[n]:
+                            cand.append(pred[n] + f * (closest - pred[n]))
+                    old = x[n].copy()
+                    best = self._worst_around(x, n)
+                    pick = -1
+                    for ci, cpt in enumerate(cand):
+                        if np.array_equal(cpt, old):
+                            continue
+                        x[n] = cpt
+                        q = self._worst_around(x, n)
+                        if q > best + 1e-6:
+                            best, pick = q, ci
+                    x[n] = cand[pick] if pick >= 0 else old
+                    if pick < 0:
+                        continue
+                    if np.array_equal(cand[pick], pred[n]) and pen < 0:
+                        contact[n] = FREE
+                    count += 1
+                    for e2 in self._tets_of(n):       # tets after e whose quality this move may have changed
+                        if e2 > e and int(e2) not in queued:
+                            queued.add(int(e2))
+                            heapq.heappush(heap, int(e2))
+            total += count
+            if count == 0:
+                break
+        return total
+
+    def smooth_interior(self, x, q_target=0.05, sweeps=6):
+        """SmoothInterior (:1776-1838): each interior node of a tet below q_target moves towards its neighbours'
+        centroid (full, half or quarter step) when that raises the worst quality of its tets; nodes in ascending
+        order at the current positions; symmetry-plane nodes stay on their planes. Returns the number of moves."""
+        m, t = self.mesh, self.mesh.tets
+        moves = 0
+        for _ in range(sweeps):
+            bad_t = self._bad_tets(x, q_target)
+            if bad_t.size == 0:
+                break
+            nodes = np.unique(t[bad_t].ravel())
+            nodes = nodes[~m.is_surface_node[nodes]]
+            sweep_moves = 0
+            for n in nodes:
+                n = int(n)
+                te = self._tets_of(n)
+                tt = t[te]
+                others = tt[tt != n]
+                if others.size == 0:
+                    continue
+                s = np.zeros(3)
+                for q in others:                    # the reference's running sum, in its order
+                    s = s + x[q]
+                worst = float(tet_quality(x, tt).min())
+                old = x[n].copy()
+                c = (1.0 / others.size) * s
+                for lam in (1.0, 0.5, 0.25):
+                    p = old + lam * (c - old)
+                    x[n] = self._sym1(p, n)
+                    if float(tet_quality(x, tt).min()) > worst + 1e-6:
+                        sweep_moves += 1
+                        break
+                    x[n] = old
+            moves += sweep_moves
+            if sweep_moves == 0:
+                break
+        return moves
+
     def try_update(self, dt: float) -> str:
-        """TryUpdate (:1455-1537) without ImprovePlacement/SmoothInterior; '' on success, else the reason."""
+        """TryUpdate (:1455-1537): predicted positions, contact and placement, the placement repair (ImprovePlacement,
+        SmoothInterior) and the collapse check on the device; '' on success, else the reason."""
         cfg = self.cfg
         gapT = max(0.0, self.gap - cfg.ram * dt)
         if gapT < 1e-9:
             gapT = 0.0
+        nrm = self.surface_normals(self.x)
         p = self._sym(self.x + dt * self.vg)
+        pred = p.copy()
         cT = np.zeros(self.nn, np.int32)
         # nodes in contact (and not released) stay on their die unless they left it
         for side, die in ((BOTTOM, cfg.bottom), (TOP, cfg.top)):
@@ -417,6 +589,7 @@ class Forging3DSolver:
         p = self._sym(p)
         if max_pen > 0.5:
             return "die penetration exceeds half the local element size"
+        self.last_repair = (self.improve_placement(p, cT, pred, nrm, gapT, 0.05), self.smooth_interior(p, 0.05, 6))
         bad = self.first_collapsed(p, 0.005)
         if bad >= 0:
             return f"tetrahedron {bad} collapsed"

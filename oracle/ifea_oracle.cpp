@@ -1051,16 +1051,160 @@ struct Oracle {
     return l2 > 0 ? 6 * std::sqrt(2.0) * V / std::pow(l2, 1.5) : -1;
   }
 
-  // TryUpdate (Forging3DSolver.cs:1455-1537), contact and placement only: the placement repair
-  // (ImprovePlacement/SmoothInterior) is host geometry outside the solve step and is not restated.
+  // SurfaceNormals (Forging3DSolver.cs:1540-1562): area-weighted outward normals of the surface nodes
+  std::vector<V3> SurfaceNormals(const std::vector<V3>& xx) const {
+    std::vector<V3> nr(nn);
+    for (size_t k = 0; k < fsym.size(); k++) {
+      if (fsym[k]) continue;
+      int a = bfaces[3 * k], b = bfaces[3 * k + 1], c = bfaces[3 * k + 2];
+      V3 an = (xx[b] - xx[a]).cross(xx[c] - xx[a]);
+      nr[a] = nr[a] + an;
+      nr[b] = nr[b] + an;
+      nr[c] = nr[c] + an;
+    }
+    for (int i = 0; i < nn; i++) {
+      if (onX0[i]) nr[i].x = 0;
+      if (onY0[i]) nr[i].y = 0;
+      if (nr[i].dot(nr[i]) > 0) nr[i] = nr[i].normalized();
+    }
+    return nr;
+  }
+
+  // PushOut (:1570-1592): back to the die surface along the workpiece's inward normal (bisection), else the closest point
+  V3 PushOut(const Die& die, double off, const V3& p, const V3& outward) const {
+    if (outward.dot(outward) == 0 || die.clearance(p, off) >= 0) return die.project(p, off, nullptr);
+    V3 d = -outward;
+    double step = 0.1 * P.h, lo = 0, hi = -1;
+    for (double t = step; t <= 2 * P.h + 1e-12; t += step) {
+      if (die.clearance(p + t * d, off) >= 0) { hi = t; break; }
+      lo = t;
+    }
+    if (hi < 0) return die.project(p, off, nullptr);
+    for (int i = 0; i < 25; i++) {
+      double mid = 0.5 * (lo + hi);
+      if (die.clearance(p + mid * d, off) >= 0) hi = mid; else lo = mid;
+    }
+    return p + hi * d;
+  }
+
+  double TetQ(const std::vector<V3>& xx, int e) const {
+    return Quality(xx[tets[4 * e]], xx[tets[4 * e + 1]], xx[tets[4 * e + 2]], xx[tets[4 * e + 3]]);
+  }
+
+  // WorstAround (:1681-1687)
+  double WorstAround(const std::vector<V3>& xx, int n) const {
+    double q = 1e308;
+    for (int p2 = ntStart[n]; p2 < ntStart[n + 1]; p2++) q = std::min(q, TetQ(xx, ntList[p2]));
+    return q;
+  }
+
+  // ImprovePlacement (:1625-1679) at the normal pass's tolerances (Tol = the local size, corner allowance 0.05 of it)
+  int ImprovePlacement(std::vector<V3>& xx, std::vector<int>& cc, const std::vector<V3>& pred, const std::vector<V3>& nrm,
+                       double gapT, double qMin) {
+    int total = 0;
+    for (int sweep = 0; sweep < 4; sweep++) {
+      int count = 0;
+      for (int e = 0; e < ne; e++) {
+        if (TetQ(xx, e) >= qMin) continue;
+        for (int a = 0; a < 4; a++) {
+          int n = tets[4 * e + a];
+          if (cc[n] == FREE) continue;
+          const Die& die = Surface(cc[n]);
+          double off = Offset(cc[n], gapT);
+          V3 cand[8];
+          int k = 0;
+          V3 closest = Sym(die.project(pred[n], off, nullptr), n);
+          cand[k++] = closest;
+          V3 push = Sym(PushOut(die, off, pred[n], nrm[n]), n);
+          if ((push - pred[n]).len() <= 0.5 * hLocal[n]) cand[k++] = push;
+          double pen = -die.clearance(pred[n], off);
+          const double fs[4] = {0.0, 0.25, 0.5, 0.75};
+          for (double f : fs)
+            if (pen * (1 - f) <= 0.05 * hLocal[n]) cand[k++] = pred[n] + f * (closest - pred[n]);
+          V3 old = xx[n];
+          double best = WorstAround(xx, n);
+          int pick = -1;
+          for (int c = 0; c < k; c++) {
+            if (cand[c].x == old.x && cand[c].y == old.y && cand[c].z == old.z) continue;
+            xx[n] = cand[c];
+            double q = WorstAround(xx, n);
+            if (q > best + 1e-6) { best = q; pick = c; }
+          }
+          xx[n] = pick >= 0 ? cand[pick] : old;
+          if (pick < 0) continue;
+          const V3& cp = cand[pick];
+          if (cp.x == pred[n].x && cp.y == pred[n].y && cp.z == pred[n].z && pen < 0) cc[n] = FREE;
+          count++;
+        }
+      }
+      total += count;
+      if (count == 0) break;
+    }
+    return total;
+  }
+
+  // SmoothInterior (:1776-1838): interior nodes of tets below qTarget towards their neighbours' centroid
+  int SmoothInterior(std::vector<V3>& xx, double qTarget, int sweeps) {
+    int moves = 0;
+    std::vector<char> bad(nn);
+    for (int s2 = 0; s2 < sweeps; s2++) {
+      std::fill(bad.begin(), bad.end(), 0);
+      bool any = false;
+      for (int e = 0; e < ne; e++) {
+        if (TetQ(xx, e) >= qTarget) continue;
+        for (int a = 0; a < 4; a++) bad[tets[4 * e + a]] = 1;
+        any = true;
+      }
+      if (!any) break;
+      int sweepMoves = 0;
+      for (int n = 0; n < nn; n++) {
+        if (!bad[n] || isSurf[n]) continue;
+        V3 sum;
+        int cnt = 0;
+        double worst = 1e308;
+        for (int p2 = ntStart[n]; p2 < ntStart[n + 1]; p2++) {
+          int e = ntList[p2];
+          for (int a = 0; a < 4; a++) {
+            int m = tets[4 * e + a];
+            if (m == n) continue;
+            sum = sum + xx[m];
+            cnt++;
+          }
+          worst = std::min(worst, TetQ(xx, e));
+        }
+        if (cnt == 0) continue;
+        V3 old = xx[n];
+        V3 c = (1.0 / cnt) * sum;
+        const double lams[3] = {1.0, 0.5, 0.25};
+        for (double lambda : lams) {
+          V3 p = old + lambda * (c - old);
+          xx[n] = V3(onX0[n] ? 0 : p.x, onY0[n] ? 0 : p.y, p.z);
+          double q = 1e308;
+          for (int p2 = ntStart[n]; p2 < ntStart[n + 1]; p2++) q = std::min(q, TetQ(xx, ntList[p2]));
+          if (q > worst + 1e-6) { sweepMoves++; break; }
+          xx[n] = old;
+        }
+      }
+      moves += sweepMoves;
+      if (sweepMoves == 0) break;
+    }
+    return moves;
+  }
+
+  int lastReverts = 0, lastSmoothMoves = 0;
+
+  // TryUpdate (Forging3DSolver.cs:1455-1537): contact and placement, then the placement repair (ImprovePlacement,
+  // SmoothInterior) and the collapse check; the recovery ladder's lenient passes are not restated.
   int TryUpdate(double dt) {
     double gapT = std::max(0.0, gap - P.ram * dt);
     if (gapT < 1e-9) gapT = 0;
-    std::vector<V3> xT(nn);
+    std::vector<V3> xT(nn), pred(nn);
     std::vector<int> cT(nn, FREE);
+    std::vector<V3> nrm = SurfaceNormals(x);
     double maxPenRel = 0;
     for (int n = 0; n < nn; n++) {
       V3 p = Sym(x[n] + dt * vg[n], n);
+      pred[n] = p;
       int c = contact[n];
       if (c != FREE && !release[n]) {
         const Die& s = Surface(c);
@@ -1080,6 +1224,8 @@ struct Oracle {
       xT[n] = p;
     }
     if (maxPenRel > 0.5) { err = "die penetration exceeds half the local element size"; return 1; }
+    lastReverts = ImprovePlacement(xT, cT, pred, nrm, gapT, 0.05);
+    lastSmoothMoves = SmoothInterior(xT, 0.05, 6);
     for (int e = 0; e < ne; e++)
       if (Quality(xT[tets[4 * e]], xT[tets[4 * e + 1]], xT[tets[4 * e + 2]], xT[tets[4 * e + 3]]) <= 0.005) {
         err = "tetrahedron collapsed";
@@ -1309,6 +1455,25 @@ void orc_get_element_state(void* h, double* elSig, double* elRate, double* elDev
   if (elDev) std::copy(o->elDev.begin(), o->elDev.end(), elDev);
   if (elMean) std::copy(o->elMean.begin(), o->elMean.end(), elMean);
   if (dmg) std::copy(o->dmg.begin(), o->dmg.end(), dmg);
+}
+
+// TryUpdate (Forging3DSolver.cs:1455-1537) with the state's velocities over dt: 0 on success (x, contact, gap and time
+// advance), else non-zero with the reason in orc_error; moves[2] = nodes moved by ImprovePlacement and SmoothInterior.
+int orc_try_update(void* h, double dt, int* moves) {
+  auto* o = (Oracle*)h;
+  int rc = o->TryUpdate(dt);
+  if (moves) { moves[0] = o->lastReverts; moves[1] = o->lastSmoothMoves; }
+  return rc;
+}
+
+// SmoothInterior (:1776-1838) on the node positions x[3nn] (in place); returns the number of moves.
+int orc_smooth_interior(void* h, double* x, double qTarget, int sweeps) {
+  auto* o = (Oracle*)h;
+  std::vector<V3> xx(o->nn);
+  for (int n = 0; n < o->nn; n++) xx[n] = V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]);
+  int m = o->SmoothInterior(xx, qTarget, sweeps);
+  for (int n = 0; n < o->nn; n++) { x[3 * n] = xx[n].x; x[3 * n + 1] = xx[n].y; x[3 * n + 2] = xx[n].z; }
+  return m;
 }
 
 // Overwrite the converged element state that AccumulateDamage reads (any pointer may be NULL).

@@ -66,6 +66,8 @@ def lib():
         L.orc_set_element_state.argtypes = [vp, dp, dp, dp, dp]
         L.orc_accumulate_damage.argtypes = [vp, C.c_double]
         L.orc_max3.argtypes = [C.c_int, dp, dp]
+        L.orc_try_update.argtypes = [vp, C.c_double, ip]
+        L.orc_smooth_interior.argtypes = [vp, dp, C.c_double, C.c_int]
         L.orc_solve_velocity.argtypes = [vp, ip, dp, ip]
         L.orc_final.restype = C.c_double
         L.orc_final.argtypes = [vp, dp]
@@ -238,6 +240,17 @@ class Oracle:
     def set_element_state(self, elSig=None, elRate=None, elDev=None, elMean=None):
         k = [F64(a) if a is not None else None for a in (elSig, elRate, elDev, elMean)]
         lib().orc_set_element_state(self.h, *(_p(a, C.c_double) for a in k))
+
+    def try_update(self, dt):
+        """TryUpdate (Forging3DSolver.cs:1455-1537): '' on success, else the reason; and the repair passes' moves."""
+        mv = (C.c_int * 2)()
+        rc = lib().orc_try_update(self.h, float(dt), mv)
+        return ("" if rc == 0 else self.error()), (mv[0], mv[1])
+
+    def smooth_interior(self, x, q_target=0.05, sweeps=6):
+        xs = F64(x).copy()
+        m = lib().orc_smooth_interior(self.h, _p(xs, C.c_double), float(q_target), int(sweeps))
+        return xs, m
 
     def accumulate_damage(self, dt):
         lib().orc_accumulate_damage(self.h, float(dt))

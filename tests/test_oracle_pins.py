@@ -354,3 +354,108 @@ def test_mkl_returned_permutation_through_the_symbolic_analysis(oracle_lib, c1_m
         assert np.linalg.norm(x1 - x2) <= 1e-7 * np.linalg.norm(x1)
     # the inverse of MKL's order is a different (and much worse) ordering: the seam would notice a transposed reading
     assert sym(np.argsort(mkl_perm).astype(np.int32))[3] > 2 * sym(mkl_perm)[3]
+
+
+# ------------------------------------------------------------------ placement repair (TryUpdate :1455-1537)
+def _grid(n):
+    I, J, K = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    x = np.stack([I.transpose(2, 1, 0).ravel(), J.transpose(2, 1, 0).ravel(), K.transpose(2, 1, 0).ravel()], 1)
+    return x.astype(np.float64), kuhn_box(n, n, n).astype(np.int32)
+
+
+def _quality(x, t):
+    """6 sqrt(2) V / l2^1.5 with l2 the mean squared edge length: 1 for the regular tet, 0 when flat."""
+    a, b, c, d = (x[t[:, k]] for k in range(4))
+    e = [b - a, c - a, d - a, c - b, d - b, d - c]
+    l2 = sum((v * v).sum(1) for v in e) / 6.0
+    vol = np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a)) / 6.0
+    return 6 * np.sqrt(2.0) * vol / l2 ** 1.5
+
+
+@pytest.mark.parametrize("d", [(0.9, -0.9, 0.0), (0.0, 0.0, 0.95), (0.9, 0.9, 0.0), (0.3, 0.2, -0.97)])
+def test_smooth_interior_returns_the_lone_interior_node_to_its_star_centroid(oracle_lib, d):
+    """SmoothInterior (:1776-1838) on the 2x2x2 Kuhn box, whose only interior node is the centre: every other node is
+    on the surface and stays put, and the centre's star is symmetric under the reflection through the centre, so the
+    centroid of its tets' other vertices (counted with multiplicity) is the centre itself. Displacing the centre far
+    enough to make tets bad (q < 0.05) therefore takes one full (lambda = 1) move straight back to it, and the next
+    sweep finds nothing to do."""
+    import oracle as O
+    x, t = _grid(2)
+    z = np.zeros(len(x), bool)
+    c = int(np.nonzero(np.all(x == 1.0, axis=1))[0][0])
+    xs = x.copy()
+    xs[c] += np.array(d)
+    assert (_quality(xs, t) < 0.05).any()
+    o = O.Oracle(_cfg(2.0), x, t, z, z.copy(), 0.0)
+    xo, moves = o.smooth_interior(xs, 0.05, 6)
+    assert moves == 1
+    assert np.abs(xo[c] - x[c]).max() <= 1e-15
+    assert np.array_equal(np.delete(xo, c, 0), np.delete(xs, c, 0))
+    # a bad tet made by a surface node alone is left to ImprovePlacement: nothing moves
+    xs2 = x.copy()
+    s = int(np.nonzero(np.all(x == (1.0, 1.0, 0.0), axis=1))[0][0])
+    xs2[s, 2] = 0.97
+    assert (_quality(xs2, t) < 0.05).any()
+    xo2, moves2 = o.smooth_interior(xs2, 0.05, 6)
+    assert moves2 == 1 and np.array_equal(xo2[s], xs2[s])      # only the centre (in a bad tet) moves
+    assert np.abs(np.delete(xo2, [c, s], 0) - np.delete(xs2, [c, s], 0)).max() == 0.0
+
+
+def placement_box_case():
+    """A 4x4x4 unit Kuhn box between flat dies (bottom at z = 0, top closing at 10 mm/s from z = 4): the top node
+    above the centre lags the die (the die overtakes it by 0.04 = 0.04 h in dt = 0.005) while the node below it rises
+    to 0.04 under the die, flattening the tets between them. Returns (cfg, x, tets, contact, vg, gap, dt, n_top,
+    n_below)."""
+    n = 4
+    x, t = _grid(n)
+    g0 = 1.0
+    cfg = StepConfig(size=4.0, h=1.0, bottom=DieSpec(0, 0, (0.0,) + (0.0,) * 7),
+                     top=DieSpec(1, 0, (4.0 - g0,) + (0.0,) * 7))
+    contact = np.where(x[:, 2] == 0, 1, np.where(x[:, 2] == n, 2, 0)).astype(np.int32)
+    idx = lambda i, j, k: i + (n + 1) * (j + (n + 1) * k)
+    top, below = idx(2, 2, 4), idx(2, 2, 3)
+    dt = 0.005
+    vg = np.zeros_like(x)
+    vg[contact == 2, 2] = -cfg.ram
+    vg[top, 2] = -0.2 * cfg.ram
+    vg[below, 2] = 0.96 / dt
+    return cfg, x, t, contact, vg, g0, dt, top, below
+
+
+def test_improve_placement_flat_die_takes_the_best_candidate(oracle_lib):
+    """ImprovePlacement (:1625-1679) on the flat-die case above. After placement the lagging top node sits on the die
+    (z = 3.95) above its rising neighbour (z = 3.91), so its tets are bad. For a flat top die its candidates are known
+    in closed form: the closest die point (its current place), the exit along the inward normal -z (the same point to
+    the bisection's 0.1 h / 2^25), and, its penetration 0.04 being within 0.05 h, the partial pushes back from the
+    predicted point z = 3.99 at f = 0, .25, .5, .75 (3.99, 3.98, 3.97, 3.96). It takes the candidate with the best
+    worst-quality of its tets (computed here from the candidate list) and keeps its top contact (it still penetrates);
+    SmoothInterior then moves the interior nodes of the remaining bad tets, and the step succeeds."""
+    import oracle as O
+    cfg, x, t, contact, vg, g0, dt, top, below = placement_box_case()
+    z = np.zeros(len(x), bool)
+    o = O.Oracle(cfg, x, t, z, z.copy(), g0)
+    o.set_state(x=x, vg=vg, contact=contact, gap=g0)
+    why, (reverts, smooth) = o.try_update(dt)
+    assert why == "" and reverts == 1 and smooth >= 1
+    s = o.state()
+    assert s["gap"] == pytest.approx(g0 - cfg.ram * dt, abs=1e-15) and s["contact"][top] == 2
+    # positions before the repair: the predicted ones, top contacts on the die at 3.95
+    xp = x + dt * vg
+    on_top = contact == 2
+    xp[on_top, 2] = 3.95
+    tets_n = np.nonzero(np.any(t == top, axis=1))[0]
+    cands = [3.95, 3.99, 3.98, 3.97, 3.96]
+    worst = []
+    for zc in cands:
+        xq = xp.copy()
+        xq[top, 2] = zc
+        worst.append(_quality(xq, t[tets_n]).min())
+    best = int(np.argmax(worst))
+    assert worst[best] > worst[0] + 1e-6
+    assert s["x"][top, 2] == pytest.approx(cands[best], abs=1e-14)
+    assert s["x"][top, 0] == 2.0 and s["x"][top, 1] == 2.0
+    # the repair left no tet below the collapse threshold and no top node other than the lagging one off the die
+    assert _quality(s["x"], t).min() > 0.05
+    others = on_top.copy()
+    others[top] = False
+    assert np.all(s["x"][others, 2] == 3.95)
