@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "spd_device.h"
 
@@ -485,17 +486,6 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
   const int rr = threadIdx.x & 127, q0 = threadIdx.x >> 7;
   const bool iok = i0 + rr < m, jok = j0 + rr < m;
   T ri[PER], rj[PER];
-  auto fetch = [&](int c) {
-    const T* cb = P + (long long)(c * KC) * m;  // wave-uniform base of the chunk
-#pragma unroll
-    for (int t = 0; t < PER; t++) {
-      const int q = q0 + (NT / 128) * t;
-      const bool cok = c * KC + q < kw;
-      const unsigned off = (unsigned)(q * m);
-      ri[t] = (cok && iok) ? cb[off + (unsigned)(i0 + rr)] : T(0);
-      rj[t] = (cok && jok) ? cb[off + (unsigned)(j0 + rr)] : T(0);
-    }
-  };
   auto stage = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < PER; t++) {
@@ -525,22 +515,46 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
         for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
     }
   };
-  fetch(0);
-  stage(0);
-  if (1 < nch) fetch(1);
-  __syncthreads();
-  for (int c = 0; c < nch; c++) {
-    const int buf = c & 1;
-    // stage chunk c + 1 (its buffer was last read before the previous barrier) and put the loads of chunk c + 2 in
-    // flight before the MFMAs of chunk c: the wait for the loads comes right after a barrier, while the other
-    // workgroup on the CU keeps the MFMA pipes busy
-    if (c + 1 < nch) {
-      stage(buf ^ 1);
-      if (c + 2 < nch) fetch(c + 2);
-    }
-    compute(buf);
+  // the chunk pipeline, compiled twice: interior tiles (both 128-row panels inside the front, every chunk full) load
+  // without per-element checks, the others with them; predicated loads cost several percent of the MFMA rate, and
+  // the choice is uniform over the workgroup
+  auto pipeline = [&](auto checked) {
+    constexpr bool CHK = decltype(checked)::value;
+    auto fetch = [&](int c) {
+      const T* cb = P + (long long)(c * KC) * m;  // wave-uniform base of the chunk
+#pragma unroll
+      for (int t = 0; t < PER; t++) {
+        const int q = q0 + (NT / 128) * t;
+        const unsigned off = (unsigned)(q * m);
+        if (CHK) {
+          const bool cok = c * KC + q < kw;
+          ri[t] = (cok && iok) ? cb[off + (unsigned)(i0 + rr)] : T(0);
+          rj[t] = (cok && jok) ? cb[off + (unsigned)(j0 + rr)] : T(0);
+        } else {
+          ri[t] = cb[off + (unsigned)(i0 + rr)];
+          rj[t] = cb[off + (unsigned)(j0 + rr)];
+        }
+      }
+    };
+    fetch(0);
+    stage(0);
+    if (1 < nch) fetch(1);
     __syncthreads();
-  }
+    for (int c = 0; c < nch; c++) {
+      const int buf = c & 1;
+      // stage chunk c + 1 (its buffer was last read before the previous barrier) and put the loads of chunk c + 2 in
+      // flight before the MFMAs of chunk c: the wait for the loads comes right after a barrier, while the other
+      // workgroup on the CU keeps the MFMA pipes busy
+      if (c + 1 < nch) {
+        stage(buf ^ 1);
+        if (c + 2 < nch) fetch(c + 2);
+      }
+      compute(buf);
+      __syncthreads();
+    }
+  };
+  if (i0 + 128 <= m && j0 + 128 <= m && kw % KC == 0) pipeline(std::false_type{});
+  else pipeline(std::true_type{});
   T* Ls = L + sd.Loff[s];
   T* Us = U + sd.Uoff[s];
   const long long mu = m - k;

@@ -5,7 +5,8 @@ Mirrors src/ImpressionFEA.Core/ThreeD/Forging3DSolver.cs:
   Step (one stroke increment, no remeshing)             :540-638
   PrepareStep host part: nodal frames and fixed DOFs    :660-727   (vectorised numpy; die geometry is host side)
   LayerLimitedIncrement                                 :236-249
-  TryUpdate (contact and placement)                     :1455-1537 (without the placement repair passes)
+  TryUpdate (contact, placement and its repair)         :1455-1537, ImprovePlacement :1625-1679, SmoothInterior
+                                                        :1776-1838 (tet quality on the device)
 Everything between PrepareStep's frames and the strain update runs on the GPU through include/ifea_step.h.
 """
 from __future__ import annotations
@@ -398,8 +399,9 @@ class Forging3DSolver:
         m = self.mesh
         f = m.boundary_faces[~m.face_is_symmetry]
         an = np.cross(x[f[:, 1]] - x[f[:, 0]], x[f[:, 2]] - x[f[:, 0]])
-        nr = np.zeros((self.nn, 3))
-        np.add.at(nr, f.ravel(), np.repeat(an, 3, axis=0))
+        # bincount adds its weights in input order: each node sums its faces' contributions in face order
+        idx = f.ravel()
+        nr = np.stack([np.bincount(idx, weights=np.repeat(an[:, c], 3), minlength=self.nn) for c in range(3)], axis=1)
         nr[m.onX0, 0] = 0.0
         nr[m.onY0, 1] = 0.0
         l2 = _dot3(nr, nr)
@@ -554,7 +556,6 @@ class Forging3DSolver:
         gapT = max(0.0, self.gap - cfg.ram * dt)
         if gapT < 1e-9:
             gapT = 0.0
-        nrm = self.surface_normals(self.x)
         p = self._sym(self.x + dt * self.vg)
         pred = p.copy()
         cT = np.zeros(self.nn, np.int32)
@@ -589,8 +590,15 @@ class Forging3DSolver:
         p = self._sym(p)
         if max_pen > 0.5:
             return "die penetration exceeds half the local element size"
-        self.last_repair = (self.improve_placement(p, cT, pred, nrm, gapT, 0.05), self.smooth_interior(p, 0.05, 6))
-        bad = self.first_collapsed(p, 0.005)
+        # both repair passes act only on tets below 0.05 (and the collapse check on tets below 0.005): one device
+        # evaluation, with a margin for the last bits, tells whether there is any
+        if self.first_collapsed(p, 0.05 + 1e-9) >= 0:
+            nrm = self.surface_normals(self.x)
+            self.last_repair = (self.improve_placement(p, cT, pred, nrm, gapT, 0.05), self.smooth_interior(p, 0.05, 6))
+            bad = self.first_collapsed(p, 0.005)
+        else:
+            self.last_repair = (0, 0)
+            bad = -1
         if bad >= 0:
             return f"tetrahedron {bad} collapsed"
         self.x = p
