@@ -76,7 +76,7 @@ def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True, start="
         log(f"  step {k}: load {r.load:.6e} N it {r.iterations} fact {r.stats['factorizations']} "
             f"pcg {r.stats['pcg_iterations']} solves {r.stats['solves']} fb {r.stats['fp64_fallbacks']} "
             f"skip {r.stats['first_solve_skips']} start {r.stats.get('start')} {r.ms_solve:.0f} ms "
-            f"(+{r.ms_update:.0f} ms update)")
+            f"(+{r.ms_update:.0f} ms update, repair moves {r.stats['repair'][0]}/{r.stats['repair'][1]})")
     wall = time.perf_counter() - t0  # every step returns after its device work (results are read back)
     info = s.info(reset=True)
     used = device_memory_used()

@@ -118,7 +118,7 @@ class SpdDevice {
     // triangular sweeps: narrow fronts one workgroup each; wide fronts blocked over several launches
     long long small_begin = 0, small_end = 0, big_begin = 0, big_end = 0;  // into sweep_sn_
     long long small_max_m = 0;
-    LaunchList fwd_gather, fwd_rect, fwd_rect_sum, bwd_xb, bwd_gemv;
+    LaunchList fwd_gather, fwd_rect, fwd_rect_sum, fwd_rect4, fwd_rect4_sum, bwd_xb, bwd_gemv;
     std::vector<LaunchList> fwd_diag, fwd_panel, bwd_diag, bwd_panel;  // per SB-column step (backward descending)
     double solve_bytes = 0;              // panel bytes per element of storage (times sizeof(T) at run time)
     long long max_m = 0;

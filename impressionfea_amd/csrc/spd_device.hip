@@ -30,12 +30,13 @@ constexpr int BIG_K = 256;      // supernodes wider than this, or with panels la
 constexpr long long BIG_PANEL = 32768;  // blocked multi-workgroup triangular sweeps
 constexpr int SB = 2048;        // diagonal block of the blocked sweeps (explicit inverse per block)
 constexpr int SWEEP_COLS = 16;  // columns per workgroup of a backward panel step
-constexpr int GEMV_COLS = 16;   // columns per workgroup of the backward GEMV (four waves, GEMV_COLS / 4 columns each)
-constexpr int GEMV_RB = 8;      // rows of each column a backward-GEMV lane loads before using them
+constexpr int GEMV_COLS = 8;    // columns per workgroup of the backward GEMV (four waves, GEMV_COLS / 4 columns each)
+constexpr int GEMV_RB = 4;      // quads (four rows) of each column a backward-GEMV lane loads before using them
 constexpr int BT = 32;          // loads a lane issues before using them in the sweep kernels
 constexpr int DIAG_WAVES = 4;   // waves of a diagonal-step workgroup (one pair of rows or columns each)
 constexpr int DB = 8;           // loads of X (and of the vector) a lane issues before using them in a diagonal step
 constexpr int RECT_WAVES = 16;  // waves of a forward update-vector or panel workgroup (they split the columns)
+constexpr int RECT_ROWS = 256;  // rows of an FP32 forward update-vector task: four consecutive rows per lane
 constexpr int RECT_SPLIT = 2048; // widest column range of one forward update-vector task
 constexpr int BWD_NC = 8;       // columns a wave reduces at once in the narrow-front backward sweep
 constexpr int BP_ROWS = 512;    // rows of a backward panel step loaded per batch (times 4 columns per wave)
@@ -979,8 +980,96 @@ __device__ __forceinline__ void fwdb_rect_item(const int4 tk, long long tid, Sym
   }
 }
 
-// V(row) = W(row) - sum over the row block's column ranges (in range order) of their partial sums.
-// task: (s, first row, index of the first range's task, number of ranges)
+// Four consecutive entries of a column, loaded as one 16-byte (FP32) or two 16-byte (FP64) accesses at the element
+// alignment of the factor (columns start anywhere: the front offsets and leading dimensions are arbitrary).
+template <typename T>
+struct Quad { T e[4]; };
+template <typename T>
+__device__ __forceinline__ Quad<T> load_quad(const T* p) {
+  Quad<T> q;
+  __builtin_memcpy(&q, p, sizeof(q));
+  return q;
+}
+
+// acc[e][u % 4] += sum over columns q = base + u in [q0, q1) of pr[q * ld + e] * yo[q]: a lane's four consecutive
+// rows of a column-major panel against a vector every lane of the wave shares, each row in four interleaved partial
+// sums (short rounding chains). Full batches issue BQ quad loads at 32-bit offsets from a wave-uniform base
+// (BQ * ld < 2^31: ld < 2^27) and take yo by wave-uniform loads. q0, q1 are wave-uniform.
+template <typename T, int BQ>
+__device__ __forceinline__ void sweep_rows4_dot(const T* __restrict__ pr, int ld, const double* __restrict__ yo,
+                                                int q0, int q1, double acc[4][4]) {
+  int base = q0;
+  for (; base + BQ <= q1; base += BQ) {
+    const T* pc = pr + (long long)base * ld;
+    Quad<T> v[BQ];
+#pragma unroll
+    for (int u = 0; u < BQ; u++) v[u] = load_quad(pc + (unsigned)(u * ld));
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < BQ; u++) {
+      const double yu = yo[base + u];
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc[e][u & 3] = fma((double)v[u].e[e], yu, acc[e][u & 3]);
+    }
+  }
+  for (int u = 0; base < q1; base++, u++) {
+    const Quad<T> v = load_quad(pr + (long long)base * ld);
+    const double yu = yo[base];
+#pragma unroll
+    for (int e = 0; e < 4; e++) acc[e][u & 3] = fma((double)v.e[e], yu, acc[e][u & 3]);
+  }
+}
+
+// The RECT_ROWS rows [r0, min(r0 + RECT_ROWS, rend)) of a task, four per lane: lane l loads rows rc .. rc + 3 with
+// rc = min(r0 + 4 l, rend - 4) (rend >= 4, so the loads stay in the column) and owns those of them at or past
+// r0 + 4 l; wave wv's partial sums go to part[wv][row - r0].
+template <typename T>
+__device__ __forceinline__ void sweep_rows4_task(const T* __restrict__ P, long long ld, long long r0, long long rend,
+                                                 const double* __restrict__ yo, int qa, int qb,
+                                                 double (*part)[RECT_ROWS]) {
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long r = r0 + 4 * lane;
+  const long long rc = min(r, rend - 4);
+  constexpr int BQ = sizeof(T) == 4 ? 16 : 8;
+  const int per = (((qb - qa + RECT_WAVES - 1) / RECT_WAVES + BQ - 1) / BQ) * BQ;  // whole batches but the last
+  const int q0 = qa + wv * per, q1 = min(qb, q0 + per);
+  double acc[4][4] = {};
+  if (q0 < q1) sweep_rows4_dot<T, BQ>(P + rc, (int)ld, yo, q0, q1, acc);
+#pragma unroll
+  for (int e = 0; e < 4; e++)
+    if (rc + e >= r && rc + e < rend) part[wv][rc + e - r0] = (acc[e][0] + acc[e][1]) + (acc[e][2] + acc[e][3]);
+}
+
+// The update-vector rows of the FP32 sweeps: fwdb_rect_item on RECT_ROWS rows per task, four consecutive rows per
+// lane (one 16-byte load per lane and column; 64 single-row lanes issue 4-byte loads and leave HBM half idle). Same
+// column ranges and partial-sum slots (RECT_ROWS per task). task: (s, first row, first column, end column).
+template <typename T>
+__device__ __forceinline__ void fwdb_rect4_item(const int4 tk, long long tid, SymDev sd, const T* __restrict__ L,
+                                                   const double* __restrict__ y, const double* __restrict__ Wg,
+                                                   double* __restrict__ V, double* __restrict__ Pp) {
+  __shared__ double part[RECT_WAVES][RECT_ROWS];
+  const int s = tk.x;
+  const int f = sd.sn_first[s];
+  const int k = sn_k(sd, s);
+  const long long m = sn_m(sd, s);
+  const int qa = tk.z, qb = tk.w;
+  // wide fronts have m > 181 rows, so the clamped quads stay inside the column
+  sweep_rows4_task<T>(L + sd.Loff[s], m, tk.y, m, y + f, qa, qb, part);
+  __syncthreads();
+  const long long row = (long long)tk.y + threadIdx.x;
+  if (threadIdx.x < RECT_ROWS && row < m) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < RECT_WAVES; q++) a += part[q][threadIdx.x];
+    if (qa == 0 && qb == k) V[sd.Voff[s] + row - k] = Wg[sd.Woff[s] + row] - a;
+    else Pp[tid * RECT_ROWS + threadIdx.x] = a;
+  }
+}
+
+// V(row) = W(row) - sum over the row block's column ranges (in range order) of their partial sums; ROWS rows per
+// task (NB for the FP64 sweeps, RECT_ROWS for the FP32 ones). task: (s, first row, index of the first range's task,
+// number of ranges)
+template <int ROWS>
 __device__ __forceinline__ void fwdb_rect_sum_item(const int4 tk, SymDev sd, const double* __restrict__ Wg,
                                                        double* __restrict__ V, const double* __restrict__ Pp) {
   const int s = tk.x;
@@ -990,7 +1079,7 @@ __device__ __forceinline__ void fwdb_rect_sum_item(const int4 tk, SymDev sd, con
   const long long row = (long long)tk.y + lane;
   if (row >= m) return;
   double a = 0.0;
-  for (int g = 0; g < tk.w; g++) a += Pp[((long long)tk.z + g) * NB + lane];
+  for (int g = 0; g < tk.w; g++) a += Pp[((long long)tk.z + g) * ROWS + lane];
   V[sd.Voff[s] + row - k] = Wg[sd.Woff[s] + row] - a;
 }
 
@@ -1006,8 +1095,9 @@ __device__ __forceinline__ void bwdb_xb_item(const int4 tk, SymDev sd,
 }
 
 // z_q = y_own[q] - L21(:, q)^T xb for the own columns q of this task: each wave takes GEMV_COLS / 4 consecutive
-// columns and shares its loads of xb among them; GEMV_RB rows of every column in flight per lane. Columns past the
-// task are clamped to its last column and their sums dropped. task: (s, first column, end column, 0)
+// columns and shares its loads of xb among them; a lane takes four consecutive rows of every column (one quad load
+// per column), GEMV_RB such quads of every column in flight. Columns past the task are clamped to its last column and
+// their sums dropped. task: (s, first column, end column, 0)
 template <typename T>
 __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T* __restrict__ L,
                                                    const double* __restrict__ y, const double* __restrict__ V,
@@ -1029,26 +1119,39 @@ __device__ __forceinline__ void bwdb_gemv_item(const int4 tk, SymDev sd, const T
   double acc[NQ][2];
 #pragma unroll
   for (int j = 0; j < NQ; j++) acc[j][0] = acc[j][1] = 0.0;
-  long long base = lane;
-  for (; base + 64 * (GEMV_RB - 1) < mb; base += 64 * GEMV_RB) {  // full batches: every load issued before use
-    double xv[GEMV_RB];
-    T v[NQ][GEMV_RB];
+  long long base = 4 * lane;
+  for (; base + 256 * (GEMV_RB - 1) + 3 < mb; base += 256 * GEMV_RB) {  // full batches: every load issued before use
+    Quad<double> xv[GEMV_RB];
+    Quad<T> v[NQ][GEMV_RB];
 #pragma unroll
-    for (int u = 0; u < GEMV_RB; u++) xv[u] = xb[base + 64 * u];
+    for (int u = 0; u < GEMV_RB; u++) xv[u] = load_quad(xb + base + 256 * u);
 #pragma unroll
     for (int j = 0; j < NQ; j++)
 #pragma unroll
-      for (int u = 0; u < GEMV_RB; u++) v[j][u] = cq[j][base + 64 * u];
+      for (int u = 0; u < GEMV_RB; u++) v[j][u] = load_quad(cq[j] + base + 256 * u);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int j = 0; j < NQ; j++)
 #pragma unroll
-      for (int u = 0; u < GEMV_RB; u++) acc[j][u & 1] += (double)v[j][u] * xv[u];
-  }
-  for (; base < mb; base += 64) {
-    const double x = xb[base];
+      for (int u = 0; u < GEMV_RB; u++)
 #pragma unroll
-    for (int j = 0; j < NQ; j++) acc[j][0] += (double)cq[j][base] * x;
+        for (int e = 0; e < 4; e++) acc[j][e & 1] = fma((double)v[j][u].e[e], xv[u].e[e], acc[j][e & 1]);
+  }
+  for (; base + 3 < mb; base += 256) {
+    const Quad<double> x = load_quad(xb + base);
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      const Quad<T> v = load_quad(cq[j] + base);
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc[j][e & 1] = fma((double)v.e[e], x.e[e], acc[j][e & 1]);
+    }
+  }
+  if (base < mb) {  // the last lane with rows: fewer than four left
+    for (long long i = base; i < mb; i++) {
+      const double x = xb[i];
+#pragma unroll
+      for (int j = 0; j < NQ; j++) acc[j][0] = fma((double)cq[j][i], x, acc[j][0]);
+    }
   }
 #pragma unroll
   for (int j = 0; j < NQ; j++) {
@@ -1171,13 +1274,15 @@ __global__ __launch_bounds__(RECT_WAVES * 64) void k_fwdb_rect(const int4* __res
                                                                const double* __restrict__ y,
                                                                const double* __restrict__ Wg, double* __restrict__ V,
                                                                double* __restrict__ Pp) {
-  fwdb_rect_item<T>(tasks[blockIdx.x], tbase + blockIdx.x, sd, L, y, Wg, V, Pp);
+  if constexpr (sizeof(T) == 4) fwdb_rect4_item<T>(tasks[blockIdx.x], tbase + blockIdx.x, sd, L, y, Wg, V, Pp);
+  else fwdb_rect_item<T>(tasks[blockIdx.x], tbase + blockIdx.x, sd, L, y, Wg, V, Pp);
 }
 
-__global__ __launch_bounds__(64) void k_fwdb_rect_sum(const int4* __restrict__ tasks, SymDev sd,
-                                                      const double* __restrict__ Wg, double* __restrict__ V,
-                                                      const double* __restrict__ Pp) {
-  fwdb_rect_sum_item(tasks[blockIdx.x], sd, Wg, V, Pp);
+template <int ROWS>
+__global__ __launch_bounds__(ROWS) void k_fwdb_rect_sum(const int4* __restrict__ tasks, SymDev sd,
+                                                        const double* __restrict__ Wg, double* __restrict__ V,
+                                                        const double* __restrict__ Pp) {
+  fwdb_rect_sum_item<ROWS>(tasks[blockIdx.x], sd, Wg, V, Pp);
 }
 
 __global__ __launch_bounds__(256) void k_bwdb_xb(const int4* __restrict__ tasks, SymDev sd,
@@ -1461,7 +1566,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       for (long long i = 0; i < mu; i++) gsrc[base + cnt[rel[i]]++] = S.Voff[c] + i;
     }
   }
-  long long rect_part_max = 0;  // tasks of the largest forward rect launch (partial-sum slots)
+  long long rect_part_max = 0;  // partial-sum slots of the largest forward rect launch (tasks times rows per task)
   for (int l = 0; l < nl; l++) {
     Level& L = levels_[l];
     std::vector<int> small, big;
@@ -1501,26 +1606,32 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
       L.fwd_diag.push_back(d);
       L.fwd_panel.push_back(pn);
     }
-    // update-vector rows in 64-row blocks; fronts wider than RECT_SPLIT columns in equal column ranges (partial sums
-    // added by a second launch, k_fwdb_rect_sum, in range order)
-    L.fwd_rect.begin = (long long)fwt.size();
-    std::vector<int4> sums;
-    for (int s : big) {
-      const int k = sn_kk(s);
-      const int g = (k + RECT_SPLIT - 1) / RECT_SPLIT;
-      const int cw = (k + g - 1) / g;
-      for (long long r0 = k; r0 < sn_mm(s); r0 += NB) {
-        const long long t0 = (long long)fwt.size() - L.fwd_rect.begin;
-        int nr = 0;
-        for (int q = 0; q < k; q += cw, nr++) fwt.push_back(make_int4(s, (int)r0, q, std::min(k, q + cw)));
-        if (nr > 1) sums.push_back(make_int4(s, (int)r0, (int)t0, nr));
+    // update-vector rows in row blocks (NB rows for the FP64 sweeps, RECT_ROWS for the FP32 ones); fronts wider than
+    // RECT_SPLIT columns in equal column ranges (partial sums added by a second launch, k_fwdb_rect_sum, in range
+    // order)
+    for (int pass = 0; pass < 2; pass++) {
+      const int rows = pass == 0 ? NB : RECT_ROWS;
+      LaunchList& lr = pass == 0 ? L.fwd_rect : L.fwd_rect4;
+      LaunchList& ls = pass == 0 ? L.fwd_rect_sum : L.fwd_rect4_sum;
+      lr.begin = (long long)fwt.size();
+      std::vector<int4> sums;
+      for (int s : big) {
+        const int k = sn_kk(s);
+        const int g = (k + RECT_SPLIT - 1) / RECT_SPLIT;
+        const int cw = (k + g - 1) / g;
+        for (long long r0 = k; r0 < sn_mm(s); r0 += rows) {
+          const long long t0 = (long long)fwt.size() - lr.begin;
+          int nr = 0;
+          for (int q = 0; q < k; q += cw, nr++) fwt.push_back(make_int4(s, (int)r0, q, std::min(k, q + cw)));
+          if (nr > 1) sums.push_back(make_int4(s, (int)r0, (int)t0, nr));
+        }
       }
+      lr.end = (long long)fwt.size();
+      rect_part_max = std::max(rect_part_max, (lr.end - lr.begin) * rows);
+      ls.begin = (long long)fwt.size();
+      fwt.insert(fwt.end(), sums.begin(), sums.end());
+      ls.end = (long long)fwt.size();
     }
-    L.fwd_rect.end = (long long)fwt.size();
-    rect_part_max = std::max(rect_part_max, L.fwd_rect.end - L.fwd_rect.begin);
-    L.fwd_rect_sum.begin = (long long)fwt.size();
-    fwt.insert(fwt.end(), sums.begin(), sums.end());
-    L.fwd_rect_sum.end = (long long)fwt.size();
     // backward
     L.bwd_xb.begin = (long long)bwt.size();
     for (int s : big)
@@ -1585,7 +1696,7 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   level_sn_.upload(lvl_sn, stream_);
   V_.alloc(std::max<long long>(1, S.Vsize));
   W_.alloc(std::max<long long>(1, Woff[ns]));
-  Pr_.alloc(std::max<long long>(1, rect_part_max * NB));
+  Pr_.alloc(std::max<long long>(1, rect_part_max));
   y_.alloc(n);
   err_.alloc(1);
   sd_ = SymDev{sn_first_.p, sn_rowptr_.p, sn_rows_.p, Loff_.p, Uoff_.p, Voff_.p, Woff_.p, rel_off_.p, rel_.p,
@@ -1781,11 +1892,16 @@ void SpdDevice::sweeps(const T* L) {
       launch_n(k_fwdb_diag<T>, DIAG_WAVES * 64, lv.fwd_diag[t], fwd_tasks_.p, XTs, y_.p, (const double*)W_.p);
       launch_n(k_fwdb_panel<T>, RECT_WAVES * 64, lv.fwd_panel[t], fwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
-    if (lv.fwd_rect.end > lv.fwd_rect.begin)
-      hipLaunchKernelGGL(k_fwdb_rect<T>, dim3((unsigned)(lv.fwd_rect.end - lv.fwd_rect.begin)), dim3(RECT_WAVES * 64),
-                         0, stream_, fwd_tasks_.p + lv.fwd_rect.begin, 0LL, sd_, L, (const double*)y_.p,
-                         (const double*)W_.p, V_.p, Pr_.p);
-    launch_n(k_fwdb_rect_sum, 64, lv.fwd_rect_sum, fwd_tasks_.p, (const double*)W_.p, V_.p, (const double*)Pr_.p);
+    // the FP32 sweeps take the update-vector rows four per lane (RECT_ROWS-row tasks), the FP64 ones one per lane
+    const LaunchList& lr = sizeof(T) == 4 ? lv.fwd_rect4 : lv.fwd_rect;
+    if (lr.end > lr.begin)
+      hipLaunchKernelGGL(k_fwdb_rect<T>, dim3((unsigned)(lr.end - lr.begin)), dim3(RECT_WAVES * 64), 0, stream_,
+                         fwd_tasks_.p + lr.begin, 0LL, sd_, L, (const double*)y_.p, (const double*)W_.p, V_.p, Pr_.p);
+    if (sizeof(T) == 4)
+      launch_n(k_fwdb_rect_sum<RECT_ROWS>, RECT_ROWS, lv.fwd_rect4_sum, fwd_tasks_.p, (const double*)W_.p, V_.p,
+               (const double*)Pr_.p);
+    else
+      launch_n(k_fwdb_rect_sum<NB>, NB, lv.fwd_rect_sum, fwd_tasks_.p, (const double*)W_.p, V_.p, (const double*)Pr_.p);
   }
   for (size_t l = levels_.size(); l-- > 0;) {
     const Level& lv = levels_[l];

@@ -649,6 +649,7 @@ class Forging3DSolver:
         self.eps, self.el_sig, self.el_rate = eps, sig, rate
         self.warm = True
         t3 = time.perf_counter()
+        st["repair"] = tuple(self.last_repair)
         return IncrementResult(self.step_index, load, reaction, st["iterations"], bool(st["converged"]), dt, self.gap,
                                st, 1e3 * ((t1 - t0) + (t3 - t2)), 1e3 * (t2 - t1))
 
