@@ -1,4 +1,4 @@
-"""ctypes binding of libifea_amd.so (include/ifea_spd.h, include/ifea_step.h).
+"""ctypes binding of libifea_amd.so (include/ifea_spd.h, include/ifea_step.h, include/ifea_remesh.h).
 
 The library is built in-tree by ``impressionfea_amd.build``. There is no CPU fallback: ``lib()`` raises
 ``EngineUnavailableError`` when the shared object is missing, and the entry points return IFEA_ERR_NO_DEVICE when no
@@ -29,6 +29,10 @@ STEP_SYMBOLS = ["ifea_step_create", "ifea_step_set_params", "ifea_step_set_optio
                 "ifea_step_advance", "ifea_step_element_results", "ifea_step_set_damage", "ifea_step_get_v",
                 "ifea_step_set_v", "ifea_step_quality", "ifea_step_quality_noalloc", "ifea_step_info",
                 "ifea_step_destroy"]
+
+REMESH_SYMBOLS = ["ifea_locator_create", "ifea_locator_find", "ifea_locator_find_nearest", "ifea_locator_interpolate",
+                  "ifea_locator_destroy", "ifea_stuff", "ifea_stuffing_sizes", "ifea_stuffing_get",
+                  "ifea_stuffing_destroy", "ifea_remesh_last_error"]
 
 IFEA_OK = 0
 IFEA_ERR_ARG = -1
@@ -124,6 +128,18 @@ def lib():
     L.ifea_step_info.argtypes = [vp, C.c_int, dp]
     L.ifea_step_destroy.argtypes = [vp]
     L.ifea_step_destroy.restype = None
+    L.ifea_locator_create.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, C.POINTER(vp)]
+    L.ifea_locator_find.argtypes = [vp, C.c_int, dp, ip, dp]
+    L.ifea_locator_find_nearest.argtypes = [vp, C.c_int, dp, ip]
+    L.ifea_locator_interpolate.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.ifea_locator_destroy.argtypes = [vp]
+    L.ifea_locator_destroy.restype = None
+    L.ifea_stuff.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]
+    L.ifea_stuffing_sizes.argtypes = [vp, ip, ip]
+    L.ifea_stuffing_get.argtypes = [vp, dp, ip]
+    L.ifea_stuffing_destroy.argtypes = [vp]
+    L.ifea_stuffing_destroy.restype = None
+    L.ifea_remesh_last_error.restype = C.c_char_p
     _lib = L
     return L
 

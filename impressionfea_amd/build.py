@@ -1,7 +1,7 @@
 """Build libifea_amd.so (HIP for gfx950) in-tree: impressionfea_amd/_lib/libifea_amd.so.
 
 Every HIP source is compiled with hipcc for gfx950 only, with packed-FP32 instruction selection disabled; the
-host-only symbolic analysis is compiled by hipcc as plain C++. Objects go to impressionfea_amd/_lib/obj.
+host-only symbolic analysis and remesh geometry are compiled by hipcc as plain C++. Objects go to impressionfea_amd/_lib/obj.
 """
 from __future__ import annotations
 
@@ -17,7 +17,7 @@ LIB = os.path.join(OUT, "libifea_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["symbolic.cpp", "vec.hip", "spd_device.hip", "step_device.hip", "mlp.hip", "api.hip"]
+SOURCES = ["symbolic.cpp", "remesh.cpp", "vec.hip", "spd_device.hip", "step_device.hip", "mlp.hip", "api.hip"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-Wno-unused-function",
           "-Wno-unused-variable"]
 HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -97,6 +97,8 @@ class StepConfig:
     quadratic_ls: bool = True
     adaptive_newton: bool = True
     extrapolate_start: bool = False
+    remesh: bool = False           # remesh when the mesh degrades, retry a failed update on a new mesh (:545, :600-621)
+    remesh_quality: float = 0.12   # SimulationConfig3D.RemeshQuality
     sym_factor: int = 1
     bottom: DieSpec = field(default_factory=lambda: DieSpec(0, 0, (0.0,) * 8))
     top: DieSpec = field(default_factory=lambda: DieSpec(1, 0, (20.0,) + (0.0,) * 7))

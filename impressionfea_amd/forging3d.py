@@ -2,7 +2,7 @@
 
 Mirrors src/ImpressionFEA.Core/ThreeD/Forging3DSolver.cs:
   constructor constants and initial contact/velocity   :125-187
-  Step (one stroke increment, no remeshing)             :540-638
+  Step (one stroke increment and its recovery ladder)   :540-638 (remesh: remesh.py)
   PrepareStep host part: nodal frames and fixed DOFs    :660-727   (vectorised numpy; die geometry is host side)
   LayerLimitedIncrement                                 :236-249
   TryUpdate (contact, placement and its repair)         :1455-1537, ImprovePlacement :1625-1679, SmoothInterior
@@ -18,6 +18,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
+from . import placement as PL
+from . import remesh as RM
+from .placement import tet_quality
 from .config import StepConfig, Workload
 from .errors import MeshDistortionError, SolverDivergenceError, raise_for
 from .mesh import TetMesh, build_pattern
@@ -116,18 +119,6 @@ def compute_frames(x, onX0, onY0, contact, gap, cfg: StepConfig):
     return e, mc.astype(np.int32), die_axis.astype(np.int32), fixed_val
 
 
-def tet_quality(x, tets):
-    """TetMesh.Quality (TetMesh.cs:107-112) of the given tets in the reference's operation order: 6 sqrt(2) V / l2^1.5
-    with l2 the mean squared edge length (1 regular, <= 0 inverted, -1 degenerate)."""
-    a, b, c, d = (x[tets[..., k]] for k in range(4))
-    l2 = (_dot3(b - a, b - a) + _dot3(c - a, c - a) + _dot3(d - a, d - a) + _dot3(c - b, c - b) + _dot3(d - b, d - b)
-          + _dot3(d - c, d - c)) / 6.0
-    vol = _dot3(b - a, np.cross(c - a, d - a)) / 6.0
-    with np.errstate(divide="ignore", invalid="ignore"):
-        q = 6 * np.sqrt(2.0) * vol / np.power(l2, 1.5)
-    return np.where(l2 > 0, q, -1.0)
-
-
 @dataclass
 class IncrementResult:
     step: int
@@ -143,17 +134,11 @@ class IncrementResult:
 
 
 class Forging3DSolver:
-    """One billet on a fixed mesh, stepped by increments on the GPU (Forging3DSolver.Step without remeshing)."""
+    """One billet stepped by increments on the GPU (Forging3DSolver.Step); remeshing when cfg.remesh is set."""
 
     def __init__(self, cfg: StepConfig, x, tets, onX0, onY0, gap0: float, precision: int = 0, perm=None,
                  warm_start=None, first_solve_policy: bool = False, device_frames: bool = True):
         self.cfg = cfg
-        self.x = np.array(x, dtype=np.float64)
-        self.mesh = TetMesh(self.x.shape[0], tets, np.asarray(onX0, bool), np.asarray(onY0, bool))
-        m = self.mesh
-        self.nn, self.ne, self.ndof = m.nn, m.ne, 3 * m.nn
-        self.nbr_start, self.nbr, self.colPtr, self.rowIdx = build_pattern(m)
-        self.nnz = int(self.colPtr[-1])
         consts = cfg.constants()
         self.rate_ref, self.rate_cut, self.sig_ref = consts["rateRef"], consts["rateCut"], consts["sigRef"]
         self.k_pen, self.u0 = consts["kPen"], consts["u0"]
@@ -165,19 +150,32 @@ class Forging3DSolver:
         self.time = 0.0
         self.warm_start = warm_start
         self.vg_prev = None
-        # LocalLengths (:442-466)
-        t = m.tets
-        sums = np.zeros(self.nn)
-        cnt = np.zeros(self.nn)
-        for a in range(4):
-            for b in range(a + 1, 4):
-                ln = np.linalg.norm(self.x[t[:, a]] - self.x[t[:, b]], axis=1)
-                np.add.at(sums, t[:, a], ln)
-                np.add.at(sums, t[:, b], ln)
-                np.add.at(cnt, t[:, a], 1)
-                np.add.at(cnt, t[:, b], 1)
-        self.h_local = np.where(cnt > 0, np.minimum(sums / np.maximum(cnt, 1), cfg.h), cfg.h)
-        # device step
+        self.recovery_pass = 0     # Step's recovery ladder (:553-621): 0 as is, 1 after a remesh
+        self.precision = precision
+        self._first_solve = bool(first_solve_policy)
+        # PrepareStep's frames on the device (the host's compute_frames is the same arithmetic, kept for checks)
+        self.device_frames = bool(device_frames)
+        self._h = None
+        self._setup_mesh(x, tets, onX0, onY0, perm)
+        # the initial volume (:184) and the model's symmetry planes, kept across remeshes
+        self.vol0 = float(np.cumsum(PL.tet_volumes(self.x, self.mesh.tets))[-1]) * cfg.sym_factor
+        self.symX, self.symY = bool(np.any(self.mesh.onX0)), bool(np.any(self.mesh.onY0))
+        self.remesh_count = 0
+        self.remesh_log = []       # (increment, reason, stats or None when the new mesh was rejected)
+        self.ms_remesh = 0.0
+
+    def _setup_mesh(self, x, tets, onX0, onY0, perm=None):
+        """Allocate (:403-438): the mesh, its pattern and local lengths, and a device step for them; per-node and
+        per-element state starts empty."""
+        cfg = self.cfg
+        self.close()
+        self.x = np.array(x, dtype=np.float64)
+        self.mesh = TetMesh(self.x.shape[0], tets, np.asarray(onX0, bool), np.asarray(onY0, bool))
+        m = self.mesh
+        self.nn, self.ne, self.ndof = m.nn, m.ne, 3 * m.nn
+        self.nbr_start, self.nbr, self.colPtr, self.rowIdx = build_pattern(m)
+        self.nnz = int(self.colPtr[-1])
+        self.h_local = PL.local_lengths(m, self.x, cfg.h)   # LocalLengths (:443-466)
         L = N.lib()
         faces = m.boundary_faces[~m.face_is_symmetry]
         owners = m.boundary_face_owner[~m.face_is_symmetry]
@@ -188,15 +186,12 @@ class Forging3DSolver:
                                 N.ptr(self._keep[4], C.c_int))
         self._check(rc, "ifea_step_create")
         self._h = h
-        self.precision = precision
         self.set_params()
         if perm is not None:
             pm = N.i32(perm)
             self._check(L.ifea_step_set_perm(self._h, N.ptr(pm, C.c_int)), "set_perm")
-        if first_solve_policy:
+        if self._first_solve:
             self._check(L.ifea_step_set_option(self._h, N.IFEA_STEP_OPT_FIRST_SOLVE, 1.0), "set_option")
-        # PrepareStep's frames on the device (the host's compute_frames is the same arithmetic, kept for checks)
-        self.device_frames = bool(device_frames)
         self._sym_flags = [np.ascontiguousarray(m.onX0, np.uint8), np.ascontiguousarray(m.onY0, np.uint8)]
         bp, tp = N.f64(np.asarray(cfg.bottom.p, float)), N.f64(np.asarray(cfg.top.p, float))
         self._check(L.ifea_step_set_geometry(self._h, N.ptr(self._sym_flags[0], C.c_ubyte),
@@ -393,58 +388,6 @@ class Forging3DSolver:
         return pw.value, rc.value
 
     # ------------------------------------------------------------------ geometry update and placement repair
-    def surface_normals(self, x):
-        """SurfaceNormals (:1540-1562): area-weighted outward unit normals at the surface nodes (zero elsewhere), the
-        faces' contributions added in face order."""
-        m = self.mesh
-        f = m.boundary_faces[~m.face_is_symmetry]
-        an = np.cross(x[f[:, 1]] - x[f[:, 0]], x[f[:, 2]] - x[f[:, 0]])
-        # bincount adds its weights in input order: each node sums its faces' contributions in face order
-        idx = f.ravel()
-        nr = np.stack([np.bincount(idx, weights=np.repeat(an[:, c], 3), minlength=self.nn) for c in range(3)], axis=1)
-        nr[m.onX0, 0] = 0.0
-        nr[m.onY0, 1] = 0.0
-        l2 = _dot3(nr, nr)
-        ok = l2 > 0
-        nr[ok] = nr[ok] / np.sqrt(l2[ok])[:, None]
-        return nr
-
-    def _tets_of(self, n):
-        m = self.mesh
-        return m.node_tet_list[m.node_tet_start[n]:m.node_tet_start[n + 1]]
-
-    def _worst_around(self, x, n):
-        return float(tet_quality(x, self.mesh.tets[self._tets_of(n)]).min())
-
-    def _push_out(self, die, off, p, outward):
-        """PushOut (:1570-1592): back to the die surface along the inward normal by bisection within 2 h, else the
-        closest point."""
-        if _dot3(outward, outward) == 0 or die.clearance(p, off) >= 0:
-            return die.project(p, off)[0]
-        d = -outward
-        h = self.cfg.h
-        step, lo, hi = 0.1 * h, 0.0, -1.0
-        t = step
-        while t <= 2 * h + 1e-12:
-            if die.clearance(p + t * d, off) >= 0:
-                hi = t
-                break
-            lo = t
-            t += step
-        if hi < 0:
-            return die.project(p, off)[0]
-        for _ in range(25):
-            mid = 0.5 * (lo + hi)
-            if die.clearance(p + mid * d, off) >= 0:
-                hi = mid
-            else:
-                lo = mid
-        return p + hi * d
-
-    def _sym1(self, p, n):
-        m = self.mesh
-        return np.array([0.0 if m.onX0[n] else p[0], 0.0 if m.onY0[n] else p[1], p[2]])
-
     def _bad_tets(self, x, q_min):
         """Tets with TetMesh.Quality < q_min at x: the device evaluates every tet, the host re-evaluates those near the
         threshold with the reference's arithmetic so the decision is the reference's."""
@@ -455,99 +398,15 @@ class Forging3DSolver:
             return cand
         return cand[tet_quality(x, self.mesh.tets[cand]) < q_min]
 
+    def surface_normals(self, x):
+        return PL.surface_normals(self.mesh, x)
+
     def improve_placement(self, x, contact, pred, nrm, gap, q_min=0.05):
-        """ImprovePlacement (:1625-1679) at the normal pass's tolerances: every die-contact node of a tet below q_min
-        tries the closest die point, the exit along the inward normal and, when it barely penetrates, partial
-        pushes, keeping whichever most raises the worst quality of its tets (tets visited in ascending order, each
-        judged at the current positions). Returns the number of nodes moved; x and contact change in place."""
-        import heapq
-        cfg, m, t = self.cfg, self.mesh, self.mesh.tets
-        total = 0
-        for _ in range(4):
-            count = 0
-            heap = [int(e) for e in self._bad_tets(x, q_min)]
-            heapq.heapify(heap)
-            queued = set(heap)
-            while heap:
-                e = heapq.heappop(heap)
-                if float(tet_quality(x, t[e:e + 1])[0]) >= q_min:
-                    continue
-                for a in range(4):
-                    n = int(t[e, a])
-                    if contact[n] == FREE:
-                        continue
-                    die, off = (cfg.top, gap) if contact[n] == TOP else (cfg.bottom, 0.0)
-                    closest = self._sym1(die.project(pred[n], off)[0], n)
-                    cand = [closest]
-                    push = self._sym1(self._push_out(die, off, pred[n], nrm[n]), n)
-                    if np.sqrt(_dot3(push - pred[n], push - pred[n])) <= 0.5 * self.h_local[n]:
-                        cand.append(push)
-                    pen = -float(die.clearance(pred[n], off))
-                    for f in (0.0, 0.25, 0.5, 0.75):
-                        if pen * (1 - f) <= 0.05 * self.h_local[n]:
-                            cand.append(pred[n] + f * (closest - pred[n]))
-                    old = x[n].copy()
-                    best = self._worst_around(x, n)
-                    pick = -1
-                    for ci, cpt in enumerate(cand):
-                        if np.array_equal(cpt, old):
-                            continue
-                        x[n] = cpt
-                        q = self._worst_around(x, n)
-                        if q > best + 1e-6:
-                            best, pick = q, ci
-                    x[n] = cand[pick] if pick >= 0 else old
-                    if pick < 0:
-                        continue
-                    if np.array_equal(cand[pick], pred[n]) and pen < 0:
-                        contact[n] = FREE
-                    count += 1
-                    for e2 in self._tets_of(n):       # tets after e whose quality this move may have changed
-                        if e2 > e and int(e2) not in queued:
-                            queued.add(int(e2))
-                            heapq.heappush(heap, int(e2))
-            total += count
-            if count == 0:
-                break
-        return total
+        return PL.improve_placement(self.mesh, x, contact, pred, nrm, gap, q_min, self.h_local, self.cfg,
+                                    self._bad_tets, self.recovery_pass)
 
     def smooth_interior(self, x, q_target=0.05, sweeps=6):
-        """SmoothInterior (:1776-1838): each interior node of a tet below q_target moves towards its neighbours'
-        centroid (full, half or quarter step) when that raises the worst quality of its tets; nodes in ascending
-        order at the current positions; symmetry-plane nodes stay on their planes. Returns the number of moves."""
-        m, t = self.mesh, self.mesh.tets
-        moves = 0
-        for _ in range(sweeps):
-            bad_t = self._bad_tets(x, q_target)
-            if bad_t.size == 0:
-                break
-            nodes = np.unique(t[bad_t].ravel())
-            nodes = nodes[~m.is_surface_node[nodes]]
-            sweep_moves = 0
-            for n in nodes:
-                n = int(n)
-                te = self._tets_of(n)
-                tt = t[te]
-                others = tt[tt != n]
-                if others.size == 0:
-                    continue
-                s = np.zeros(3)
-                for q in others:                    # the reference's running sum, in its order
-                    s = s + x[q]
-                worst = float(tet_quality(x, tt).min())
-                old = x[n].copy()
-                c = (1.0 / others.size) * s
-                for lam in (1.0, 0.5, 0.25):
-                    p = old + lam * (c - old)
-                    x[n] = self._sym1(p, n)
-                    if float(tet_quality(x, tt).min()) > worst + 1e-6:
-                        sweep_moves += 1
-                        break
-                    x[n] = old
-            moves += sweep_moves
-            if sweep_moves == 0:
-                break
-        return moves
+        return PL.smooth_interior(self.mesh, x, q_target, sweeps, self._bad_tets)
 
     def try_update(self, dt: float) -> str:
         """TryUpdate (:1455-1537): predicted positions, contact and placement, the placement repair (ImprovePlacement,
@@ -607,38 +466,91 @@ class Forging3DSolver:
         self.time += dt
         return ""
 
+    # ------------------------------------------------------------------ remeshing (:1867-1880, :1945-2102)
+    def remesh_trigger(self):
+        """RemeshTrigger (:1867-1880): interior smoothing of the current mesh, then the reason to remesh, or None."""
+        return RM.remesh_trigger(self.mesh, self.x, self.cfg.remesh_quality, self.cfg.h)
+
+    def remesh(self, reason: str) -> bool:
+        """Remesh / RemeshCore (:1945-2102): a stuffed mesh of the current workpiece with contact, placement repair and
+        the state remapped; a new device step for it (its pattern analysed at the next factorisation). False when the
+        new mesh is rejected (the state stays on the old one). The remapped element mean stress is not put on the
+        device: the next FinalEvaluation writes it before it is read."""
+        t0 = time.perf_counter()
+        er = self.element_results()
+        state = dict(eps=self.eps, dmg=er["dmg"], sig=self.el_sig, rate=self.el_rate, mean=er["elMean"], vg=self.vg)
+        res = RM.remesh_core(self.cfg, self.mesh, self.x, state, self.gap, self.vol0, self.cfg.sym_factor,
+                             self.sig_ref, self.symX, self.symY, self.recovery_pass)
+        if res is None:
+            self.remesh_log.append((self.step_index, reason, None))
+            self.ms_remesh += 1e3 * (time.perf_counter() - t0)
+            return False
+        self._setup_mesh(res.x, res.tets, res.onX0, res.onY0)
+        self.contact = np.asarray(res.contact, np.int32).copy()
+        self.vg = res.vg
+        self.set_element_state(res.eps, res.sig)
+        self.el_rate = np.asarray(res.rate, np.float64)
+        self.set_damage(res.dmg)
+        self.warm = False
+        self.vg_prev = None
+        self.remesh_count += 1
+        self.remesh_log.append((self.step_index, reason, res.stats))
+        self.ms_remesh += 1e3 * (time.perf_counter() - t0)
+        return True
+
     def step(self) -> IncrementResult:
-        """One stroke increment (Forging3DSolver.Step :540-638) on the fixed mesh."""
+        """One stroke increment (Forging3DSolver.Step :540-638). With cfg.remesh the remesh trigger runs first and a
+        failed geometry update (or a stalled velocity solution) is retried once on a new mesh with the corner allowance
+        of the first recovery pass (the ladder's refinement passes are not built); otherwise the mesh stays fixed."""
         cfg = self.cfg
         self.step_index += 1
+        self.ms_remesh = 0.0
+        if cfg.remesh:
+            trig = self.remesh_trigger()
+            if trig:
+                self.remesh(trig)
+        passes = 2 if cfg.remesh else 1
+        ok, why = False, ""
         t0 = time.perf_counter()
-        judged = None
-        if self.warm_start is not None and self.warm:
-            judged = self._learned_start()
-        else:
-            self.prepare()
-        st = self.solve_velocity()
-        trend = cfg.extrapolate_start and self.warm and self.vg_prev is not None
-        st["start"] = judged or ("trend" if trend else "previous")
-        if not st["converged"] and st["last_rel"] >= 1e-2:
-            raise SolverDivergenceError(f"Velocity solution did not converge at step {self.step_index}: "
-                                        f"|dv|/|v| = {st['last_rel']:.2E}.")
-        power, reaction = self.final_evaluation()
-        load = power / cfg.ram * cfg.sym_factor
-        reaction *= cfg.sym_factor
-        t1 = time.perf_counter()
-        ds = min(min(self.ds, cfg.stroke_increment), self.layer_limited_increment())
-        dt = min(ds, self.gap) / cfg.ram
-        if self.gap - cfg.ram * dt < 0.05 * ds:
-            dt = self.gap / cfg.ram
-        why = ""
-        for attempt in range(6):
-            dta = dt / (1 << attempt)
-            why = self.try_update(dta)
-            if not why:
-                dt = dta
+        ms_pass = 0.0
+        for pas in range(passes):
+            self.recovery_pass = pas
+            t0 = time.perf_counter()
+            judged = None
+            if self.warm_start is not None and self.warm:
+                judged = self._learned_start()
+            else:
+                self.prepare()
+            st = self.solve_velocity()
+            trend = cfg.extrapolate_start and self.warm and self.vg_prev is not None
+            st["start"] = judged or ("trend" if trend else "previous")
+            if not st["converged"] and st["last_rel"] >= 1e-2:
+                if pas < passes - 1 and self.remesh("stalled velocity solution"):
+                    continue
+                self.recovery_pass = 0
+                raise SolverDivergenceError(f"Velocity solution did not converge at step {self.step_index}: "
+                                            f"|dv|/|v| = {st['last_rel']:.2E}.")
+            power, reaction = self.final_evaluation()
+            load = power / cfg.ram * cfg.sym_factor
+            reaction *= cfg.sym_factor
+            t1 = time.perf_counter()
+            ds = min(min(self.ds, cfg.stroke_increment), self.layer_limited_increment())
+            dt = min(ds, self.gap) / cfg.ram
+            if self.gap - cfg.ram * dt < 0.05 * ds:
+                dt = self.gap / cfg.ram
+            for attempt in range(6):
+                dta = dt / (1 << attempt)
+                why = self.try_update(dta)
+                if not why:
+                    dt = dta
+                    ok = True
+                    break
+            if ok or pas == passes - 1:
                 break
-        if why:
+            ms_pass += 1e3 * (t1 - t0)
+            self.remesh("fold during update")   # a rejected remesh still leaves the next pass
+        self.recovery_pass = 0
+        if not ok:
             raise MeshDistortionError(f"Mesh distortion at step {self.step_index}: {why}.")
         t2 = time.perf_counter()
         eps = np.zeros(self.ne)
@@ -650,8 +562,10 @@ class Forging3DSolver:
         self.warm = True
         t3 = time.perf_counter()
         st["repair"] = tuple(self.last_repair)
+        st["remesh_ms"] = self.ms_remesh
+        st["remeshes"] = self.remesh_count
         return IncrementResult(self.step_index, load, reaction, st["iterations"], bool(st["converged"]), dt, self.gap,
-                               st, 1e3 * ((t1 - t0) + (t3 - t2)), 1e3 * (t2 - t1))
+                               st, ms_pass + 1e3 * ((t1 - t0) + (t3 - t2)), 1e3 * (t2 - t1))
 
     # ------------------------------------------------------------------ parity helpers
     def assemble(self, v, matrix=True, newton=True):
@@ -705,13 +619,26 @@ class Forging3DSolver:
             h=float(self.cfg.h), step=int(self.step_index))
 
     def restore_from(self, cp, warm: bool = False):
-        """RestoreFrom (:1899-1921) on this solver's own mesh: positions, velocities, contact, element history, gap,
-        increment and time. Like the reference the next increment starts cold (Picard first, no trend history) unless
-        `warm` keeps the checkpoint's trend history, which replays a run's continuation. A checkpoint of another mesh
-        needs the reference's remesh and remap, which this solver does not do."""
+        """RestoreFrom (:1899-1921): positions, velocities, contact, element history, gap, increment and time. Like the
+        reference the next increment starts cold (Picard first, no trend history). A checkpoint of another mesh (or any
+        checkpoint when cfg.remesh is set) is remeshed at this solver's h and remapped, as the reference always does;
+        one of this solver's own mesh with remeshing off is restored onto it as it is, and then `warm` may keep its
+        trend history, which replays a run's continuation."""
         m = self.mesh
-        if not (np.array_equal(cp.tets, m.tets) and np.array_equal(cp.on_x0, m.onX0) and np.array_equal(cp.on_y0, m.onY0)):
-            raise NotImplementedError("the checkpoint holds another mesh: restoring it needs a remesh and remap")
+        same = np.array_equal(cp.tets, m.tets) and np.array_equal(cp.on_x0, m.onX0) and np.array_equal(cp.on_y0, m.onY0)
+        if not same or self.cfg.remesh:
+            self._setup_mesh(cp.x, cp.tets, cp.on_x0, cp.on_y0)
+            self.vg = np.array(cp.vg, dtype=np.float64)
+            self.contact = np.array(cp.contact, dtype=np.int32)
+            self.set_element_state(cp.eps, cp.sigma)
+            self.el_rate = np.array(cp.rate, dtype=np.float64)
+            self.set_damage(cp.damage)
+            self.gap, self.time, self.step_index = float(cp.gap), float(cp.time), int(cp.step)
+            self.ds = min(float(cp.ds) * self.cfg.h / float(cp.h), self.cfg.stroke_increment)
+            self.vg_prev, self.warm, self.n_converged = None, False, 1
+            if not self.remesh(f"restart from step {cp.step} at h {self.cfg.h:.2f} mm (was {cp.h:.2f})"):
+                raise MeshDistortionError(f"Could not remesh the checkpoint of step {cp.step} at h {self.cfg.h:.2f} mm.")
+            return
         self.x = np.array(cp.x, dtype=np.float64)
         self.vg = np.array(cp.vg, dtype=np.float64)
         self.contact = np.array(cp.contact, dtype=np.int32)

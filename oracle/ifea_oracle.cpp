@@ -357,6 +357,270 @@ std::vector<int> min_degree(int n, const std::vector<int>& adjp, const std::vect
 }
 
 // ---------------------------------------------------------------------------------------------- the solver
+// TetLocator (TetMesh.cs:115-262): bucket grid of the tets' bounding boxes; Find tests a cell's tets in ascending order.
+struct OLocator {
+  const std::vector<int>* T = nullptr;
+  const std::vector<V3>* X = nullptr;
+  double cell = 1;
+  V3 mn;
+  int nx = 0, ny = 0, nz = 0;
+  std::vector<int> start, items;
+  int Cx(double v) const { return ClampIdx((v - mn.x) / cell, nx); }
+  int Cy(double v) const { return ClampIdx((v - mn.y) / cell, ny); }
+  int Cz(double v) const { return ClampIdx((v - mn.z) / cell, nz); }
+  static int ClampIdx(double q, int n) {
+    // Math.Clamp((int)q, 0, n - 1) for the range the grid covers
+    if (!(q >= 0)) return 0;
+    if (q >= n - 1) return n - 1;
+    return (int)q;
+  }
+  static double Vol(const V3& a, const V3& b, const V3& c, const V3& d) { return (b - a).dot((c - a).cross(d - a)) / 6.0; }
+  void Build(const std::vector<int>& tets, const std::vector<V3>& x, double c) {
+    T = &tets; X = &x; cell = c;
+    double x0 = 1e308, y0 = 1e308, z0 = 1e308, x1 = -1e308, y1 = -1e308, z1 = -1e308;
+    for (const V3& p : x) {
+      x0 = std::min(x0, p.x); x1 = std::max(x1, p.x);
+      y0 = std::min(y0, p.y); y1 = std::max(y1, p.y);
+      z0 = std::min(z0, p.z); z1 = std::max(z1, p.z);
+    }
+    mn = V3(x0 - c, y0 - c, z0 - c);
+    nx = (int)((x1 - mn.x) / c) + 2;
+    ny = (int)((y1 - mn.y) / c) + 2;
+    nz = (int)((z1 - mn.z) / c) + 2;
+    int ne = (int)(tets.size() / 4);
+    std::vector<int> count((size_t)nx * ny * nz + 1, 0);
+    auto each = [&](int e, bool fillPass, std::vector<int>& fill) {
+      V3 lo = x[tets[4 * e]], hi = lo;
+      for (int a = 1; a < 4; a++) {
+        const V3& p = x[tets[4 * e + a]];
+        lo = V3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+        hi = V3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+      }
+      for (int k = Cz(lo.z); k <= Cz(hi.z); k++)
+        for (int j = Cy(lo.y); j <= Cy(hi.y); j++)
+          for (int i = Cx(lo.x); i <= Cx(hi.x); i++) {
+            int cc = (k * ny + j) * nx + i;
+            if (fillPass) items[fill[cc]++] = e; else count[cc + 1]++;
+          }
+    };
+    std::vector<int> dummy;
+    for (int e = 0; e < ne; e++) each(e, false, dummy);
+    for (size_t i = 0; i + 1 < count.size(); i++) count[i + 1] += count[i];
+    start = count;
+    items.assign(count.back(), 0);
+    std::vector<int> fill = count;
+    for (int e = 0; e < ne; e++) each(e, true, fill);
+  }
+  int Find(const V3& p, double* w) const {
+    int c = (Cz(p.z) * ny + Cy(p.y)) * nx + Cx(p.x);
+    const std::vector<int>& t = *T;
+    const std::vector<V3>& x = *X;
+    for (int k = start[c]; k < start[c + 1]; k++) {
+      int e = items[k];
+      V3 a = x[t[4 * e]], b = x[t[4 * e + 1]], cc = x[t[4 * e + 2]], d = x[t[4 * e + 3]];
+      double v = Vol(a, b, cc, d);
+      if (v <= 0) continue;
+      double b0 = Vol(p, b, cc, d) / v, b1 = Vol(a, p, cc, d) / v, b2 = Vol(a, b, p, d) / v;
+      double b3 = 1 - b0 - b1 - b2;
+      if (b0 >= -1e-9 && b1 >= -1e-9 && b2 >= -1e-9 && b3 >= -1e-9) {
+        if (w) { w[0] = b0; w[1] = b1; w[2] = b2; w[3] = b3; }
+        return e;
+      }
+    }
+    return -1;
+  }
+  bool Contains(const V3& p) const { return Find(p, nullptr) >= 0; }
+  V3 Interpolate(const V3& p, const std::vector<V3>& field) const {
+    double w[4];
+    int e = Find(p, w);
+    const std::vector<int>& t = *T;
+    if (e >= 0) return w[0] * field[t[4 * e]] + w[1] * field[t[4 * e + 1]] + w[2] * field[t[4 * e + 2]] + w[3] * field[t[4 * e + 3]];
+    int best = 0;
+    double dmin = 1e308;
+    for (size_t n = 0; n < X->size(); n++) {
+      V3 d = (*X)[n] - p;
+      double dd = d.x * d.x + d.y * d.y + d.z * d.z;
+      if (dd < dmin) { dmin = dd; best = (int)n; }
+    }
+    return field[best];
+  }
+  int FindNearest(const V3& p) const {
+    int e = Find(p, nullptr);
+    if (e >= 0) return e;
+    int c = (Cz(p.z) * ny + Cy(p.y)) * nx + Cx(p.x);
+    const std::vector<int>& t = *T;
+    const std::vector<V3>& x = *X;
+    double best = 1e308;
+    int bestE = -1;
+    auto test = [&](int q) {
+      V3 cen = 0.25 * (x[t[4 * q]] + x[t[4 * q + 1]] + x[t[4 * q + 2]] + x[t[4 * q + 3]]);
+      V3 d = cen - p;
+      double dd = d.x * d.x + d.y * d.y + d.z * d.z;
+      if (dd < best) { best = dd; bestE = q; }
+    };
+    for (int k = start[c]; k < start[c + 1]; k++) test(items[k]);
+    if (bestE >= 0) return bestE;
+    for (int q = 0; q < (int)(t.size() / 4); q++) test(q);
+    return bestE;
+  }
+};
+
+// IsosurfaceStuffing.Stuff / StuffLattice (IsosurfaceStuffing.cs:17-88, :115-294) with the inside test In(p).
+template <typename InFn>
+void OStuff(InFn In, const V3& mn, const V3& mx, double h, bool symX, bool symY, std::vector<V3>& outX,
+            std::vector<int>& outT) {
+  const double alphaLong = 0.24999, alphaShort = 0.41189;
+  V3 off(0.31 * h, 0.37 * h, 0.17 * h);
+  int i0 = (int)std::floor(mn.x / h) - 1, i1 = (int)std::ceil(mx.x / h) + 1;
+  int j0 = (int)std::floor(mn.y / h) - 1, j1 = (int)std::ceil(mx.y / h) + 1;
+  int k0 = (int)std::floor(mn.z / h) - 1, k1 = (int)std::ceil(mx.z / h) + 1;
+  int ni = i1 - i0, nj = j1 - j0, nk = k1 - k0;
+  int nCorner = (ni + 1) * (nj + 1) * (nk + 1);
+  auto Corner = [&](int i, int j, int k) { return ((k - k0) * (nj + 1) + (j - j0)) * (ni + 1) + (i - i0); };
+  auto Center = [&](int i, int j, int k) { return nCorner + ((k - k0) * nj + (j - j0)) * ni + (i - i0); };
+  std::vector<V3> pos;
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) pos.push_back(V3(i * h, j * h, k * h) + off);
+  for (int k = k0; k < k1; k++)
+    for (int j = j0; j < j1; j++)
+      for (int i = i0; i < i1; i++) pos.push_back(V3((i + 0.5) * h, (j + 0.5) * h, (k + 0.5) * h) + off);
+  std::vector<int> lat;
+  auto AddPair = [&](int c1, int c2, int a, int b, int c, int d) {
+    int q[16] = {c1, c2, a, b, c1, c2, b, c, c1, c2, c, d, c1, c2, d, a};
+    lat.insert(lat.end(), q, q + 16);
+  };
+  for (int k = k0; k < k1; k++)
+    for (int j = j0; j < j1; j++)
+      for (int i = i0; i < i1; i++) {
+        int c = Center(i, j, k);
+        if (i + 1 < i1) AddPair(c, Center(i + 1, j, k), Corner(i + 1, j, k), Corner(i + 1, j + 1, k), Corner(i + 1, j + 1, k + 1), Corner(i + 1, j, k + 1));
+        if (j + 1 < j1) AddPair(c, Center(i, j + 1, k), Corner(i, j + 1, k), Corner(i + 1, j + 1, k), Corner(i + 1, j + 1, k + 1), Corner(i, j + 1, k + 1));
+        if (k + 1 < k1) AddPair(c, Center(i, j, k + 1), Corner(i, j, k + 1), Corner(i + 1, j, k + 1), Corner(i + 1, j + 1, k + 1), Corner(i, j + 1, k + 1));
+      }
+  std::vector<int> sign(pos.size());
+  for (size_t v = 0; v < pos.size(); v++) sign[v] = In(pos[v]) ? -1 : 1;
+  auto isCorner = [&](int v) { return v < nCorner; };
+  auto cross_ = [&](int u, int w) { return (sign[u] < 0 && sign[w] > 0) || (sign[u] > 0 && sign[w] < 0); };
+  // cuts: insertion-ordered
+  std::map<std::pair<int, int>, int> cutIndex;
+  std::vector<std::pair<int, int>> cutKeys;
+  std::vector<V3> cutPos;
+  const int ea[6] = {0, 0, 0, 1, 1, 2}, eb[6] = {1, 2, 3, 2, 3, 3};
+  for (size_t t = 0; t < lat.size(); t += 4)
+    for (int q = 0; q < 6; q++) {
+      int u = lat[t + ea[q]], w = lat[t + eb[q]];
+      if (!cross_(u, w)) continue;
+      std::pair<int, int> key = u < w ? std::make_pair(u, w) : std::make_pair(w, u);
+      if (cutIndex.count(key)) continue;
+      V3 pin = sign[u] < 0 ? pos[u] : pos[w], pout = sign[u] < 0 ? pos[w] : pos[u];
+      for (int it = 0; it < 16; it++) {
+        V3 m = 0.5 * (pin + pout);
+        if (In(m)) pin = m; else pout = m;
+      }
+      V3 cp = 0.5 * (pin + pout);
+      if (symX && std::fabs(cp.x) < 1e-4 * h) cp.x = 0;
+      if (symY && std::fabs(cp.y) < 1e-4 * h) cp.y = 0;
+      cutIndex[key] = (int)cutPos.size();
+      cutKeys.push_back(key);
+      cutPos.push_back(cp);
+    }
+  std::map<int, std::pair<V3, double>> warpTo;
+  for (size_t q = 0; q < cutKeys.size(); q++) {
+    int u = cutKeys[q].first, w = cutKeys[q].second;
+    const V3& c = cutPos[q];
+    double len = (pos[w] - pos[u]).len();
+    double alpha = isCorner(u) == isCorner(w) ? alphaLong : alphaShort;
+    int vv[2] = {u, w};
+    for (int v : vv) {
+      double d = (c - pos[v]).len();
+      if (sign[v] == 0 || d >= alpha * len) continue;
+      auto it = warpTo.find(v);
+      if (it == warpTo.end() || d < it->second.second) warpTo[v] = std::make_pair(c, d);
+    }
+  }
+  for (auto& kv : warpTo) { pos[kv.first] = kv.second.first; sign[kv.first] = 0; }
+  std::map<std::pair<int, int>, int> cutId;
+  auto CutVertex = [&](int u, int w) {
+    std::pair<int, int> key = u < w ? std::make_pair(u, w) : std::make_pair(w, u);
+    auto it = cutId.find(key);
+    if (it != cutId.end()) return it->second;
+    pos.push_back(cutPos[cutIndex.at(key)]);
+    sign.push_back(0);
+    int id = (int)pos.size() - 1;
+    cutId[key] = id;
+    return id;
+  };
+  double minVol = 1e-6 * h * h * h;
+  std::vector<int> outTets;
+  auto Emit = [&](int a, int b, int c, int d) {
+    double v = OLocator::Vol(pos[a], pos[b], pos[c], pos[d]);
+    if (std::fabs(v) < minVol) return;
+    if (v < 0) std::swap(c, d);
+    int q[4] = {a, b, c, d};
+    outTets.insert(outTets.end(), q, q + 4);
+  };
+  auto ConeFace = [&](const std::vector<int>& f, int apex) {
+    if (f.size() < 3 || std::find(f.begin(), f.end(), apex) != f.end()) return;
+    int s = 0;
+    for (size_t q = 1; q < f.size(); q++) if (f[q] < f[s]) s = (int)q;
+    int n = (int)f.size();
+    for (int q = 1; q < n - 1; q++) Emit(apex, f[s], f[(s + q) % n], f[(s + q + 1) % n]);
+  };
+  auto OrderPlanar = [&](std::vector<int>& f, const std::vector<int>& all) {
+    V3 c;
+    for (int x : f) c = c + pos[x];
+    c = (1.0 / f.size()) * c;
+    V3 inner;
+    int cnt = 0;
+    for (int x : all) if (sign[x] < 0) { inner = inner + pos[x]; cnt++; }
+    V3 nrm = (c - (1.0 / std::max(cnt, 1)) * inner).normalized();
+    V3 u = (std::fabs(nrm.x) < 0.9 ? V3(1, 0, 0) : V3(0, 1, 0)).cross(nrm).normalized();
+    V3 w = nrm.cross(u);
+    std::vector<std::pair<double, int>> key;
+    for (int x : f) key.push_back(std::make_pair(std::atan2((pos[x] - c).dot(w), (pos[x] - c).dot(u)), x));
+    // ties keep their order (the reference's List.Sort leaves the order of equal angles unspecified)
+    std::stable_sort(key.begin(), key.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first < b.first; });
+    for (size_t q = 0; q < f.size(); q++) f[q] = key[q].second;
+  };
+  const int faces[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  for (size_t t = 0; t < lat.size(); t += 4) {
+    int v[4] = {lat[t], lat[t + 1], lat[t + 2], lat[t + 3]};
+    int neg = 0, posCount = 0;
+    for (int x : v) { if (sign[x] < 0) neg++; else if (sign[x] > 0) posCount++; }
+    if (neg == 0) continue;
+    if (posCount == 0) { Emit(v[0], v[1], v[2], v[3]); continue; }
+    std::vector<int> poly, iface;
+    for (int x : v) { if (sign[x] <= 0) poly.push_back(x); if (sign[x] == 0) iface.push_back(x); }
+    for (int q = 0; q < 6; q++) {
+      int u = v[ea[q]], w = v[eb[q]];
+      if (cross_(u, w)) { int c = CutVertex(u, w); poly.push_back(c); iface.push_back(c); }
+    }
+    int apex = *std::min_element(poly.begin(), poly.end());
+    for (const auto& f : faces) {
+      std::vector<int> face;
+      for (int q = 0; q < 3; q++) {
+        int a = v[f[q]], b = v[f[(q + 1) % 3]];
+        if (sign[a] <= 0) face.push_back(a);
+        if (cross_(a, b)) face.push_back(CutVertex(a, b));
+      }
+      ConeFace(face, apex);
+    }
+    if (iface.size() >= 3) { OrderPlanar(iface, poly); ConeFace(iface, apex); }
+  }
+  std::map<int, int> map;
+  outX.clear();
+  outT.assign(outTets.size(), 0);
+  for (size_t q = 0; q < outTets.size(); q++) {
+    int old = outTets[q];
+    auto it = map.find(old);
+    int id;
+    if (it == map.end()) { id = (int)outX.size(); map[old] = id; outX.push_back(pos[old]); }
+    else id = it->second;
+    outT[q] = id;
+  }
+}
+
 struct Oracle {
   OrcParams P;
   Law law;
@@ -1098,7 +1362,11 @@ struct Oracle {
     return q;
   }
 
-  // ImprovePlacement (:1625-1679) at the normal pass's tolerances (Tol = the local size, corner allowance 0.05 of it)
+  // CornerAllowance (:1611-1612) in the normal pass and the first recovery pass (the lenient passes are not restated)
+  int recoveryPass = 0;
+  double CornerAllowance(double local) const { return (recoveryPass >= 1 ? 0.1 : 0.05) * local; }
+
+  // ImprovePlacement (:1625-1679) with Tol = the local size (the strict passes)
   int ImprovePlacement(std::vector<V3>& xx, std::vector<int>& cc, const std::vector<V3>& pred, const std::vector<V3>& nrm,
                        double gapT, double qMin) {
     int total = 0;
@@ -1120,7 +1388,7 @@ struct Oracle {
           double pen = -die.clearance(pred[n], off);
           const double fs[4] = {0.0, 0.25, 0.5, 0.75};
           for (double f : fs)
-            if (pen * (1 - f) <= 0.05 * hLocal[n]) cand[k++] = pred[n] + f * (closest - pred[n]);
+            if (pen * (1 - f) <= CornerAllowance(hLocal[n])) cand[k++] = pred[n] + f * (closest - pred[n]);
           V3 old = xx[n];
           double best = WorstAround(xx, n);
           int pick = -1;
@@ -1193,6 +1461,228 @@ struct Oracle {
 
   int lastReverts = 0, lastSmoothMoves = 0;
 
+  // ---------------------------------------------------------------------------------------------- remeshing
+  double vol0 = 0;             // Volume() of the initial mesh (times the symmetry factor), :184
+  bool symX = false, symY = false;
+  int rmImprove = 0, rmReverts = 0, rmSmooth = 0, rmUntangle = 0;
+  double rmQOld = 0, rmQNew = 0;
+
+  static double TetVol(const V3& a, const V3& b, const V3& c, const V3& d) { return (b - a).dot((c - a).cross(d - a)) / 6.0; }
+  double MinQuality(const std::vector<V3>& xx) const {
+    double q = 1e308;
+    for (int e = 0; e < ne; e++) q = std::min(q, TetQ(xx, e));
+    return q;
+  }
+
+  // Allocate (:403-438): a new mesh (its symmetry-plane flags) and positions; every per-node and per-element array is
+  // reset
+  void Allocate(const std::vector<V3>& nx, const std::vector<int>& nt, const std::vector<char>& ox,
+                const std::vector<char>& oy) {
+    nn = (int)nx.size(); ne = (int)(nt.size() / 4); ndof = 3 * nn;
+    tets = nt; x = nx;
+    onX0 = ox; onY0 = oy;
+    vg.assign(nn, V3()); vgPrev.clear();
+    eps.assign(ne, 0); dmg.assign(ne, 0); elDev.assign(6 * (size_t)ne, 0);
+    elSig.assign(ne, 0); elRate.assign(ne, 0); elMean.assign(ne, 0);
+    grad.assign(12 * (size_t)ne, 0); vol.assign(ne, 0);
+    contact.assign(nn, FREE); nFix.assign(nn, 0); dieAxis.assign(nn, -1);
+    fixedv.assign(ndof, 0); fixedVal.assign(ndof, 0); release.assign(nn, 0);
+    kFric.assign(nn, 0); aFric.assign(nn, 0);
+    ax0.assign(nn, V3()); ax1.assign(nn, V3()); ax2.assign(nn, V3());
+    v.assign(ndof, 0); r.assign(ndof, 0); dv.assign(ndof, 0); vTrial.assign(ndof, 0);
+    rvol.assign(ndof, 0); vNode.assign(nn, 0);
+    pr.assign(ndof, 0); pz.assign(ndof, 0); pp.assign(ndof, 0); pq.assign(ndof, 0);
+    eR.assign(12 * (size_t)ne, 0); eEnergy.assign(ne, 0); ePower.assign(ne, 0);
+    BuildTopology();
+    LocalLengths();
+    BuildPattern();
+    needAnalyze = true; haveFactor = false; pcgFailures = pcgSkip = 0;
+    chol = SparseChol();
+    pardiso.reset();
+    useUserPerm = false;
+  }
+
+  // CompensateVolume (:2108-2145): free-surface nodes move along their normals to restore the initial volume
+  void CompensateVolume(std::vector<V3>& xx, const std::vector<int>& cc) const {
+    double vv = 0;
+    for (int e = 0; e < ne; e++) vv += TetVol(xx[tets[4 * e]], xx[tets[4 * e + 1]], xx[tets[4 * e + 2]], xx[tets[4 * e + 3]]);
+    double deficit = vol0 / P.sym_factor - vv;
+    std::vector<V3> nr(nn);
+    for (size_t k = 0; k < fsym.size(); k++) {
+      if (fsym[k]) continue;
+      int a = bfaces[3 * k], b = bfaces[3 * k + 1], c = bfaces[3 * k + 2];
+      V3 an = 0.5 * (xx[b] - xx[a]).cross(xx[c] - xx[a]);
+      nr[a] = nr[a] + an; nr[b] = nr[b] + an; nr[c] = nr[c] + an;
+    }
+    double rate = 0;
+    std::vector<char> use(nn, 0);
+    for (int n = 0; n < nn; n++) {
+      if (!isSurf[n] || cc[n] != FREE) { nr[n] = V3(); continue; }
+      V3 q = nr[n];
+      if (onX0[n]) q.x = 0;
+      if (onY0[n]) q.y = 0;
+      rate += q.len() / 3;
+      nr[n] = q.normalized();
+    }
+    if (rate <= 0) return;
+    double shift = std::min(std::max(deficit / rate, -P.h), P.h);
+    for (int n = 0; n < nn; n++) {
+      if (nr[n].x == 0 && nr[n].y == 0 && nr[n].z == 0) continue;
+      V3 moved = xx[n] + shift * nr[n];
+      if (bottom.clearance(moved, 0) > 0 && top.clearance(moved, gap) > 0) xx[n] = moved;
+    }
+  }
+
+  // UntangleSurface (:1698-1772)
+  int UntangleSurface(std::vector<V3>& xx, std::vector<int>& cc, const std::vector<V3>& stuffed, double qTarget, int sweeps) const {
+    int moves = 0;
+    std::vector<char> bad(nn);
+    for (int s2 = 0; s2 < sweeps; s2++) {
+      std::fill(bad.begin(), bad.end(), 0);
+      bool any = false;
+      for (int e = 0; e < ne; e++) {
+        if (TetQ(xx, e) >= qTarget) continue;
+        for (int a = 0; a < 4; a++) bad[tets[4 * e + a]] = 1;
+        any = true;
+      }
+      if (!any) break;
+      int sweepMoves = 0;
+      for (int n = 0; n < nn; n++) {
+        if (!bad[n] || !isSurf[n]) continue;
+        V3 sum;
+        int cnt = 0;
+        for (int p2 = ntStart[n]; p2 < ntStart[n + 1]; p2++) {
+          int e = ntList[p2];
+          for (int a = 0; a < 4; a++) { int m = tets[4 * e + a]; if (m == n) continue; sum = sum + xx[m]; cnt++; }
+        }
+        if (cnt == 0) continue;
+        V3 old = xx[n];
+        double worst = WorstAround(xx, n);
+        V3 centroid = (1.0 / cnt) * sum;
+        V3 best = old;
+        double bestQ = worst;
+        const V3 targets[2] = {stuffed[n], centroid};
+        const double lams[3] = {1.0, 0.5, 0.25};
+        for (const V3& target : targets)
+          for (double lambda : lams) {
+            V3 p = Sym(old + lambda * (target - old), n);
+            if (std::min(bottom.clearance(p, 0), top.clearance(p, gap)) < 0) continue;
+            xx[n] = p;
+            double q = WorstAround(xx, n);
+            if (q > bestQ + 1e-6) { best = p; bestQ = q; }
+          }
+        xx[n] = best;
+        if (!(best.x == old.x && best.y == old.y && best.z == old.z)) { cc[n] = FREE; sweepMoves++; }
+      }
+      moves += sweepMoves;
+      if (sweepMoves == 0) break;
+    }
+    return moves;
+  }
+
+  // RemeshTrigger (:1867-1880) without the graded-mesh case: interior smoothing first, then a remesh when the worst
+  // tet is still below q or a surface edge is longer than 2.8 h. 0 none, 1 quality, 2 stretched surface.
+  int RemeshTrigger(double remeshQuality) {
+    if (MinQuality(x) < remeshQuality) SmoothInterior(x, remeshQuality, 6);
+    if (MinQuality(x) < remeshQuality) return 1;
+    double maxEdge = 0;
+    for (size_t k = 0; k < bfaces.size(); k += 3) {
+      const V3 &a = x[bfaces[k]], &b = x[bfaces[k + 1]], &c = x[bfaces[k + 2]];
+      maxEdge = std::max(maxEdge, std::max((b - a).len(), std::max((c - b).len(), (a - c).len())));
+    }
+    return maxEdge > 2.8 * P.h ? 2 : 0;
+  }
+
+  // RemeshCore (:1958-2102) with the uniform lattice and no local refinement (RefinementLevels' hot spots and the
+  // graded lattice are not restated): true when the new mesh is accepted and the state moved onto it.
+  bool RemeshCore() {
+    rmImprove = rmReverts = rmSmooth = rmUntangle = 0;
+    rmQOld = MinQuality(x);
+    OLocator loc;
+    loc.Build(tets, x, 2 * P.h);
+    V3 mn = x[0], mx = x[0];
+    for (const V3& q : x) {
+      mn = V3(std::min(mn.x, q.x), std::min(mn.y, q.y), std::min(mn.z, q.z));
+      mx = V3(std::max(mx.x, q.x), std::max(mx.y, q.y), std::max(mx.z, q.z));
+    }
+    const double eps1 = 1e-7 * P.h;
+    auto In = [&](const V3& q) {
+      if ((symX && q.x < 0) || (symY && q.y < 0)) return false;
+      return loc.Contains(V3(symX ? std::max(q.x, eps1) : q.x, symY ? std::max(q.y, eps1) : q.y, q.z));
+    };
+    std::vector<V3> nx;
+    std::vector<int> nt;
+    OStuff(In, mn, mx, P.h, symX, symY, nx, nt);
+    if (nt.empty()) { rmQNew = -1; return false; }
+    // the new mesh's topology and local lengths at the stuffed positions
+    Oracle M;
+    M.P = P; M.bottom = bottom; M.top = top; M.symX = symX; M.symY = symY; M.gap = gap; M.recoveryPass = recoveryPass;
+    M.nn = (int)nx.size(); M.ne = (int)(nt.size() / 4); M.ndof = 3 * M.nn;
+    M.tets = nt; M.x = nx;
+    M.onX0.assign(M.nn, 0); M.onY0.assign(M.nn, 0);
+    for (int n = 0; n < M.nn; n++) { M.onX0[n] = symX && nx[n].x == 0; M.onY0[n] = symY && nx[n].y == 0; }
+    M.vol0 = vol0;
+    M.BuildTopology();
+    if (!M.err.empty()) { rmQNew = -1; return false; }
+    M.LocalLengths();
+    std::vector<V3> xs = nx, stuffed = nx;
+    std::vector<int> cc(M.nn, FREE);
+    std::vector<V3> newNrm = M.SurfaceNormals(xs);
+    const std::vector<double>& hLoc = M.hLocal;
+    for (int n = 0; n < M.nn; n++) {
+      if (!M.isSurf[n]) continue;
+      double cb = bottom.clearance(xs[n], 0), ctp = top.clearance(xs[n], gap), tolC = 0.12 * hLoc[n];
+      if (cb < tolC && cb <= ctp) cc[n] = BOTTOM;
+      else if (ctp < tolC) cc[n] = TOP;
+      if (cc[n] != FREE) {
+        V3 q = Surface(cc[n]).project(xs[n], Offset(cc[n], gap), nullptr);
+        xs[n] = V3(M.onX0[n] ? 0 : q.x, M.onY0[n] ? 0 : q.y, q.z);
+      }
+    }
+    M.CompensateVolume(xs, cc);
+    rmImprove = M.ImprovePlacement(xs, cc, stuffed, newNrm, gap, 0.1);
+    for (int sweep = 0; sweep < 4; sweep++) {
+      int reverted = 0;
+      for (int e = 0; e < M.ne; e++) {
+        if (M.TetQ(xs, e) >= 0.03) continue;
+        for (int a = 0; a < 4; a++) {
+          int n = M.tets[4 * e + a];
+          if (xs[n].x == stuffed[n].x && xs[n].y == stuffed[n].y && xs[n].z == stuffed[n].z) continue;
+          if (std::min(bottom.clearance(stuffed[n], 0), top.clearance(stuffed[n], gap)) < -0.05 * hLoc[n]) continue;
+          xs[n] = stuffed[n];
+          cc[n] = FREE;
+          reverted++;
+        }
+      }
+      rmReverts += reverted;
+      if (reverted == 0) break;
+    }
+    rmSmooth = M.SmoothInterior(xs, 0.2, 10);
+    rmUntangle = M.UntangleSurface(xs, cc, stuffed, 0.03, 6);
+    rmSmooth += M.SmoothInterior(xs, 0.2, 4);
+    rmQNew = M.MinQuality(xs);
+    if (rmQNew < 0.02 && !(rmQNew > 0.002 && rmQNew > rmQOld + 0.002)) return false;
+    // remap: element history from the old tet nearest the new centroid, velocities interpolated
+    std::vector<double> nEps(M.ne), nDmg(M.ne), nSig(M.ne), nRate(M.ne), nMean(M.ne);
+    for (int e = 0; e < M.ne; e++) {
+      V3 c = 0.25 * (xs[nt[4 * e]] + xs[nt[4 * e + 1]] + xs[nt[4 * e + 2]] + xs[nt[4 * e + 3]]);
+      int f = loc.FindNearest(c);
+      nEps[e] = f >= 0 ? eps[f] : 0;
+      nDmg[e] = f >= 0 ? dmg[f] : 0;
+      nSig[e] = f >= 0 ? elSig[f] : sigRef;
+      nRate[e] = f >= 0 ? elRate[f] : 0;
+      nMean[e] = f >= 0 ? elMean[f] : 0;
+    }
+    std::vector<V3> nVg(M.nn);
+    for (int n = 0; n < M.nn; n++) nVg[n] = loc.Interpolate(xs[n], vg);
+    Allocate(xs, nt, M.onX0, M.onY0);  // the symmetry flags of the stuffed points (BuildMesh, :205-206)
+    eps = nEps; dmg = nDmg; elSig = nSig; elRate = nRate; elMean = nMean;
+    vg = nVg;
+    contact = cc;
+    warm = false;
+    return true;
+  }
+
   // TryUpdate (Forging3DSolver.cs:1455-1537): contact and placement, then the placement repair (ImprovePlacement,
   // SmoothInterior) and the collapse check; the recovery ladder's lenient passes are not restated.
   int TryUpdate(double dt) {
@@ -1250,24 +1740,43 @@ struct Oracle {
   }
 
   // One increment (Forging3DSolver.cs:540-638) on a fixed mesh: no remeshing, no recovery ladder.
+  // Step (Forging3DSolver.cs:540-638). With remeshing on (remeshQuality > 0): the remesh trigger first, then the
+  // recovery ladder's first two passes (as is; after a remesh, with the first pass's corner allowance doubled); the
+  // refinement passes are not restated.
+  double remeshQuality = 0;
+  int remeshCount = 0;
   int Step(double& load, double& reaction, int& iterations) {
-    int rc = PrepareStep();
-    if (rc) return rc;
-    int converged = 0;
-    iterations = SolveVelocity(converged);
-    if (iterations < 0) return iterations;
-    if (!converged && lastRel >= 1e-2) { err = "Velocity solution did not converge"; return -7; }
-    double power = FinalEvaluation(reaction);
-    load = power / P.ram * P.sym_factor;
-    reaction *= P.sym_factor;
-    double dsx = std::min(std::min(ds, P.stroke_increment), LayerLimitedIncrement());
-    double dt = std::min(dsx, gap) / P.ram;
-    if (gap - P.ram * dt < 0.05 * dsx) dt = gap / P.ram;
+    if (remeshQuality > 0 && RemeshTrigger(remeshQuality) && RemeshCore()) remeshCount++;
+    const int passes = remeshQuality > 0 ? 2 : 1;
     bool done = false;
-    for (int attempt = 0; attempt <= 5 && !done; attempt++) {
-      double dta = dt / (1 << attempt);
-      if (TryUpdate(dta) == 0) { dt = dta; done = true; }
+    double dt = 0;
+    for (int pass = 0; pass < passes && !done; pass++) {
+      recoveryPass = pass;
+      int rc = PrepareStep();
+      if (rc) { recoveryPass = 0; return rc; }
+      int converged = 0;
+      iterations = SolveVelocity(converged);
+      if (iterations < 0) { recoveryPass = 0; return iterations; }
+      if (!converged && lastRel >= 1e-2) {
+        if (pass < passes - 1 && RemeshCore()) { remeshCount++; continue; }
+        err = "Velocity solution did not converge";
+        recoveryPass = 0;
+        return -7;
+      }
+      double power = FinalEvaluation(reaction);
+      load = power / P.ram * P.sym_factor;
+      reaction *= P.sym_factor;
+      double dsx = std::min(std::min(ds, P.stroke_increment), LayerLimitedIncrement());
+      dt = std::min(dsx, gap) / P.ram;
+      if (gap - P.ram * dt < 0.05 * dsx) dt = gap / P.ram;
+      for (int attempt = 0; attempt <= 5 && !done; attempt++) {
+        double dta = dt / (1 << attempt);
+        if (TryUpdate(dta) == 0) { dt = dta; done = true; }
+      }
+      if (done || pass == passes - 1) break;
+      if (RemeshCore()) remeshCount++;  // a rejected remesh still leaves the next pass
     }
+    recoveryPass = 0;
     if (!done) return -8;
     for (int e = 0; e < ne; e++) eps[e] += elRate[e] * dt;
     AccumulateDamage(dt);
@@ -1320,6 +1829,10 @@ void* orc_create(const OrcParams* prm, int nn, int ne, const int* tets, const do
   o->LocalLengths();
   o->BuildPattern();
   std::fill(o->elSig.begin(), o->elSig.end(), o->sigRef);
+  for (int n = 0; n < nn; n++) { o->symX = o->symX || o->onX0[n]; o->symY = o->symY || o->onY0[n]; }
+  for (int e = 0; e < ne; e++)
+    o->vol0 += Oracle::TetVol(o->x[tets[4 * e]], o->x[tets[4 * e + 1]], o->x[tets[4 * e + 2]], o->x[tets[4 * e + 3]]);
+  o->vol0 *= prm->sym_factor;
   return o;
 }
 
@@ -1464,6 +1977,116 @@ int orc_try_update(void* h, double dt, int* moves) {
   int rc = o->TryUpdate(dt);
   if (moves) { moves[0] = o->lastReverts; moves[1] = o->lastSmoothMoves; }
   return rc;
+}
+
+// RemeshTrigger (:1867-1880) on the state (its interior smoothing moves x): 0 none, 1 quality, 2 stretched surface.
+int orc_remesh_trigger(void* h, double remeshQuality) { return ((Oracle*)h)->RemeshTrigger(remeshQuality); }
+
+// RemeshCore (:1958-2102) at recovery pass `pass` (0 or 1): 1 when the new mesh was accepted (the state now lives on
+// it: orc_sizes, orc_get_mesh, orc_get_state), 0 when rejected. stats[6]: ImprovePlacement moves, reverts, smoothing
+// moves, untangling moves, worst quality before and after.
+int orc_remesh(void* h, int pass, double* stats) {
+  auto* o = (Oracle*)h;
+  o->recoveryPass = pass;
+  bool ok = o->RemeshCore();
+  o->recoveryPass = 0;
+  if (stats) {
+    stats[0] = o->rmImprove; stats[1] = o->rmReverts; stats[2] = o->rmSmooth; stats[3] = o->rmUntangle;
+    stats[4] = o->rmQOld; stats[5] = o->rmQNew;
+  }
+  return ok ? 1 : 0;
+}
+
+// UntangleSurface (:1698-1772) and CompensateVolume (:2108-2145) on positions x[3nn] and contact[nn] (in place), with
+// the stuffed positions stuffed[3nn] for UntangleSurface; the first returns its number of moves.
+int orc_untangle_surface(void* h, double* x, int* contact, const double* stuffed, double qTarget, int sweeps) {
+  auto* o = (Oracle*)h;
+  std::vector<V3> xx(o->nn), st(o->nn);
+  std::vector<int> cc(contact, contact + o->nn);
+  for (int n = 0; n < o->nn; n++) {
+    xx[n] = V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]);
+    st[n] = V3(stuffed[3 * n], stuffed[3 * n + 1], stuffed[3 * n + 2]);
+  }
+  int m = o->UntangleSurface(xx, cc, st, qTarget, sweeps);
+  for (int n = 0; n < o->nn; n++) { x[3 * n] = xx[n].x; x[3 * n + 1] = xx[n].y; x[3 * n + 2] = xx[n].z; contact[n] = cc[n]; }
+  return m;
+}
+void orc_compensate_volume(void* h, double* x, const int* contact) {
+  auto* o = (Oracle*)h;
+  std::vector<V3> xx(o->nn);
+  std::vector<int> cc(contact, contact + o->nn);
+  for (int n = 0; n < o->nn; n++) xx[n] = V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]);
+  o->CompensateVolume(xx, cc);
+  for (int n = 0; n < o->nn; n++) { x[3 * n] = xx[n].x; x[3 * n + 1] = xx[n].y; x[3 * n + 2] = xx[n].z; }
+}
+
+// Overwrite the accumulated Cockcroft-Latham damage dmg[ne].
+void orc_set_damage(void* h, const double* dmg) {
+  auto* o = (Oracle*)h;
+  std::copy(dmg, dmg + o->ne, o->dmg.begin());
+}
+
+// Remeshing inside orc_step: the trigger's quality threshold (RemeshQuality, 0.12 by default in the reference), 0 off.
+void orc_set_remesh(void* h, double remeshQuality) { ((Oracle*)h)->remeshQuality = remeshQuality; }
+int orc_remesh_count(void* h) { return ((Oracle*)h)->remeshCount; }
+
+// The current mesh: tets[4ne], onX0/onY0[nn].
+void orc_get_mesh(void* h, int* tets, unsigned char* onX0, unsigned char* onY0) {
+  auto* o = (Oracle*)h;
+  if (tets) std::copy(o->tets.begin(), o->tets.end(), tets);
+  for (int n = 0; n < o->nn; n++) {
+    if (onX0) onX0[n] = o->onX0[n];
+    if (onY0) onY0[n] = o->onY0[n];
+  }
+}
+
+// Initial volume (Forging3DSolver.cs:184) and the symmetry planes of the model (set at creation from onX0 / onY0).
+void orc_set_remesh_info(void* h, double vol0, int symX, int symY) {
+  auto* o = (Oracle*)h;
+  o->vol0 = vol0; o->symX = symX != 0; o->symY = symY != 0;
+}
+
+// TetLocator (TetMesh.cs:115-262) over (tets, x) with cell size `cell`: per query point Find (tet, weights), FindNearest
+// and Interpolate of field[3nn] (any output may be NULL).
+void orc_locate(int nn, const double* x, int ne, const int* tets, double cell, int np, const double* p, int* found,
+                double* w, int* nearest, const double* field, double* interp) {
+  std::vector<V3> xx(nn);
+  for (int n = 0; n < nn; n++) xx[n] = V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]);
+  std::vector<int> tt(tets, tets + 4 * (size_t)ne);
+  OLocator loc;
+  loc.Build(tt, xx, cell);
+  std::vector<V3> f;
+  if (field) { f.resize(nn); for (int n = 0; n < nn; n++) f[n] = V3(field[3 * n], field[3 * n + 1], field[3 * n + 2]); }
+  for (int i = 0; i < np; i++) {
+    V3 q(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    if (found) found[i] = loc.Find(q, w ? w + 4 * i : nullptr);
+    if (nearest) nearest[i] = loc.FindNearest(q);
+    if (field && interp) { V3 v = loc.Interpolate(q, f); interp[3 * i] = v.x; interp[3 * i + 1] = v.y; interp[3 * i + 2] = v.z; }
+  }
+}
+
+// IsosurfaceStuffing.Stuff (IsosurfaceStuffing.cs:17-88) of the interior of the mesh (tets, x) (inside test: its
+// TetLocator with cell `cell`) over [bmin, bmax] at spacing h. Call with outX = NULL to get the sizes, then again with
+// room for nn_out points and ne_out tets.
+void orc_stuff(int nn, const double* x, int ne, const int* tets, double cell, const double* bmin, const double* bmax,
+               double h, int symX, int symY, int* nn_out, int* ne_out, double* outX, int* outT) {
+  std::vector<V3> xx(nn);
+  for (int n = 0; n < nn; n++) xx[n] = V3(x[3 * n], x[3 * n + 1], x[3 * n + 2]);
+  std::vector<int> tt(tets, tets + 4 * (size_t)ne);
+  OLocator loc;
+  loc.Build(tt, xx, cell);
+  const double e1 = 1e-7 * h;
+  auto In = [&](const V3& q) {
+    if ((symX && q.x < 0) || (symY && q.y < 0)) return false;
+    return loc.Contains(V3(symX ? std::max(q.x, e1) : q.x, symY ? std::max(q.y, e1) : q.y, q.z));
+  };
+  std::vector<V3> px;
+  std::vector<int> pt;
+  OStuff(In, V3(bmin[0], bmin[1], bmin[2]), V3(bmax[0], bmax[1], bmax[2]), h, symX != 0, symY != 0, px, pt);
+  *nn_out = (int)px.size();
+  *ne_out = (int)(pt.size() / 4);
+  if (outX) for (size_t n = 0; n < px.size(); n++) { outX[3 * n] = px[n].x; outX[3 * n + 1] = px[n].y; outX[3 * n + 2] = px[n].z; }
+  if (outT) std::copy(pt.begin(), pt.end(), outT);
 }
 
 // SmoothInterior (:1776-1838) on the node positions x[3nn] (in place); returns the number of moves.

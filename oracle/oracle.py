@@ -68,6 +68,25 @@ def lib():
         L.orc_max3.argtypes = [C.c_int, dp, dp]
         L.orc_try_update.argtypes = [vp, C.c_double, ip]
         L.orc_smooth_interior.argtypes = [vp, dp, C.c_double, C.c_int]
+        L.orc_remesh_trigger.argtypes = [vp, C.c_double]
+        L.orc_remesh.argtypes = [vp, C.c_int, dp]
+        L.orc_untangle_surface.argtypes = [vp, dp, ip, dp, C.c_double, C.c_int]
+        L.orc_compensate_volume.argtypes = [vp, dp, ip]
+        L.orc_compensate_volume.restype = None
+        L.orc_set_damage.argtypes = [vp, dp]
+        L.orc_set_damage.restype = None
+        L.orc_set_remesh.argtypes = [vp, C.c_double]
+        L.orc_set_remesh.restype = None
+        L.orc_remesh_count.argtypes = [vp]
+        L.orc_get_mesh.argtypes = [vp, ip, u8p, u8p]
+        L.orc_get_mesh.restype = None
+        L.orc_set_remesh_info.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+        L.orc_set_remesh_info.restype = None
+        L.orc_locate.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, C.c_int, dp, ip, dp, ip, dp, dp]
+        L.orc_locate.restype = None
+        L.orc_stuff.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, dp, dp, C.c_double, C.c_int, C.c_int, ip, ip,
+                                dp, ip]
+        L.orc_stuff.restype = None
         L.orc_solve_velocity.argtypes = [vp, ip, dp, ip]
         L.orc_final.restype = C.c_double
         L.orc_final.argtypes = [vp, dp]
@@ -252,6 +271,58 @@ class Oracle:
         m = lib().orc_smooth_interior(self.h, _p(xs, C.c_double), float(q_target), int(sweeps))
         return xs, m
 
+    def remesh_trigger(self, remesh_quality=0.12):
+        """RemeshTrigger (Forging3DSolver.cs:1867-1880): 0 none, 1 quality, 2 stretched surface (smoothing moves x)."""
+        return lib().orc_remesh_trigger(self.h, float(remesh_quality))
+
+    def remesh(self, recovery_pass=0):
+        """RemeshCore (:1958-2102): True when the new mesh was accepted (the state now lives on it), and the stats
+        (ImprovePlacement moves, reverts, smoothing moves, untangling moves, q before, q after)."""
+        st = np.zeros(6)
+        ok = lib().orc_remesh(self.h, int(recovery_pass), _p(st, C.c_double))
+        if ok:
+            s = (C.c_longlong * 6)()
+            lib().orc_sizes(self.h, s)
+            self.nn, self.ne, self.ndof, self.nnz, self.nbf = s[0], s[1], s[2], s[3], s[4]
+        return bool(ok), dict(improve=int(st[0]), reverts=int(st[1]), smooth=int(st[2]), untangle=int(st[3]),
+                              q_old=st[4], q_new=st[5])
+
+    def untangle_surface(self, x, contact, stuffed, q_target=0.03, sweeps=6):
+        xs, cc, st = F64(x).copy(), I32(contact).copy(), F64(stuffed)
+        m = lib().orc_untangle_surface(self.h, _p(xs, C.c_double), _p(cc, C.c_int), _p(st, C.c_double),
+                                       float(q_target), int(sweeps))
+        return xs, cc, m
+
+    def compensate_volume(self, x, contact):
+        xs, cc = F64(x).copy(), I32(contact)
+        lib().orc_compensate_volume(self.h, _p(xs, C.c_double), _p(cc, C.c_int))
+        return xs
+
+    def set_damage(self, dmg):
+        d = F64(dmg)
+        lib().orc_set_damage(self.h, _p(d, C.c_double))
+
+    def set_remesh(self, remesh_quality=0.12):
+        """Remeshing inside step() (0 turns it off)."""
+        lib().orc_set_remesh(self.h, float(remesh_quality))
+
+    def remesh_count(self):
+        return lib().orc_remesh_count(self.h)
+
+    def refresh_sizes(self):
+        s = (C.c_longlong * 6)()
+        lib().orc_sizes(self.h, s)
+        self.nn, self.ne, self.ndof, self.nnz, self.nbf = s[0], s[1], s[2], s[3], s[4]
+
+    def mesh(self):
+        t = np.zeros((self.ne, 4), np.int32)
+        ox, oy = np.zeros(self.nn, np.uint8), np.zeros(self.nn, np.uint8)
+        lib().orc_get_mesh(self.h, _p(t, C.c_int), _p(ox, C.c_ubyte), _p(oy, C.c_ubyte))
+        return t, ox.astype(bool), oy.astype(bool)
+
+    def set_remesh_info(self, vol0, symX, symY):
+        lib().orc_set_remesh_info(self.h, float(vol0), int(symX), int(symY))
+
     def accumulate_damage(self, dt):
         lib().orc_accumulate_damage(self.h, float(dt))
 
@@ -271,6 +342,7 @@ class Oracle:
     def step(self):
         out = np.zeros(8)
         rc = lib().orc_step(self.h, _p(out, C.c_double))
+        self.refresh_sizes()   # a remesh inside the step changes the mesh
         if rc:
             raise RuntimeError(f"oracle Step failed ({rc}): {self.error()}")
         return dict(load=out[0], reaction=out[1], iterations=int(out[2]), dt=out[3], gap=out[4],
@@ -285,6 +357,32 @@ class Oracle:
 
     def factor_nnz(self):
         return lib().orc_factor_nnz(self.h)
+
+
+def locate(x, tets, cell, p, field=None):
+    """TetLocator (TetMesh.cs:115-262): (Find tets, weights, FindNearest tets, Interpolate of field or None)."""
+    x, t, p = F64(x), I32(tets), F64(np.atleast_2d(p))
+    npt = p.shape[0]
+    found, near = np.zeros(npt, np.int32), np.zeros(npt, np.int32)
+    w = np.zeros((npt, 4))
+    f = F64(field) if field is not None else None
+    out = np.zeros((npt, 3)) if field is not None else None
+    lib().orc_locate(x.shape[0], _p(x, C.c_double), t.shape[0], _p(t, C.c_int), float(cell), npt, _p(p, C.c_double),
+                     _p(found, C.c_int), _p(w, C.c_double), _p(near, C.c_int), _p(f, C.c_double), _p(out, C.c_double))
+    return found, w, near, out
+
+
+def stuff(x, tets, cell, bmin, bmax, h, symX, symY):
+    """IsosurfaceStuffing.Stuff (IsosurfaceStuffing.cs:17-88) of the mesh interior: (points, tets)."""
+    x, t = F64(x), I32(tets)
+    lo, hi = F64(np.asarray(bmin, float)), F64(np.asarray(bmax, float))
+    nn, ne = C.c_int(0), C.c_int(0)
+    args = (x.shape[0], _p(x, C.c_double), t.shape[0], _p(t, C.c_int), float(cell), _p(lo, C.c_double),
+            _p(hi, C.c_double), float(h), int(symX), int(symY), C.byref(nn), C.byref(ne))
+    lib().orc_stuff(*args, None, None)
+    px, pt = np.zeros((nn.value, 3)), np.zeros((ne.value, 4), np.int32)
+    lib().orc_stuff(*args, _p(px, C.c_double), _p(pt, C.c_int))
+    return px, pt
 
 
 class OracleCholesky:

@@ -24,7 +24,7 @@ def lib():
 
 def _header_functions():
     names = set()
-    for h in ("ifea_spd.h", "ifea_step.h", "ifea_mlp.h"):
+    for h in ("ifea_spd.h", "ifea_step.h", "ifea_mlp.h", "ifea_remesh.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         names |= set(re.findall(r"\b(ifea_\w+)\s*\(", txt))
@@ -36,7 +36,7 @@ def test_exports_every_declared_symbol(lib):
     assert len(decl) >= 25
     missing = [n for n in decl if not hasattr(lib, n)]
     assert not missing, missing
-    assert set(decl) == set(N.SPD_SYMBOLS) | set(N.STEP_SYMBOLS) | set(N.MLP_SYMBOLS)
+    assert set(decl) == set(N.SPD_SYMBOLS) | set(N.STEP_SYMBOLS) | set(N.MLP_SYMBOLS) | set(N.REMESH_SYMBOLS)
 
 
 def _sym(lib, n, cp, ri, perm=None, coords=None):
