@@ -32,7 +32,7 @@ STEP_SYMBOLS = ["ifea_step_create", "ifea_step_set_params", "ifea_step_set_optio
 
 REMESH_SYMBOLS = ["ifea_locator_create", "ifea_locator_find", "ifea_locator_find_nearest", "ifea_locator_interpolate",
                   "ifea_locator_destroy", "ifea_stuff", "ifea_stuffing_sizes", "ifea_stuffing_get",
-                  "ifea_stuffing_destroy", "ifea_remesh_last_error"]
+                  "ifea_stuffing_destroy", "ifea_smooth_interior", "ifea_remesh_last_error"]
 
 IFEA_OK = 0
 IFEA_ERR_ARG = -1
@@ -139,6 +139,7 @@ def lib():
     L.ifea_stuffing_get.argtypes = [vp, dp, ip]
     L.ifea_stuffing_destroy.argtypes = [vp]
     L.ifea_stuffing_destroy.restype = None
+    L.ifea_smooth_interior.argtypes = [C.c_int, C.c_int, ip, ip, ip, u8p, u8p, u8p, dp, C.c_double, C.c_int, ip]
     L.ifea_remesh_last_error.restype = C.c_char_p
     _lib = L
     return L

@@ -1,4 +1,5 @@
-// Host geometry of the remesh step: point location in a tet mesh and the isosurface-stuffing mesher.
+// Host geometry of the update and remesh steps: interior smoothing, point location in a tet mesh and the
+// isosurface-stuffing mesher.
 //
 // Replaces TetLocator (TetMesh.cs:115-262) and IsosurfaceStuffing.Stuff / StuffLattice (IsosurfaceStuffing.cs:17-88,
 // :115-294),
@@ -10,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -396,6 +398,86 @@ class Stuffer {
   std::vector<int> out_;
 };
 
+// TetMesh.Quality (TetMesh.cs:107-112): 6 sqrt(2) V / l2^1.5, l2 the mean squared edge length; -1 when degenerate
+inline double tet_quality(const P3& a, const P3& b, const P3& c, const P3& d) {
+  const double l2 = (len2(sub(b, a)) + len2(sub(c, a)) + len2(sub(d, a)) + len2(sub(c, b)) + len2(sub(d, b)) +
+                     len2(sub(d, c))) / 6.0;
+  return l2 > 0 ? 6 * std::sqrt(2.0) * tet_volume(a, b, c, d) / std::pow(l2, 1.5) : -1;
+}
+
+// SmoothInterior (Forging3DSolver.cs:1776-1838) with the tets' qualities kept current: a sweep marks the nodes of the
+// tets below qTarget, then each marked interior node (ascending) tries its neighbours' centroid at full, half and
+// quarter steps and keeps the first that raises the worst quality of its tets by more than 1e-6; only the moved node's
+// tets are re-evaluated, so every decision sees the qualities a full re-evaluation would give.
+int smooth_interior(int nn, int ne, const int* t, const int* nts, const int* ntl, const unsigned char* surf,
+                    const unsigned char* ox, const unsigned char* oy, double* xp, double qTarget, int sweeps) {
+  auto X = [&](int n) { return P3{xp[3 * (size_t)n], xp[3 * (size_t)n + 1], xp[3 * (size_t)n + 2]}; };
+  auto Q = [&](int e) { return tet_quality(X(t[4 * (size_t)e]), X(t[4 * (size_t)e + 1]), X(t[4 * (size_t)e + 2]),
+                                           X(t[4 * (size_t)e + 3])); };
+  std::vector<double> q(ne);
+  {
+    // every tet's quality is independent: evaluate them on a few host threads
+    const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int w = 0; w < nt; w++)
+      th.emplace_back([&, w] {
+        for (int e = (int)((long long)ne * w / nt); e < (int)((long long)ne * (w + 1) / nt); e++) q[e] = Q(e);
+      });
+    for (auto& x : th) x.join();
+  }
+  std::vector<char> bad(nn);
+  int moves = 0;
+  for (int s = 0; s < sweeps; s++) {
+    std::fill(bad.begin(), bad.end(), 0);
+    bool any = false;
+    for (int e = 0; e < ne; e++)
+      if (q[e] < qTarget) {
+        for (int a = 0; a < 4; a++) bad[t[4 * (size_t)e + a]] = 1;
+        any = true;
+      }
+    if (!any) break;
+    int sweep_moves = 0;
+    for (int n = 0; n < nn; n++) {
+      if (!bad[n] || surf[n]) continue;
+      P3 sum{0, 0, 0};
+      int cnt = 0;
+      double worst = INFINITY;
+      for (int k = nts[n]; k < nts[n + 1]; k++) {
+        const int e = ntl[k];
+        for (int a = 0; a < 4; a++) {
+          const int m = t[4 * (size_t)e + a];
+          if (m == n) continue;
+          sum = add(sum, X(m));
+          cnt++;
+        }
+        worst = std::min(worst, q[e]);
+      }
+      if (cnt == 0) continue;
+      const P3 old = X(n);
+      const P3 c = scale(1.0 / cnt, sum);
+      for (double lam : {1.0, 0.5, 0.25}) {
+        const P3 p = add(old, scale(lam, sub(c, old)));
+        xp[3 * (size_t)n] = ox[n] ? 0 : p.x;
+        xp[3 * (size_t)n + 1] = oy[n] ? 0 : p.y;
+        xp[3 * (size_t)n + 2] = p.z;
+        double qn = INFINITY;
+        for (int k = nts[n]; k < nts[n + 1]; k++) qn = std::min(qn, Q(ntl[k]));
+        if (qn > worst + 1e-6) {
+          sweep_moves++;
+          for (int k = nts[n]; k < nts[n + 1]; k++) q[ntl[k]] = Q(ntl[k]);
+          break;
+        }
+        xp[3 * (size_t)n] = old.x;
+        xp[3 * (size_t)n + 1] = old.y;
+        xp[3 * (size_t)n + 2] = old.z;
+      }
+    }
+    moves += sweep_moves;
+    if (sweep_moves == 0) break;
+  }
+  return moves;
+}
+
 thread_local char g_err[256] = "";
 void set_err(const char* m) { std::snprintf(g_err, sizeof g_err, "%s", m); }
 
@@ -476,6 +558,24 @@ int ifea_stuffing_get(ifea_stuffing_t s, double* x, int* tets) {
 }
 
 void ifea_stuffing_destroy(ifea_stuffing_t s) { delete s; }
+
+int ifea_smooth_interior(int nn, int ne, const int* tets, const int* node_tet_start, const int* node_tet_list,
+                         const unsigned char* is_surface, const unsigned char* on_x0, const unsigned char* on_y0,
+                         double* x, double q_target, int sweeps, int* moves) {
+  if (nn <= 0 || ne < 0 || !tets || !node_tet_start || !node_tet_list || !is_surface || !on_x0 || !on_y0 || !x ||
+      !moves || sweeps < 0) {
+    set_err("invalid smoothing arguments");
+    return -1;
+  }
+  try {
+    *moves = smooth_interior(nn, ne, tets, node_tet_start, node_tet_list, is_surface, on_x0, on_y0, x, q_target,
+                             sweeps);
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return -6;
+  }
+}
 
 const char* ifea_remesh_last_error(void) { return g_err; }
 

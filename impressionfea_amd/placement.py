@@ -117,6 +117,40 @@ def push_out(die, off, p, outward, h):
     return p + hi * d
 
 
+def push_out_many(die, off, p, outward, h):
+    """push_out for many points at once: the same steps and bisections per point, in lockstep."""
+    p = np.atleast_2d(p)
+    out = np.array(die.project(p, off)[0], dtype=np.float64)
+    nz = dot3(outward, outward) != 0
+    act = nz & (die.clearance(p, off) < 0)
+    if not act.any():
+        return out
+    idx = np.nonzero(act)[0]
+    pa, d = p[idx], -outward[idx]
+    step = 0.1 * h
+    lo = np.zeros(idx.size)
+    hi = np.full(idx.size, -1.0)
+    found = np.zeros(idx.size, bool)
+    t = step
+    while t <= 2 * h + 1e-12:
+        open_ = ~found
+        if not open_.any():
+            break
+        hit = open_ & (die.clearance(pa + t * d, off) >= 0)
+        hi[hit] = t
+        found |= hit
+        lo[open_ & ~hit] = t
+        t += step
+    for _ in range(25):
+        mid = 0.5 * (lo + hi)
+        ok = die.clearance(pa + mid[:, None] * d, off) >= 0
+        hi = np.where(found & ok, mid, hi)
+        lo = np.where(found & ~ok, mid, lo)
+    res = pa + hi[:, None] * d
+    out[idx[found]] = res[found]
+    return out
+
+
 def die_of(cfg, side, gap):
     return (cfg.top, gap) if side == TOP else (cfg.bottom, 0.0)
 
@@ -128,10 +162,34 @@ def improve_placement(mesh, x, contact, pred, nrm, gap, q_min, h_local, cfg, bad
     judged at the current positions (a heap holds the bad tets and the later tets of every moved node)."""
     t = mesh.tets
     corner = CORNER_LADDER[min(recovery_pass, len(CORNER_LADDER) - 1)]
+    # a node's candidates depend only on its predicted position, normal and die: computed once per die side, for the
+    # contact nodes of the tets below q_min in one go, for others when first needed
+    cache = {}
+
+    def candidates(n):
+        key = (n, int(contact[n]))
+        if key not in cache:
+            die, off = die_of(cfg, contact[n], gap)
+            cache[key] = (die.project(pred[n], off)[0], push_out(die, off, pred[n], nrm[n], cfg.h),
+                          -float(die.clearance(pred[n], off)))
+        return cache[key]
+
+    first = bad_tets(x, q_min)
+    if first.size:
+        nodes = np.unique(t[first].ravel())
+        for side in (BOTTOM, TOP):
+            sel = nodes[np.asarray(contact)[nodes] == side]
+            if sel.size:
+                die, off = die_of(cfg, side, gap)
+                cl = die.project(pred[sel], off)[0]
+                pu = push_out_many(die, off, pred[sel], nrm[sel], cfg.h)
+                pe = -die.clearance(pred[sel], off)
+                for i, n in enumerate(sel):
+                    cache[(int(n), side)] = (cl[i], pu[i], float(pe[i]))
     total = 0
-    for _ in range(4):
+    for sweep in range(4):
         count = 0
-        heap = [int(e) for e in bad_tets(x, q_min)]
+        heap = [int(e) for e in (first if sweep == 0 else bad_tets(x, q_min))]
         heapq.heapify(heap)
         queued = set(heap)
         while heap:
@@ -142,13 +200,12 @@ def improve_placement(mesh, x, contact, pred, nrm, gap, q_min, h_local, cfg, bad
                 n = int(t[e, a])
                 if contact[n] == FREE:
                     continue
-                die, off = die_of(cfg, contact[n], gap)
-                closest = sym1(mesh, die.project(pred[n], off)[0], n)
+                closest, push, pen = candidates(n)
+                closest = sym1(mesh, closest, n)
                 cand = [closest]
-                push = sym1(mesh, push_out(die, off, pred[n], nrm[n], cfg.h), n)
+                push = sym1(mesh, push, n)
                 if np.sqrt(dot3(push - pred[n], push - pred[n])) <= 0.5 * h_local[n]:
                     cand.append(push)
-                pen = -float(die.clearance(pred[n], off))
                 for f in (0.0, 0.25, 0.5, 0.75):
                     if pen * (1 - f) <= corner * h_local[n]:
                         cand.append(pred[n] + f * (closest - pred[n]))
@@ -189,36 +246,27 @@ def _neighbour_centroid(mesh, x, n):
     return tt, (1.0 / others.size) * s
 
 
-def smooth_interior(mesh, x, q_target, sweeps, bad_tets):
+def smooth_interior(mesh, x, q_target, sweeps, bad_tets=None):
     """SmoothInterior (:1776-1838): each interior node of a tet below q_target moves towards its neighbours' centroid
     (full, half or quarter step) when that raises the worst quality of its tets; nodes in ascending order at the
-    current positions; symmetry-plane nodes stay on their planes."""
-    t = mesh.tets
-    moves = 0
-    for _ in range(sweeps):
-        bad_t = bad_tets(x, q_target)
-        if bad_t.size == 0:
-            break
-        nodes = np.unique(t[bad_t].ravel())
-        nodes = nodes[~mesh.is_surface_node[nodes]]
-        sweep_moves = 0
-        for n in nodes:
-            n = int(n)
-            tt, c = _neighbour_centroid(mesh, x, n)
-            if c is None:
-                continue
-            worst = float(tet_quality(x, tt).min())
-            old = x[n].copy()
-            for lam in (1.0, 0.5, 0.25):
-                x[n] = sym1(mesh, old + lam * (c - old), n)
-                if float(tet_quality(x, tt).min()) > worst + 1e-6:
-                    sweep_moves += 1
-                    break
-                x[n] = old
-        moves += sweep_moves
-        if sweep_moves == 0:
-            break
-    return moves
+    current positions; symmetry-plane nodes stay on their planes. The library's host routine (ifea_smooth_interior)
+    does the sweeps; bad_tets is not needed (it keeps every tet's quality current itself)."""
+    import ctypes as C
+    from . import _native as N
+    if not x.flags["C_CONTIGUOUS"] or x.dtype != np.float64:
+        raise ValueError("x must be a C-contiguous float64 array (it is updated in place)")
+    if not hasattr(mesh, "_smooth_args"):
+        mesh._smooth_args = (N.i32(mesh.tets), N.i32(mesh.node_tet_start), N.i32(mesh.node_tet_list),
+                             np.ascontiguousarray(mesh.is_surface_node, np.uint8),
+                             np.ascontiguousarray(mesh.onX0, np.uint8), np.ascontiguousarray(mesh.onY0, np.uint8))
+    t, st, lst, su, ox, oy = mesh._smooth_args
+    moves = C.c_int(0)
+    rc = N.lib().ifea_smooth_interior(mesh.nn, mesh.ne, N.ptr(t, C.c_int), N.ptr(st, C.c_int), N.ptr(lst, C.c_int),
+                                      N.ptr(su, C.c_ubyte), N.ptr(ox, C.c_ubyte), N.ptr(oy, C.c_ubyte),
+                                      N.ptr(x, C.c_double), float(q_target), int(sweeps), C.byref(moves))
+    if rc:
+        raise RuntimeError(f"ifea_smooth_interior failed ({rc}): {N.lib().ifea_remesh_last_error().decode()}")
+    return moves.value
 
 
 def untangle_surface(mesh, x, contact, stuffed, q_target, sweeps, cfg, gap, bad_tets):

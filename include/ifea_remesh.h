@@ -1,10 +1,10 @@
-/* ifea_remesh.h — host geometry of the remesh step of the 3D forging solver (no GPU work).
+/* ifea_remesh.h — host geometry of the update and remesh steps of the 3D forging solver (no GPU work).
  *
  * The reference remeshes inside Forging3DSolver.RemeshCore (src/ImpressionFEA.Core/ThreeD/Forging3DSolver.cs:1958-2102):
  * a point locator over the old mesh gives the inside test of the workpiece, isosurface stuffing meshes that domain
  * anew, and the same locator remaps element history and nodal velocities onto the new mesh. These entry points are
- * those two tools; the per-node placement logic of RemeshCore (contact, volume compensation, placement repair,
- * smoothing) stays with the caller, as it does for TryUpdate.
+ * those two tools and the interior smoothing that TryUpdate and RemeshCore both run; the rest of the per-node
+ * placement logic (contact, volume compensation, placement repair) stays with the caller.
  *
  * Arrays are caller-owned and read during the call only: node positions x[3 nn] (x, y, z per node), tets[4 ne]
  * (positively oriented node ids), query points p[3 np]. Return 0 on success, -1 for invalid arguments, -6 when the
@@ -47,6 +47,15 @@ int ifea_stuff(ifea_locator_t inside, const double* bmin, const double* bmax, do
 int ifea_stuffing_sizes(ifea_stuffing_t s, int* nn, int* ne);
 int ifea_stuffing_get(ifea_stuffing_t s, double* x, int* tets);
 void ifea_stuffing_destroy(ifea_stuffing_t s);
+
+/* SmoothInterior (Forging3DSolver.cs:1776-1838) on x[3 nn] in place, for TryUpdate and the remesh: each interior node
+ * (is_surface[n] == 0) of a tet below q_target moves towards its neighbours' centroid at a full, half or quarter step
+ * when that raises the worst quality of its tets; nodes ascending, up to `sweeps` sweeps, symmetry-plane nodes stay on
+ * their planes. node_tet_start[nn+1] / node_tet_list: each node's tets, ascending (TetMesh.cs:18-26). *moves: the
+ * number of node moves. */
+int ifea_smooth_interior(int nn, int ne, const int* tets, const int* node_tet_start, const int* node_tet_list,
+                         const unsigned char* is_surface, const unsigned char* on_x0, const unsigned char* on_y0,
+                         double* x, double q_target, int sweeps, int* moves);
 
 const char* ifea_remesh_last_error(void);
 
