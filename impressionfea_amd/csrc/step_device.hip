@@ -25,6 +25,7 @@ constexpr int NODE_WAVES = 4;   // nodes per 256-thread block
 // mesh, not to MAX_DEG, so that more workgroups share a CU)
 inline size_t node_lds_bytes(int md) { return (size_t)NODE_WAVES * md * (9 * sizeof(double) + sizeof(int)); }
 constexpr int TAN_WAVES = 2;    // nodes per workgroup of the tangent assembly
+constexpr int NFIX_BATCH = 4;   // column entries per lane (of 64) whose rows' fixed-DOF counts the assembly gathers at once
 // dynamic LDS of the tangent assembly: per wave 18 md + 26 me doubles and md + 9 me ints (md: largest neighbour list,
 // me: most elements around a node)
 inline size_t node_tan_lds_bytes(int md, int me) {
@@ -505,18 +506,65 @@ __global__ __launch_bounds__(64 * TAN_WAVES) void k_node(int md, int me, int nn,
   int* nbr = (int*)(node_lds + (long long)TAN_WAVES * (18 * md + 26 * me)) + wv * (md + 9 * me);
   int* et = nbr + md;                                             // 4 me: neighbour positions of the element's nodes
   int* eb = et + 4 * me;                                          // me: local index of j in the element
-  for (int q = lane; q < deg; q += 64) nbr[q] = nb[nbs[j] + q];
-  for (int c = 0; c < 3; c++) {
-    const double* src = kvol + colPtr[3 * j + c];
-    for (int q = lane; q < d3; q += 64) acc[c * d3 + q] = src[q];
-  }
-  for (int q = lane; q < 9 * deg; q += 64) axn[q] = ax[9 * (long long)nb[nbs[j] + q / 9] + q % 9];
-  // r_vol = K_vol v (row j = column j, symmetric); energy 1/2 v.r_vol
+  // The staging loops below issue every load of a batch before using any: a node's work is a few short chains of
+  // dependent loads (index, then gather), and one wave per node leaves the latency exposed unless the chains overlap.
+  const int* nbj = nb + nbs[j];
+  for (int q = lane; q < deg; q += 64) nbr[q] = nbj[q];
+  // the node's three columns of K_vol into acc, and r_vol = K_vol v (row j = column j, symmetric), each lane summing
+  // its entries in ascending order
   double rj[3];
-  for (int c = 0; c < 3; c++) {
-    double s = 0.0;
-    for (int p = colPtr[3 * j + c] + lane; p < colPtr[3 * j + c + 1]; p += 64) s += kvol[p] * v[rowIdx[p]];
-    rj[c] = wsum(s);
+  {
+    constexpr int KB = 4;  // entries of a column per lane per batch
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int base = 0; base < d3; base += 64 * KB) {
+      double kv[3][KB], vv[3][KB];
+      int ri[3][KB];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+          const int q = base + lane + 64 * k;
+          const int p = colPtr[3 * j + c] + min(q, d3 - 1);
+          kv[c][k] = kvol[p];
+          ri[c][k] = rowIdx[p];
+        }
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < KB; k++) vv[c][k] = v[ri[c][k]];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+          const int q = base + lane + 64 * k;
+          if (q < d3) {
+            acc[c * d3 + q] = kv[c][k];
+            s[c] += kv[c][k] * vv[c][k];
+          }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) rj[c] = wsum(s[c]);
+  }
+  // the frames of the neighbours
+  {
+    constexpr int KB = 8;
+    for (int base = 0; base < 9 * deg; base += 64 * KB) {
+      int nbk[KB];
+      double av[KB];
+#pragma unroll
+      for (int k = 0; k < KB; k++) nbk[k] = nbj[min(base + lane + 64 * k, 9 * deg - 1) / 9];
+#pragma unroll
+      for (int k = 0; k < KB; k++) {
+        const int q = min(base + lane + 64 * k, 9 * deg - 1);
+        av[k] = ax[9 * (long long)nbk[k] + q % 9];
+      }
+#pragma unroll
+      for (int k = 0; k < KB; k++) {
+        const int q = base + lane + 64 * k;
+        if (q < 9 * deg) axn[q] = av[k];
+      }
+    }
   }
   double en = 0.5 * v[3 * j] * rj[0] + 0.5 * v[3 * j + 1] * rj[1] + 0.5 * v[3 * j + 2] * rj[2];
   double pw = 2.0 * en;
@@ -654,18 +702,32 @@ __global__ __launch_bounds__(64 * TAN_WAVES) void k_node(int md, int me, int nn,
     wave_sync_lds();
     double dmax = 0.0;
     if (mode == 1) {
-      // identity rows/columns for fixed DOFs, largest free diagonal
+      // identity rows/columns for fixed DOFs, largest free diagonal. The fixed-DOF counts of the rows a lane holds
+      // (entries lane + 64 k of every column) are gathered at once; nodes with more column entries than the batch
+      // covers take the plain loop
+      const bool batched = d3 <= 64 * NFIX_BATCH;
+      int nfix_l[NFIX_BATCH];
+#pragma unroll
+      for (int k = 0; k < NFIX_BATCH; k++) {
+        const int q = lane + 64 * k;
+        nfix_l[k] = (batched && q < d3) ? nfix[nbr[q / 3]] : 0;
+      }
+      auto fix_entry = [&](int c, int q, int nfi) {
+        const int pos = q / 3, rr = q % 3;
+        const int i = nbr[pos];
+        const bool diag = (i == j) && (rr == c);
+        double vv = acc[c * d3 + q];
+        if (c < nf || rr < nfi) vv = diag ? 1.0 : 0.0;
+        else if (diag) dmax = fmax(dmax, vv);
+        acc[c * d3 + q] = vv;
+      };
       for (int c = 0; c < 3; c++) {
-        const bool fcol = c < nf;
-        for (int q = lane; q < d3; q += 64) {
-          const int pos = q / 3, rr = q % 3;
-          const int i = nbr[pos];
-          const bool frow = rr < nfix[i];
-          const bool diag = (i == j) && (rr == c);
-          double vv = acc[c * d3 + q];
-          if (fcol || frow) vv = diag ? 1.0 : 0.0;
-          else if (diag) dmax = fmax(dmax, vv);
-          acc[c * d3 + q] = vv;
+        if (batched) {
+#pragma unroll
+          for (int k = 0; k < NFIX_BATCH; k++)
+            if (lane + 64 * k < d3) fix_entry(c, lane + 64 * k, nfix_l[k]);
+        } else {
+          for (int q = lane; q < d3; q += 64) fix_entry(c, q, nfix[nbr[q / 3]]);
         }
       }
       for (int off = 32; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_down(dmax, off, 64));
