@@ -448,22 +448,37 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 
 // Outer and mid trailing updates with 128x128 output tiles: NT/64 waves, two along the rows (64 each) and
 // NT/128 along the columns, each wave a block of 16x16 MFMA tiles (64x32 with 8 waves). X is streamed in 16-column
-// chunks through a double-buffered LDS tile; the global loads of the next chunk are in flight while the current
-// chunk feeds the MFMAs, one barrier per chunk. Register use is kept at or below 128 VGPRs (one chunk of loads in
-// registers, 32-bit offsets from a wave-uniform column base) so that two 8-wave workgroups share a CU and one
-// workgroup's MFMAs run while the other waits at its barrier; the tile is subtracted by atomic adds in L2 (one update
-// per entry per launch, so the result does not depend on their order). task: (supernode s, row tile I, column tile
-// J, 0), I >= J. Fronts are below 2^27 rows (checked at analysis), so chunk offsets fit in 32 bits.
+// chunks through two LDS slots, each chunk's 256 rows (the tile's 128 rows, then its 128 columns) in k-major order.
+// The loads run two chunks ahead in registers: those of chunk c + 2 are issued before the MFMAs of chunk c and
+// written to LDS after the MFMAs of chunk c + 1, one barrier per chunk. They are buffer loads: the front's block
+// column kb is the descriptor base, a thread's row is its one 32-bit VGPR offset and the chunk's column a wave-uniform
+// SGPR offset, so no load needs a 64-bit address; a load that must read zero (a row beyond the front, a column
+// beyond the block) takes an offset past the descriptor's range, for which the hardware returns 0. Register use stays
+// at or below 128 VGPRs so that two 8-wave workgroups share a CU and one workgroup's MFMAs run while the other waits
+// at its barrier; the tile is subtracted by atomic adds in L2 (one update per entry per launch, so the result does
+// not depend on their order). task: (supernode s, row tile I, column tile J, 0), I >= J. Fronts are below 2^19 rows
+// (checked at analysis), so the byte offsets within a block of at most OB2 columns fit in 31 bits.
 // MID = false (outer): X = the outer block [kb, kb + min(OB2, k - kb)), C = every column to its right (L and U).
 // MID = true: X = the first OB columns [kb, kb + OB) of an outer block, C = its second half [kb + OB, min(k, kb +
 // OB2)), all rows below; the panels of the second half are then factored against a complete update.
+template <typename T>
+__device__ __forceinline__ T buffer_ld(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+
 template <typename T, int NT, int KC, bool MID>
-__global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 128))) void k_syrk_big(const int4* __restrict__ tasks, int kb, SymDev sd,
                                                  T* __restrict__ L, T* __restrict__ U) {
-  constexpr int PER = KC * 128 / NT;   // elements of each operand a thread stages per chunk
+  static_assert(NT % 256 == 0, "a thread stages one of the 256 rows");
+  constexpr int PER = KC * 256 / NT;   // loads of a thread per chunk: one row, every (NT/256)-th column
+  constexpr int CS = NT / 256;         // column stride of a thread's loads
   constexpr int WJ = NT / 128;         // waves along the columns
   constexpr int BJ = 128 / WJ / 16;    // 16x16 blocks of a wave along the columns (rows: 4)
-  constexpr int LD = 144;              // LDS row stride: consecutive k rows start 32 banks apart (no conflicts)
+  constexpr int LD = 272;              // LDS row stride: consecutive k rows start 32 banks apart (no conflicts)
+  constexpr int OOB = (int)0x80000000u;  // offset past any descriptor range: the load returns 0
   // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous range of the task list (tiles
   // of the same tile row and neighbouring columns), so the panel rows they share stay in that XCD's L2
   const int nt = gridDim.x, xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
@@ -476,25 +491,19 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
   const int rb = kb + kw;
   const int cend = MID ? min(k, kb + OB2) : m;      // columns updated: [rb, cend)
   const T* P = L + sd.Loff[s] + (long long)kb * m;  // column kb of the front
-  __shared__ T Xi[2][KC][LD];
-  __shared__ T Xj[2][KC][LD];
+  __shared__ T X[2][KC][LD];
   const int i0 = rb + 128 * tk.y, j0 = rb + 128 * tk.z;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int ib = (wv & 1) * 64, jb = (wv >> 1) * (128 / WJ);
+  const int ib = (wv & 1) * 64, jb = 128 + (wv >> 1) * (128 / WJ);
   const int l16 = lane & 15;
   const int nch = (kw + KC - 1) / KC;
-  // a thread stages tile row rr of columns q0, q0 + NT/128, ... of every chunk
-  const int rr = threadIdx.x & 127, q0 = threadIdx.x >> 7;
-  const bool iok = i0 + rr < m, jok = j0 + rr < m;
-  T ri[PER], rj[PER];
-  auto stage = [&](int buf) {
-#pragma unroll
-    for (int t = 0; t < PER; t++) {
-      const int q = q0 + (NT / 128) * t;
-      Xi[buf][q][rr] = ri[t];
-      Xj[buf][q][rr] = rj[t];
-    }
-  };
+  // a thread stages row vr of the chunk (vr < 128: tile row i0 + vr, else tile column j0 + vr - 128) in columns
+  // q0, q0 + CS, ...
+  const int vr = threadIdx.x & 255, q0 = threadIdx.x >> 8;
+  const int grow = vr < 128 ? i0 + vr : j0 + (vr - 128);
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)P, 0, kw * m * (int)sizeof(T), 0x00020000);
+  const int vo = grow < m ? (q0 * m + grow) * (int)sizeof(T) : OOB;
   typedef typename Mfma<T>::v4 v4;
   v4 acc[BJ][4];
 #pragma unroll
@@ -507,50 +516,46 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
       const int q = kk + (lane >> 4);
       T av[BJ], bv[4];
 #pragma unroll
-      for (int a = 0; a < BJ; a++) av[a] = Xj[buf][q][jb + 16 * a + l16];
+      for (int a = 0; a < BJ; a++) av[a] = X[buf][q][jb + 16 * a + l16];
 #pragma unroll
-      for (int b = 0; b < 4; b++) bv[b] = Xi[buf][q][ib + 16 * b + l16];
+      for (int b = 0; b < 4; b++) bv[b] = X[buf][q][ib + 16 * b + l16];
 #pragma unroll
       for (int a = 0; a < BJ; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
     }
   };
-  // the chunk pipeline, compiled twice: interior tiles (both 128-row panels inside the front, every chunk full) load
-  // without per-element checks, the others with them; predicated loads cost several percent of the MFMA rate, and
-  // the choice is uniform over the workgroup
+  // the chunk pipeline, compiled twice: interior tiles (both 128-row panels inside the front, every chunk full) skip
+  // the per-column checks; the choice is uniform over the workgroup. Chunks past the last one load zeros (the
+  // pipeline's body has no branches; an odd chunk count adds one chunk of zeros)
   auto pipeline = [&](auto checked) {
     constexpr bool CHK = decltype(checked)::value;
-    auto fetch = [&](int c) {
-      const T* cb = P + (long long)(c * KC) * m;  // wave-uniform base of the chunk
+    auto fetch = [&](T* r, int c) {
+      const bool live = c < nch;
+      const int cb = live ? c * KC * m * (int)sizeof(T) : 0;
 #pragma unroll
       for (int t = 0; t < PER; t++) {
-        const int q = q0 + (NT / 128) * t;
-        const unsigned off = (unsigned)(q * m);
-        if (CHK) {
-          const bool cok = c * KC + q < kw;
-          ri[t] = (cok && iok) ? cb[off + (unsigned)(i0 + rr)] : T(0);
-          rj[t] = (cok && jok) ? cb[off + (unsigned)(j0 + rr)] : T(0);
-        } else {
-          ri[t] = cb[off + (unsigned)(i0 + rr)];
-          rj[t] = cb[off + (unsigned)(j0 + rr)];
-        }
+        const int v = CHK ? (c * KC + q0 + CS * t < kw ? vo : OOB) : (live ? vo : OOB);
+        r[t] = buffer_ld<T>(rs, v, cb + CS * t * m * (int)sizeof(T));
       }
     };
-    fetch(0);
-    stage(0);
-    if (1 < nch) fetch(1);
+    auto stage = [&](const T* r, int buf) {
+#pragma unroll
+      for (int t = 0; t < PER; t++) X[buf][q0 + CS * t][vr] = r[t];
+    };
+    T ra[PER], rc[PER];
+    fetch(ra, 0);
+    stage(ra, 0);
+    fetch(rc, 1);
     __syncthreads();
-    for (int c = 0; c < nch; c++) {
-      const int buf = c & 1;
-      // stage chunk c + 1 (its buffer was last read before the previous barrier) and put the loads of chunk c + 2 in
-      // flight before the MFMAs of chunk c: the wait for the loads comes right after a barrier, while the other
-      // workgroup on the CU keeps the MFMA pipes busy
-      if (c + 1 < nch) {
-        stage(buf ^ 1);
-        if (c + 2 < nch) fetch(c + 2);
-      }
-      compute(buf);
+    for (int c = 0; c < nch; c += 2) {
+      fetch(ra, c + 2);
+      compute(0);
+      stage(rc, 1);
+      __syncthreads();
+      fetch(rc, c + 3);
+      compute(1);
+      stage(ra, 0);
       __syncthreads();
     }
   };
@@ -565,7 +570,7 @@ __global__ __launch_bounds__(NT) void k_syrk_big(const int4* __restrict__ tasks,
     for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        long long j = j0 + jb + 16 * a + Mfma<T>::row(lane, r);
+        long long j = j0 + (jb - 128) + 16 * a + Mfma<T>::row(lane, r);
         long long i = i0 + ib + 16 * b + l16;
         if (i < m && j < cend && i >= j) {
           // each entry of the trailing triangle belongs to exactly one tile of the launch: a hardware atomic add in
@@ -1364,10 +1369,12 @@ void SpdDevice::analyze(int n, const int* colPtr, const int* rowIdx, const int* 
   if (rc) throw Error(rc == -2 ? -2 : (rc == -1 ? -1 : -3), "symbolic analysis failed: " + S_.error);
   const Symbolic& S = S_;
   const int ns = S.ns;
-  // the outer-update kernel addresses a front's chunk of 16 columns with 32-bit offsets
+  // the sweeps address a front's panel with 32-bit element offsets, and the outer update reads a 512-column block of
+  // a front through a buffer descriptor with 32-bit byte offsets (512 x 2^19 x 8 B = 2^31; such a front's update
+  // block alone would need 2 TB)
   for (int s = 0; s < ns; s++)
-    if (S.sn_rowptr[s + 1] - S.sn_rowptr[s] >= (1LL << 27))
-      throw Error(-3, "a supernode front has 2^27 rows or more");
+    if (S.sn_rowptr[s + 1] - S.sn_rowptr[s] >= (1LL << 19))
+      throw Error(-3, "a supernode front has 2^19 rows or more");
   // children lists (ascending)
   std::vector<int> cptr(ns + 1, 0), clist;
   for (int s = 0; s < ns; s++) if (S.sn_parent[s] >= 0) cptr[S.sn_parent[s] + 1]++;
