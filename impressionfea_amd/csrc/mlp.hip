@@ -1,4 +1,4 @@
-// Per-node warm-start network on MI355X: layer kernels (BF16 weights, FP32 accumulation), backward, Adam.
+// Per-node warm-start network on MI355X: layer products on BF16 MFMA with FP32 accumulation, backward, Adam.
 // See include/ifea_mlp.h. All reductions over nodes are fixed-order partial sums.
 #include <hip/hip_runtime.h>
 
@@ -30,121 +30,135 @@ __device__ __forceinline__ float gelu_d(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
 }
 
-// Z = A W + b, Aout = gelu(Z) (or Z) ; A [n x K], W [K x M] bf16, 64 rows per block, thread = (row, column lane)
+// ---------------------------------------------------------------- the products on BF16 MFMA
+// Every product of the network is a tile C[64 x 128] = sum_r A(row, r) B(r, col) with both operands rounded to BF16
+// (nearest even) and FP32 accumulation: v_mfma_f32_16x16x16_bf16, four waves of 16 rows x 128 columns (8 blocks of
+// 16x16) per workgroup. Each 16-deep step of the reduction is staged in LDS as BF16, A row-major (the 4 consecutive
+// r a lane feeds are one 8-byte read) and B column-major (likewise). The operand accessors return 0 outside the
+// matrices, so partial tiles and feature counts below 16 (the 11 inputs, the 3 outputs) need no other handling.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int TR = 64, TC = 128, TK = 16;  // tile rows, tile columns, reduction step
+constexpr int SA_LD = TK + 4, SB_LD = TK + 4;  // LDS row strides in BF16 elements (8-byte aligned, spread banks)
+
+// lane l of the 16x16x16 instruction feeds A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16], i = 0..3, and
+// receives D[4 (l / 16) + i][l % 16]
+template <class FA, class FB>
+__device__ __forceinline__ void mfma_tile(int R, FA fa, FB fb, unsigned short (*sA)[SA_LD],
+                                          unsigned short (*sB)[SB_LD], f32x4 acc[TC / 16]) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int l16 = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+  for (int c = 0; c < TC / 16; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int r0 = 0; r0 < R; r0 += TK) {
+    __syncthreads();  // the previous step's operands are consumed
+    // A: 64 x 16 (4 per thread), B: 16 x 128 (8 per thread), all loads before the LDS writes
+    float va[TR * TK / 256], vb[TK * TC / 256];
+#pragma unroll
+    for (int u = 0; u < TR * TK / 256; u++) {
+      const int e = tid + 256 * u, row = e / TK, r = e % TK;
+      va[u] = fa(row, r0 + r);
+    }
+#pragma unroll
+    for (int u = 0; u < TK * TC / 256; u++) {
+      const int e = tid + 256 * u, col = e / TK, r = e % TK;
+      vb[u] = fb(r0 + r, col);
+    }
+#pragma unroll
+    for (int u = 0; u < TR * TK / 256; u++) {
+      const int e = tid + 256 * u;
+      sA[e / TK][e % TK] = f2bf(va[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < TK * TC / 256; u++) {
+      const int e = tid + 256 * u;
+      sB[e / TK][e % TK] = f2bf(vb[u]);
+    }
+    __syncthreads();
+    bf16x4 av;
+    __builtin_memcpy(&av, &sA[16 * wv + l16][kq], 8);
+#pragma unroll
+    for (int c = 0; c < TC / 16; c++) {
+      bf16x4 bv;
+      __builtin_memcpy(&bv, &sB[16 * c + l16][kq], 8);
+      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[c], 0, 0, 0);
+    }
+  }
+}
+
+// Z = A W + b, Aout = gelu(Z) (or Z); A [n x K] FP32, W [K x M] BF16; 64 rows per workgroup
 __global__ __launch_bounds__(256) void k_lin_fwd(const float* __restrict__ A, int n, int K,
                                                  const unsigned short* __restrict__ W, const float* __restrict__ b,
                                                  int M, int act, float* __restrict__ Z, float* __restrict__ Aout) {
-  __shared__ float sA[ROWS][MAXF + 1];
-  __shared__ float sW[MAXF][MAXF];
-  const int r0 = blockIdx.x * ROWS;
-  for (int i = threadIdx.x; i < ROWS * K; i += blockDim.x) {
-    int r = i / K, k = i % K;
-    sA[r][k] = r0 + r < n ? A[(long long)(r0 + r) * K + k] : 0.0f;
-  }
-  for (int i = threadIdx.x; i < K * M; i += blockDim.x) sW[i / M][i % M] = bf2f(W[i]);
-  __syncthreads();
-  const int r = threadIdx.x >> 2, cg = threadIdx.x & 3;
-  float acc[32];
+  __shared__ unsigned short sA[TR][SA_LD], sB[TC][SB_LD];
+  const int r0 = blockIdx.x * TR;
+  f32x4 acc[TC / 16];
+  mfma_tile(
+      K, [&](int row, int k) { return r0 + row < n && k < K ? A[(long long)(r0 + row) * K + k] : 0.0f; },
+      [&](int k, int col) { return k < K && col < M ? bf2f(W[k * M + col]) : 0.0f; }, sA, sB, acc);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < 32; j++) acc[j] = 0.0f;
-  for (int k = 0; k < K; k++) {
-    float a = sA[r][k];
+  for (int c = 0; c < TC / 16; c++)
 #pragma unroll
-    for (int j = 0; j < 32; j++) {
-      int c = cg + 4 * j;
-      if (c < M) acc[j] = fmaf(a, sW[k][c], acc[j]);
-    }
-  }
-  if (r0 + r < n) {
-#pragma unroll
-    for (int j = 0; j < 32; j++) {
-      int c = cg + 4 * j;
-      if (c < M) {
-        float z = acc[j] + b[c];
-        long long o = (long long)(r0 + r) * M + c;
+    for (int i = 0; i < 4; i++) {
+      const int row = r0 + 16 * wv + 4 * (lane >> 4) + i, col = 16 * c + (lane & 15);
+      if (row < n && col < M) {
+        const float z = acc[c][i] + b[col];
+        const long long o = (long long)row * M + col;
         if (Z) Z[o] = z;
         Aout[o] = act ? gelu(z) : z;
       }
     }
-  }
 }
 
 // dA[n x K] = (dZ[n x M] W^T) * (act ? gelu'(Zp) : 1)
 __global__ __launch_bounds__(256) void k_lin_bwd_data(const float* __restrict__ dZ, int n, int M,
                                                       const unsigned short* __restrict__ W, int K,
                                                       const float* __restrict__ Zp, int act, float* __restrict__ dA) {
-  __shared__ float sD[ROWS][MAXF + 1];
-  __shared__ float sW[MAXF][MAXF + 1];
-  const int r0 = blockIdx.x * ROWS;
-  for (int i = threadIdx.x; i < ROWS * M; i += blockDim.x) {
-    int r = i / M, m = i % M;
-    sD[r][m] = r0 + r < n ? dZ[(long long)(r0 + r) * M + m] : 0.0f;
-  }
-  for (int i = threadIdx.x; i < K * M; i += blockDim.x) sW[i / M][i % M] = bf2f(W[i]);
-  __syncthreads();
-  const int r = threadIdx.x >> 2, cg = threadIdx.x & 3;
-  float acc[32];
+  __shared__ unsigned short sA[TR][SA_LD], sB[TC][SB_LD];
+  const int r0 = blockIdx.x * TR;
+  f32x4 acc[TC / 16];
+  mfma_tile(
+      M, [&](int row, int m) { return r0 + row < n && m < M ? dZ[(long long)(r0 + row) * M + m] : 0.0f; },
+      [&](int m, int k) { return m < M && k < K ? bf2f(W[k * M + m]) : 0.0f; }, sA, sB, acc);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < 32; j++) acc[j] = 0.0f;
-  for (int m = 0; m < M; m++) {
-    float d = sD[r][m];
+  for (int c = 0; c < TC / 16; c++)
 #pragma unroll
-    for (int j = 0; j < 32; j++) {
-      int k = cg + 4 * j;
-      if (k < K) acc[j] = fmaf(d, sW[k][m], acc[j]);
-    }
-  }
-  if (r0 + r < n) {
-#pragma unroll
-    for (int j = 0; j < 32; j++) {
-      int k = cg + 4 * j;
-      if (k < K) {
-        long long o = (long long)(r0 + r) * K + k;
-        dA[o] = act ? acc[j] * gelu_d(Zp[o]) : acc[j];
+    for (int i = 0; i < 4; i++) {
+      const int row = r0 + 16 * wv + 4 * (lane >> 4) + i, k = 16 * c + (lane & 15);
+      if (row < n && k < K) {
+        const long long o = (long long)row * K + k;
+        dA[o] = act ? acc[c][i] * gelu_d(Zp[o]) : acc[c][i];
       }
     }
-  }
 }
 
-// part[c][(K+1) x M]: sum over the chunk's rows of [A | 1]^T dZ (the last row of the block is the bias gradient)
+// part[c][(K+1) x M]: over chunk c's rows, A^T dZ on MFMA (rows of the product: 64 input features per workgroup,
+// blockIdx.y the feature tile) and, by the first feature tile, the bias gradient sum(dZ) in FP32, rows ascending
 __global__ __launch_bounds__(256) void k_lin_bwd_w(const float* __restrict__ A, const float* __restrict__ dZ, int n,
                                                    int K, int M, int rows_per_chunk, float* __restrict__ part) {
-  __shared__ float sA[32][MAXF + 1];
-  __shared__ float sD[32][MAXF + 1];
-  const int c = blockIdx.x;
+  __shared__ unsigned short sA[TR][SA_LD], sB[TC][SB_LD];
+  const int c = blockIdx.x, k0 = blockIdx.y * TR;
   const int r_begin = c * rows_per_chunk, r_end = min(n, r_begin + rows_per_chunk);
-  const int nout = (K + 1) * M;
-  float acc[65];
+  const int nr = r_end - r_begin;
+  f32x4 acc[TC / 16];
+  mfma_tile(
+      nr, [&](int kk, int r) { return k0 + kk < K && r < nr ? A[(long long)(r_begin + r) * K + k0 + kk] : 0.0f; },
+      [&](int r, int m) { return r < nr && m < M ? dZ[(long long)(r_begin + r) * M + m] : 0.0f; }, sA, sB, acc);
+  float* pc = part + (long long)c * (K + 1) * M;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < 65; j++) acc[j] = 0.0f;
-  for (int t0 = r_begin; t0 < r_end; t0 += 32) {
-    int rows = min(32, r_end - t0);
-    __syncthreads();
-    for (int i = threadIdx.x; i < 32 * (K + 1); i += blockDim.x) {
-      int r = i / (K + 1), k = i % (K + 1);
-      sA[r][k] = r < rows ? (k < K ? A[(long long)(t0 + r) * K + k] : 1.0f) : 0.0f;
-    }
-    for (int i = threadIdx.x; i < 32 * M; i += blockDim.x) {
-      int r = i / M, m = i % M;
-      sD[r][m] = r < rows ? dZ[(long long)(t0 + r) * M + m] : 0.0f;
-    }
-    __syncthreads();
+  for (int cc = 0; cc < TC / 16; cc++)
 #pragma unroll
-    for (int j = 0; j < 65; j++) {
-      int o = threadIdx.x + 256 * j;
-      if (o < nout) {
-        int k = o / M, m = o % M;
-        float s = acc[j];
-        for (int r = 0; r < rows; r++) s = fmaf(sA[r][k], sD[r][m], s);
-        acc[j] = s;
-      }
+    for (int i = 0; i < 4; i++) {
+      const int k = k0 + 16 * wv + 4 * (lane >> 4) + i, m = 16 * cc + (lane & 15);
+      if (k < K && m < M) pc[k * M + m] = acc[cc][i];
     }
-  }
-  float* pc = part + (long long)c * nout;
-#pragma unroll
-  for (int j = 0; j < 65; j++) {
-    int o = threadIdx.x + 256 * j;
-    if (o < nout) pc[o] = acc[j];
+  if (blockIdx.y == 0 && threadIdx.x < M) {
+    float sb = 0.0f;
+    for (int r = r_begin; r < r_end; r++) sb += dZ[(long long)r * M + threadIdx.x];
+    pc[K * M + threadIdx.x] = sb;
   }
 }
 
@@ -291,7 +305,7 @@ struct Mlp {
       int K = fin[l], M = fout[l];
       int rpc = (n + WCHUNKS - 1) / WCHUNKS;
       int C = (n + rpc - 1) / rpc;
-      hipLaunchKernelGGL(k_lin_bwd_w, dim3(C), dim3(256), 0, s, in, dz, n, K, M, rpc, part.p);
+      hipLaunchKernelGGL(k_lin_bwd_w, dim3(C, (K + TR - 1) / TR), dim3(256), 0, s, in, dz, n, K, M, rpc, part.p);
       int len2 = (K + 1) * M;
       hipLaunchKernelGGL(k_reduce_part, dim3((len2 + 255) / 256), dim3(256), 0, s, part.p, C, len2, g.p + woff[l],
                          g.p + boff[l], K, M);

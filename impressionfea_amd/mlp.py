@@ -1,7 +1,7 @@
 """Learned warm start: a per-node MLP that predicts each stroke increment's nodal velocity field.
 
-The network (11 -> 128 -> 128 -> 128 -> 3, GELU, BF16 weights with FP32 accumulation) runs and trains through the
-hand-written HIP kernels behind include/ifea_mlp.h. Inputs per node, all dimensionless: position / billet size (3),
+The network (11 -> 128 -> 128 -> 128 -> 3, GELU; its products on BF16 MFMA with FP32 accumulation) runs and trains
+through the hand-written HIP kernels behind include/ifea_mlp.h. Inputs per node, all dimensionless: position / billet size (3),
 the last two converged nodal velocities / ram speed (6), die-contact flag (1), distance to the nearest die / billet
 size (1). Output: the change of the nodal velocity over the coming increment / ram speed. The prediction plugs into
 the reference's LoadVelocityGuess slot (Forging3DSolver.cs:854-869); the solver judges it (see Forging3DSolver.step)

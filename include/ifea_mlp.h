@@ -5,7 +5,9 @@
  * and the ExtrapolateStart trend (:44, :856-859).
  *
  * Network: n_in -> width (GELU) -> ... (depth hidden layers) -> n_out, row-major activations [nodes x features].
- * Weights are BF16 in the products with FP32 accumulation; FP32 master weights and Adam moments are kept on the device.
+ * Every product runs on BF16 MFMA (v_mfma_f32_16x16x16_bf16) with both operands rounded to BF16 and FP32 accumulation;
+ * activations and gradients stay FP32 between the products (the bias gradient is an FP32 sum), and FP32 master weights
+ * and Adam moments are kept on the device.
  * Parameter vector layout (FP32): for each layer l: W_l [fan_in x fan_out] row-major, then b_l [fan_out].
  * Reductions over nodes (weight gradients, loss) use fixed-order partial sums: results are bit-reproducible.
  */

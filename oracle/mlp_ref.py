@@ -1,8 +1,10 @@
 """TEST INFRASTRUCTURE ONLY — FP64 CPU reference of the warm-start network (forward, MSE loss, backward).
 
 The network has no counterpart in the reference; this is the checker for the HIP kernels behind include/ifea_mlp.h.
-Weights are rounded to BF16 (round to nearest even) before use, as the device does, so the comparison isolates the
-FP32-accumulation error of the kernels. Pinned by finite differences in tests/test_mlp.py.
+Both operands of every product are rounded to BF16 (round to nearest even) before use, as the device's BF16 MFMA
+products take them (weights, activations, and the backward's output gradients; the bias gradient is summed from the
+unrounded gradients), so the comparison isolates the FP32-accumulation error of the kernels. With bf16=False it is
+the exact network, pinned by finite differences in tests/test_mlp.py.
 """
 from __future__ import annotations
 
@@ -43,7 +45,7 @@ def forward(params, shapes, x, bf16=True):
     zs, acts = [], [a]
     for l, (W, b) in enumerate(layers):
         Wq = to_bf16(W.astype(np.float32)) if bf16 else W
-        z = a @ Wq + b
+        z = (to_bf16(a) if bf16 else a) @ Wq + b
         zs.append(z)
         a = gelu(z) if l < len(layers) - 1 else z
         acts.append(a)
@@ -62,10 +64,11 @@ def loss_grad(params, shapes, x, y, bf16=True):
     for l in range(len(layers) - 1, -1, -1):
         W, b = layers[l]
         Wq = to_bf16(W.astype(np.float32)) if bf16 else W
-        gW = acts[l].T @ dz
+        q = to_bf16 if bf16 else (lambda v: v)
+        gW = q(acts[l]).T @ q(dz)
         gb = dz.sum(axis=0)
         grads[l] = (gW, gb)
         if l > 0:
-            dz = (dz @ Wq.T) * gelu_d(zs[l - 1])
+            dz = (q(dz) @ Wq.T) * gelu_d(zs[l - 1])
     flat = np.concatenate([np.concatenate([g.ravel(), gb]) for g, gb in grads])
     return loss, flat

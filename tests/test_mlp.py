@@ -43,11 +43,20 @@ def test_device_forward_backward_match_reference():
     y = rng.standard_normal((1000, 3)).astype(np.float32)
     yh = net.forward(x)
     yr = R.forward(p, shapes, x)[0]
-    assert np.abs(yh - yr).max() <= 1e-4 * np.abs(yr).max()
+    # Both sides round the same operands to BF16, but from activations that differ in the last FP32 bits (FP32
+    # accumulation on the device, FP64 in the reference): an activation within those bits of a BF16 rounding boundary
+    # rounds to the neighbouring BF16 value on one side (a 2^-8 relative step), so a few outputs differ by up to a few
+    # such steps of one term while almost all agree to the FP32 accumulation error.
+    err = np.abs(yh - yr) / np.abs(yr).max()
+    flipped = float(np.mean(err > 1e-5))
+    print(f"forward: median {np.median(err):.2e}, max {err.max():.2e}, outputs beyond 1e-5: {100 * flipped:.2f} %")
+    assert np.median(err) <= 1e-6 and flipped <= 0.02 and err.max() <= 4e-3
     loss, g = net.loss_grad(x, y)
     lr, gr = R.loss_grad(p, shapes, x, y)
+    gerr = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print(f"loss {loss:.8e} vs {lr:.8e}; gradient relative error {gerr:.2e}")
     assert loss == pytest.approx(lr, rel=1e-5)
-    assert np.linalg.norm(g - gr) <= 1e-4 * np.linalg.norm(gr)
+    assert gerr <= 1e-4
     # bit-reproducible reductions
     loss2, g2 = net.loss_grad(x, y)
     assert loss2 == loss and np.array_equal(g, g2)
