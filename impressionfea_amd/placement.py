@@ -29,7 +29,12 @@ def dot3(a, b):
 def tet_quality(x, tets):
     """TetMesh.Quality (TetMesh.cs:107-112) of the given tets in the reference's operation order: 6 sqrt(2) V / l2^1.5
     with l2 the mean squared edge length (1 regular, <= 0 inverted, -1 degenerate)."""
-    a, b, c, d = (x[tets[..., k]] for k in range(4))
+    return tet_quality_pts(x[tets])
+
+
+def tet_quality_pts(P):
+    """tet_quality of tets given by their corner positions P[..., 4, 3] (the same operations, element by element)."""
+    a, b, c, d = (P[..., k, :] for k in range(4))
     l2 = (dot3(b - a, b - a) + dot3(c - a, c - a) + dot3(d - a, d - a) + dot3(c - b, c - b) + dot3(d - b, d - b)
           + dot3(d - c, d - c)) / 6.0
     vol = dot3(b - a, np.cross(c - a, d - a)) / 6.0
@@ -86,6 +91,15 @@ def tets_of(mesh, n):
 
 def worst_around(mesh, x, n):
     return float(tet_quality(x, mesh.tets[tets_of(mesh, n)]).min())
+
+
+def worst_around_at(mesh, x, n, positions):
+    """worst_around(mesh, x, n) with node n moved to each of positions[k] in turn (one evaluation for all of them)."""
+    tt = mesh.tets[tets_of(mesh, n)]
+    P = np.repeat(x[tt][None], len(positions), axis=0)       # (k, tets, 4, 3)
+    at = tt == n
+    P[:, at] = np.asarray(positions, dtype=np.float64)[:, None, :]
+    return tet_quality_pts(P).min(axis=1)
 
 
 def sym1(mesh, p, n):
@@ -210,13 +224,15 @@ def improve_placement(mesh, x, contact, pred, nrm, gap, q_min, h_local, cfg, bad
                     if pen * (1 - f) <= corner * h_local[n]:
                         cand.append(pred[n] + f * (closest - pred[n]))
                 old = x[n].copy()
-                best = worst_around(mesh, x, n)
+                # the worst quality around n at its position and at every candidate, evaluated at once; then the
+                # reference's choice, candidate by candidate
+                qs = worst_around_at(mesh, x, n, [old] + cand)
+                best = float(qs[0])
                 pick = -1
                 for ci, cpt in enumerate(cand):
                     if np.array_equal(cpt, old):
                         continue
-                    x[n] = cpt
-                    q = worst_around(mesh, x, n)
+                    q = float(qs[ci + 1])
                     if q > best + 1e-6:
                         best, pick = q, ci
                 x[n] = cand[pick] if pick >= 0 else old
