@@ -86,6 +86,10 @@ class SpdDevice {
   hipStream_t chain_ = nullptr, outer_ = nullptr;
   hipEvent_t ev_main_ = nullptr, ev_chain_ = nullptr;
   hipStream_t chain_stream();
+  // the sweeps' side stream and fork/join events (two per level)
+  hipStream_t side_ = nullptr;
+  std::vector<hipEvent_t> sweep_ev_;
+  void sweep_events();
   Symbolic S_;
   bool analyzed_ = false, factored_ = false;
   int prec_ = 0;

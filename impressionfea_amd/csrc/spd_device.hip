@@ -1335,6 +1335,11 @@ __global__ void k_perm_out(const double* __restrict__ y, const int* __restrict__
 SpdDevice::~SpdDevice() {
   drop_graph();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+  if (side_) {
+    (void)hipStreamSynchronize(side_);
+    (void)hipStreamDestroy(side_);
+  }
+  for (hipEvent_t e : sweep_ev_) (void)hipEventDestroy(e);
   if (chain_) {
     (void)hipStreamSynchronize(chain_);
     (void)hipStreamSynchronize(outer_);
@@ -1886,13 +1891,27 @@ void SpdDevice::sweeps(const T* L) {
       hipLaunchKernelGGL(kern, dim3((unsigned)(ll.end - ll.begin)), dim3(threads), 0, stream_, tasks + ll.begin, sd_,
                          args...);
   };
+  // Within a level the narrow fronts (one workgroup each) and the wide fronts' step chain touch disjoint fronts: the
+  // narrow ones run on the side stream, forked from the level's start and joined before the next level (in the
+  // captured graph, two parallel branches per level)
+  auto fork = [&](size_t l) {
+    IFEA_HIP(hipEventRecord(sweep_ev_[2 * l], stream_));
+    IFEA_HIP(hipStreamWaitEvent(side_, sweep_ev_[2 * l], 0));
+  };
+  auto join = [&](size_t l) {
+    IFEA_HIP(hipEventRecord(sweep_ev_[2 * l + 1], side_));
+    IFEA_HIP(hipStreamWaitEvent(stream_, sweep_ev_[2 * l + 1], 0));
+  };
+  sweep_events();
   for (size_t l = 0; l < levels_.size(); l++) {
     const Level& lv = levels_[l];
     unsigned cnt = (unsigned)(lv.small_end - lv.small_begin);
+    const bool both = cnt && lv.fwd_gather.end > lv.fwd_gather.begin;
     if (cnt) {
       size_t sh = (size_t)std::min<long long>(lv.small_max_m, cap) * sizeof(double);
-      hipLaunchKernelGGL(k_fwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L,
-                         Linv_.p, y_.p, V_.p, W_.p, cap);
+      if (both) fork(l);
+      hipLaunchKernelGGL(k_fwd<T>, dim3(cnt), dim3(256), sh, both ? side_ : stream_, sweep_sn_.p + lv.small_begin,
+                         sd_, L, Linv_.p, y_.p, V_.p, W_.p, cap);
     }
     launch(k_fwdb_gather, lv.fwd_gather, fwd_tasks_.p, (const double*)y_.p, (const double*)V_.p, W_.p);
     for (size_t t = 0; t < lv.fwd_diag.size(); t++) {
@@ -1909,14 +1928,17 @@ void SpdDevice::sweeps(const T* L) {
                (const double*)Pr_.p);
     else
       launch_n(k_fwdb_rect_sum<NB>, NB, lv.fwd_rect_sum, fwd_tasks_.p, (const double*)W_.p, V_.p, (const double*)Pr_.p);
+    if (both) join(l);
   }
   for (size_t l = levels_.size(); l-- > 0;) {
     const Level& lv = levels_[l];
     unsigned cnt = (unsigned)(lv.small_end - lv.small_begin);
+    const bool both = cnt && lv.bwd_gemv.end > lv.bwd_gemv.begin;
     if (cnt) {
       size_t sh = (size_t)std::min<long long>(lv.small_max_m, cap) * sizeof(double);
-      hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, stream_, sweep_sn_.p + lv.small_begin, sd_, L,
-                         Linv_.p, y_.p, W_.p, cap);
+      if (both) fork(l);
+      hipLaunchKernelGGL(k_bwd<T>, dim3(cnt), dim3(256), sh, both ? side_ : stream_, sweep_sn_.p + lv.small_begin,
+                         sd_, L, Linv_.p, y_.p, W_.p, cap);
     }
     launch(k_bwdb_xb, lv.bwd_xb, bwd_tasks_.p, (const double*)y_.p, V_.p);
     launch(k_bwdb_gemv<T>, lv.bwd_gemv, bwd_tasks_.p, L, (const double*)y_.p, (const double*)V_.p, W_.p);
@@ -1924,6 +1946,18 @@ void SpdDevice::sweeps(const T* L) {
       launch_n(k_bwdb_diag<T>, DIAG_WAVES * 64, lv.bwd_diag[t], bwd_tasks_.p, Xs, y_.p, (const double*)W_.p);
       launch(k_bwdb_panel<T>, lv.bwd_panel[t], bwd_tasks_.p, L, (const double*)y_.p, W_.p);
     }
+    if (both) join(l);
+  }
+}
+
+// The sweeps' side stream (the narrow fronts of a level, beside the wide fronts' chain on the main stream) and one
+// fork/join event pair per level, created once (default priority).
+void SpdDevice::sweep_events() {
+  if (!side_) IFEA_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+  while (sweep_ev_.size() < 2 * levels_.size()) {
+    hipEvent_t e;
+    IFEA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    sweep_ev_.push_back(e);
   }
 }
 
