@@ -201,14 +201,24 @@ def improve_placement(mesh, x, contact, pred, nrm, gap, q_min, h_local, cfg, bad
                 for i, n in enumerate(sel):
                     cache[(int(n), side)] = (cl[i], pu[i], float(pe[i]))
     total = 0
+    # qualities evaluated at the current positions stay valid until a node moves: the queued tets are evaluated in one
+    # call and a node's candidate qualities are kept, both dropped at the first move (moves are rare; most sweeps
+    # only confirm that no candidate helps)
+    qcache, ccache = {}, {}
     for sweep in range(4):
         count = 0
         heap = [int(e) for e in (first if sweep == 0 else bad_tets(x, q_min))]
         heapq.heapify(heap)
         queued = set(heap)
+        if heap:
+            qcache = dict(zip(heap, tet_quality(x, t[np.asarray(heap)]).tolist()))
+        ccache = {}
         while heap:
             e = heapq.heappop(heap)
-            if float(tet_quality(x, t[e:e + 1])[0]) >= q_min:
+            qe = qcache.get(e)
+            if qe is None:
+                qe = float(tet_quality(x, t[e:e + 1])[0])
+            if qe >= q_min:
                 continue
             for a in range(4):
                 n = int(t[e, a])
@@ -226,7 +236,9 @@ def improve_placement(mesh, x, contact, pred, nrm, gap, q_min, h_local, cfg, bad
                 old = x[n].copy()
                 # the worst quality around n at its position and at every candidate, evaluated at once; then the
                 # reference's choice, candidate by candidate
-                qs = worst_around_at(mesh, x, n, [old] + cand)
+                qs = ccache.get(n)
+                if qs is None:
+                    qs = ccache[n] = worst_around_at(mesh, x, n, [old] + cand)
                 best = float(qs[0])
                 pick = -1
                 for ci, cpt in enumerate(cand):
@@ -241,6 +253,7 @@ def improve_placement(mesh, x, contact, pred, nrm, gap, q_min, h_local, cfg, bad
                 if np.array_equal(cand[pick], pred[n]) and pen < 0:
                     contact[n] = FREE
                 count += 1
+                qcache, ccache = {}, {}
                 for e2 in tets_of(mesh, n):       # tets after e whose quality this move may have changed
                     if e2 > e and int(e2) not in queued:
                         queued.add(int(e2))
