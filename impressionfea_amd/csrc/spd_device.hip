@@ -139,6 +139,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_inv(const int4* __restrict__ tasks, int c0, SymDev sd,
                                                    T* __restrict__ L, double* __restrict__ Linv,
                                                    int* __restrict__ err) {
+  // the panel chain is the factorization's critical path and shares CUs with the outer updates of the previous
+  // block: its waves take issue priority over theirs (a wave priority, not a queue priority)
+  __builtin_amdgcn_s_setprio(3);
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
@@ -309,6 +312,7 @@ __global__ __launch_bounds__(256) void k_potrf_inv(const int4* __restrict__ task
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ tasks, int c0, SymDev sd,
                                                    T* __restrict__ L, const double* __restrict__ Linv) {
+  __builtin_amdgcn_s_setprio(3);  // a panel-chain kernel: issue priority, as in k_potrf_inv
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
@@ -372,6 +376,7 @@ __global__ __launch_bounds__(256) void k_trsm_mfma(const int4* __restrict__ task
 template <typename T>
 __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, int kb, SymDev sd,
                                               T* __restrict__ L) {
+  __builtin_amdgcn_s_setprio(3);  // a panel-chain kernel: issue priority, as in k_potrf_inv
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x;
   const int k = sn_k(sd, s);
