@@ -607,39 +607,40 @@ __global__ __launch_bounds__(64 * TAN_WAVES) void k_node(int md, int me, int nn,
 #pragma unroll
   for (int c = 0; c < 3; c++) rj[c] += wsum(c3[c]);
   wave_sync_lds();
-  // elements around j in ascending order, 36 lanes per element for the 4 x 3 x 3 block entries
+  // elements around j in ascending order, 36 lanes per element for the 4 x 3 x 3 block entries. Lane (a, c, d) =
+  // 9 a + 3 c + d computes the element matrix entry Ke_ab[c][d] once; the rotated block
+  // blk[r][c'] = sum_c Ri[r][c] (sum_d Ke[c][d] Rj[c'][d]) then takes two three-term products, the inner one from
+  // the row (a, c, *) and the outer one from the column (a, *, c') of the lanes' values (lane shuffles)
   const int la = lane / 9, lr = (lane / 3) % 3, lc = lane % 3;
+  const int lac = min(la, 3);  // lanes past the 36 compute a copy of block 3 and store nothing
   const double rl0 = lc == 0 ? Rjv[0] : (lc == 1 ? Rjv[3] : Rjv[6]);
   const double rl1 = lc == 0 ? Rjv[1] : (lc == 1 ? Rjv[4] : Rjv[7]);
   const double rl2 = lc == 0 ? Rjv[2] : (lc == 1 ? Rjv[5] : Rjv[8]);
+  const int src_row = 9 * lac + 3 * lr, src_col = 9 * lac + lc;
   for (int q = 0; q < nej; q++) {
     const int pos = lane < 36 ? et[4 * q + la] : -1;
+    const int b = eb[q];
+    const double w = ew[q], wnn = ewn[q];
+    const double* ga = eg + 12 * q + 3 * lac;
+    const double* gb = eg + 12 * q + 3 * b;
+    const double* Gq = eG + 12 * q;
+    const double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+    // Ke_ab[c][d] = wm * (B^T M B)_{3a+c, 3b+d} + wn * G_{3a+c} G_{3b+d}, with c = lr, d = lc
+    const double ga_c = ga[lr], gb_d = gb[lc], gb_c = gb[lr], ga_d = ga[lc];
+    const double dl = lr == lc ? 1.0 : 0.0;
+    const double bmb = 2.0 / 3.0 * ga_c * gb_d * (dl - 1.0 / 3.0) + (dl * (gab - 2.0 * ga_c * gb_c) + ga_d * gb_c) / 3.0;
+    const double kcd = w * bmb + wnn * Gq[3 * lac + lr] * Gq[3 * b + lc];
+    // kr[c][c'] = sum_d Ke[c][d] Rj[c'][d] for c = lr, c' = lc
+    const double k0 = __shfl(kcd, src_row, 64), k1 = __shfl(kcd, src_row + 1, 64), k2 = __shfl(kcd, src_row + 2, 64);
+    const double kr = k0 * rl0 + k1 * rl1 + k2 * rl2;
+    // blk[r][c'] = sum_c Ri[r][c] kr[c][c'] for r = lr, c' = lc
+    const double t0 = __shfl(kr, src_col, 64), t1 = __shfl(kr, src_col + 3, 64), t2 = __shfl(kr, src_col + 6, 64);
     if (pos >= 0) {
-      const int b = eb[q];
-      const double w = ew[q], wnn = ewn[q];
-      const double* ga = eg + 12 * q + 3 * la;
-      const double* gb = eg + 12 * q + 3 * b;
-      const double* Gq = eG + 12 * q;
-      const double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-      // Ke_ab[c][d] = wm * (B^T M B)_{3a+c, 3b+d} + wn * G_{3a+c} G_{3b+d}
-      double K[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          double dl = c == d ? 1.0 : 0.0;
-          double bmb = 2.0 / 3.0 * ga[c] * gb[d] * (dl - 1.0 / 3.0) +
-                       (dl * (gab - 2.0 * ga[c] * gb[c]) + ga[d] * gb[c]) / 3.0;
-          K[c][d] = w * bmb + wnn * Gq[3 * la + c] * Gq[3 * b + d];
-        }
       const double* Ri = axn + 9 * pos;
-      // blk[r][c'] = sum_c sum_d Ri[r][c] K[c][d] Rj[c'][d]
       double sum = 0.0;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        double kr = K[c][0] * rl0 + K[c][1] * rl1 + K[c][2] * rl2;
-        sum += Ri[3 * lr + c] * kr;
-      }
+      sum += Ri[3 * lr] * t0;
+      sum += Ri[3 * lr + 1] * t1;
+      sum += Ri[3 * lr + 2] * t2;
       acc[lc * d3 + 3 * pos + lr] += sum;
     }
     wave_sync_lds();
