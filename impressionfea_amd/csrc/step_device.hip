@@ -1238,31 +1238,39 @@ int StepDevice::pcg(const double* b, double* x, double tol, int maxIt, ifea_step
     IFEA_HIP(hipMemsetAsync(x, 0, ndof_ * sizeof(double), s_));
     return 0;
   }
-  lin_solve(b, x, st);
-  spmv_sym(s_, ndof_, colPtr_.p, rowIdx_.p, val_.p, x, pq_.p);
-  IFEA_HIP(hipMemcpyAsync(pr_.p, b, ndof_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
-  axpby(s_, ndof_, -1.0, pq_.p, 1.0, pr_.p);
-  lin_solve(pr_.p, pz_.p, st);
-  IFEA_HIP(hipMemcpyAsync(pp_.p, pz_.p, ndof_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
-  vec_->dot(pr_.p, pz_.p, ndof_, S_RZ);
-  for (int it = 0; it < maxIt; it++) {
+  // The reference tests the residual at the top of each iteration, after the preconditioner has been applied to it;
+  // here the test comes right after the residual update, so the preconditioner sweep of the residual that meets the
+  // tolerance (or ends the budget) is not made. Iterates, tests and return values are those of the reference.
+  auto residual_test = [&]() {
     vec_->dot(pr_.p, pr_.p, ndof_, S_DOT1);
     double rn2;
     vec_->read(S_DOT1, 1, &rn2);
     if (!std::isfinite(rn2)) return -2;
-    if (std::sqrt(rn2) <= tol * bnorm) return it;
+    return std::sqrt(rn2) <= tol * bnorm ? 1 : 0;
+  };
+  lin_solve(b, x, st);
+  spmv_sym(s_, ndof_, colPtr_.p, rowIdx_.p, val_.p, x, pq_.p);
+  IFEA_HIP(hipMemcpyAsync(pr_.p, b, ndof_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+  axpby(s_, ndof_, -1.0, pq_.p, 1.0, pr_.p);
+  int t = residual_test();
+  if (t != 0) return t > 0 ? 0 : -2;
+  if (maxIt <= 0) return -1;
+  lin_solve(pr_.p, pz_.p, st);
+  IFEA_HIP(hipMemcpyAsync(pp_.p, pz_.p, ndof_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+  vec_->dot(pr_.p, pz_.p, ndof_, S_RZ);
+  for (int it = 0; it < maxIt; it++) {
     if (st) st->pcg_iterations++;
     spmv_sym(s_, ndof_, colPtr_.p, rowIdx_.p, val_.p, pp_.p, pq_.p);
     vec_->dot(pp_.p, pq_.p, ndof_, S_PQ);
     pcg_update_x_r(s_, ndof_, vec_->slots(), pp_.p, pq_.p, x, pr_.p);
+    t = residual_test();
+    if (t != 0) return t > 0 ? it + 1 : -2;
+    if (it + 1 == maxIt) break;
     lin_solve(pr_.p, pz_.p, st);
     vec_->dot(pr_.p, pz_.p, ndof_, S_RZN);
     pcg_update_p(s_, ndof_, vec_->slots(), pz_.p, pp_.p);
   }
-  vec_->dot(pr_.p, pr_.p, ndof_, S_DOT1);
-  double rn2;
-  vec_->read(S_DOT1, 1, &rn2);
-  return std::sqrt(rn2) <= tol * bnorm ? maxIt : -1;
+  return -1;
 }
 
 // SolveLinear (:965-1019) with the mixed-precision refinement slot
