@@ -40,23 +40,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# The arms: (factor precision, start, adaptive first-solve policy). "fp64" is the reference's algorithm on the GPU;
-# the two middle arms separate the effect of the start from that of the precision; "mixed" is the full method.
+# The arms: (factor precision, start, adaptive first-solve policy, PCG tolerance). "fp64" is the reference's algorithm
+# on the GPU (PCG to 1e-7, Forging3DSolver.cs:984); the two middle arms separate the effect of the start from that of
+# the precision; "mixed" is the full method, whose Newton iteration is inexact: PCG stops at a relative residual of
+# 1e-4 (Newton's convergence test on the velocity step is the reference's).
 ARMS = {
-    "fp64": dict(precision=0, start="previous", first_solve=False),
-    "fp64_trend": dict(precision=0, start="trend", first_solve=False),
-    "mixed_previous": dict(precision=1, start="previous", first_solve=False),
-    "mixed": dict(precision=1, start="learned", first_solve=True),
+    "fp64": dict(precision=0, start="previous", first_solve=False, pcg_tol=1e-7),
+    "fp64_trend": dict(precision=0, start="trend", first_solve=False, pcg_tol=1e-7),
+    "mixed_previous": dict(precision=1, start="previous", first_solve=False, pcg_tol=1e-7),
+    "mixed": dict(precision=1, start="learned", first_solve=True, pcg_tol=1e-4),
 }
 
 
-def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True, start="previous", first_solve=False):
+def run_arm(wl, precision, steps, warmup, warm_start=None, profile=True, start="previous", first_solve=False,
+            pcg_tol=1e-7):
     from dataclasses import replace
     from impressionfea_amd.forging3d import Forging3DSolver
     if start == "trend":
         wl = replace(wl, cfg=replace(wl.cfg, extrapolate_start=True))
     s = Forging3DSolver.from_workload(wl, precision=precision, warm_start=warm_start if start == "learned" else None,
-                                      first_solve_policy=first_solve)
+                                      first_solve_policy=first_solve, pcg_tol=pcg_tol)
     s.set_params(profile=profile)
     loads, ms_solve, ms_update, stats, all_loads = [], [], [], [], []
     for k in range(warmup):
@@ -265,9 +268,9 @@ def main():
     for a in arms:
         spec = ARMS[a]
         log(f"arm {a}: precision {spec['precision']}, {spec['start']} start, adaptive first solve "
-            f"{'on' if spec['first_solve'] else 'off'}")
+            f"{'on' if spec['first_solve'] else 'off'}, PCG tolerance {spec['pcg_tol']:g}")
         res[a] = run_arm(wl, spec["precision"], args.steps, args.warmup, warm_start=ws, start=spec["start"],
-                         first_solve=spec["first_solve"])
+                         first_solve=spec["first_solve"], pcg_tol=spec["pcg_tol"])
     head = res.get("mixed") or res[arms[-1]]
     ips = {k: len(v["ms_solve"]) / v["wall"] for k, v in res.items()}                  # whole increments, wall clock
     ips_solve = {k: len(v["ms_solve"]) / (sum(v["ms_solve"]) / 1e3) for k, v in res.items()}  # without TryUpdate

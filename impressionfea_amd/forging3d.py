@@ -137,7 +137,8 @@ class Forging3DSolver:
     """One billet stepped by increments on the GPU (Forging3DSolver.Step); remeshing when cfg.remesh is set."""
 
     def __init__(self, cfg: StepConfig, x, tets, onX0, onY0, gap0: float, precision: int = 0, perm=None,
-                 warm_start=None, first_solve_policy: bool = False, device_frames: bool = True):
+                 warm_start=None, first_solve_policy: bool = False, device_frames: bool = True,
+                 pcg_tol: float = 1e-7):
         self.cfg = cfg
         consts = cfg.constants()
         self.rate_ref, self.rate_cut, self.sig_ref = consts["rateRef"], consts["rateCut"], consts["sigRef"]
@@ -153,6 +154,10 @@ class Forging3DSolver:
         self.recovery_pass = 0     # Step's recovery ladder (:553-621): 0 as is, 1 after a remesh
         self.precision = precision
         self._first_solve = bool(first_solve_policy)
+        # relative residual at which PCG on the cached factor stops: the reference's 1e-7 (SolveLinear,
+        # Forging3DSolver.cs:984) unless an inexact Newton iteration is asked for (a looser tolerance; Newton's own
+        # convergence test on the velocity step is unchanged)
+        self.pcg_tol = float(pcg_tol)
         # PrepareStep's frames on the device (the host's compute_frames is the same arithmetic, kept for checks)
         self.device_frames = bool(device_frames)
         self._h = None
@@ -223,7 +228,7 @@ class Forging3DSolver:
         p = N.StepParams()
         p.C, p.n, p.m, p.eps0 = c.C, c.n, c.m, c.eps0
         p.friction, p.ram, p.kpen, p.rate_cut, p.u0 = c.friction, c.ram, self.k_pen, self.rate_cut, self.u0
-        p.newton_blend, p.tolerance, p.pcg_tol = c.newton_blend, c.tolerance, 1e-7
+        p.newton_blend, p.tolerance, p.pcg_tol = c.newton_blend, c.tolerance, self.pcg_tol
         p.max_iterations, p.pcg_budget = c.max_iterations, c.pcg_budget
         p.quadratic_ls, p.adaptive_newton = int(c.quadratic_ls), int(c.adaptive_newton)
         p.precision, p.newton_limit = int(self.precision), 20
