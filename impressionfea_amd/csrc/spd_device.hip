@@ -496,7 +496,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 128))) 
   const int rb = kb + kw;
   const int cend = MID ? min(k, kb + OB2) : m;      // columns updated: [rb, cend)
   const T* P = L + sd.Loff[s] + (long long)kb * m;  // column kb of the front
-  __shared__ T X[2][KC][LD];
   const int i0 = rb + 128 * tk.y, j0 = rb + 128 * tk.z;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ib = (wv & 1) * 64, jb = 128 + (wv >> 1) * (128 / WJ);
@@ -515,15 +514,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 128))) 
   for (int a = 0; a < BJ; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = v4{0, 0, 0, 0};
-  auto compute = [&](int buf) {
+  auto compute = [&](const T (*Xb)[LD]) {
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int q = kk + (lane >> 4);
       T av[BJ], bv[4];
 #pragma unroll
-      for (int a = 0; a < BJ; a++) av[a] = X[buf][q][jb + 16 * a + l16];
+      for (int a = 0; a < BJ; a++) av[a] = Xb[q][jb + 16 * a + l16];
 #pragma unroll
-      for (int b = 0; b < 4; b++) bv[b] = X[buf][q][ib + 16 * b + l16];
+      for (int b = 0; b < 4; b++) bv[b] = Xb[q][ib + 16 * b + l16];
 #pragma unroll
       for (int a = 0; a < BJ; a++)
 #pragma unroll
@@ -533,6 +532,61 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 128))) 
   // the chunk pipeline, compiled twice: interior tiles (both 128-row panels inside the front, every chunk full) skip
   // the per-column checks; the choice is uniform over the workgroup. Chunks past the last one load zeros (the
   // pipeline's body has no branches; an odd chunk count adds one chunk of zeros)
+  if constexpr (sizeof(T) == 8 && NT == 512 && KC == 16) {
+    // FP64: 16-byte buffer loads straight into LDS (no staging registers, no LDS stores). Lane l of a load brings
+    // tile rows 2l and 2l + 1 of one segment of a k-row (the tile's 128 rows, or its 128 columns), so one load fills
+    // a 1 KB segment; wave wv loads k-rows 2 wv and 2 wv + 1 of a chunk, both segments. The hardware writes zeros
+    // for offsets past the descriptor's range: k-rows past the block take such an offset. The next chunk's loads
+    // are in flight while the current chunk's MFMAs run; each chunk ends with the wave's own loads complete and
+    // one barrier.
+    __shared__ T X0[KC][LD], X1[KC][LD];
+    const int vo0 = (i0 + 2 * lane) * 8, vo1 = (j0 + 2 * lane) * 8;
+    const int wvu = __builtin_amdgcn_readfirstlane(wv);
+    auto issue = [&](T (*Xb)[LD], int c) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int q = 2 * wvu + h, col = c * KC + q;
+        const bool live = col < kw;
+        const int so = __builtin_amdgcn_readfirstlane(live ? col * m * 8 : 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&Xb[q][0], 16,
+                                                 live ? vo0 : OOB, so, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&Xb[q][128], 16,
+                                                 live ? vo1 : OOB, so, 0, 0);
+      }
+    };
+    // s_waitcnt vmcnt(0), other counters unconstrained
+    auto loads_done = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };
+    // a chunk: all of its MFMA operands are read from LDS first, then the next chunk's loads are issued (so no LDS
+    // read waits on them), then the MFMAs run while those loads are in flight; past the block the loads bring zeros
+    auto chunk = [&](const T (*Xb)[LD], T (*Xn)[LD], int cn) {
+      T av[KC / 4][BJ], bv[KC / 4][4];
+#pragma unroll
+      for (int t = 0; t < KC / 4; t++) {
+        const int q = 4 * t + (lane >> 4);
+#pragma unroll
+        for (int a = 0; a < BJ; a++) av[t][a] = Xb[q][jb + 16 * a + l16];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bv[t][b] = Xb[q][ib + 16 * b + l16];
+      }
+      issue(Xn, cn);
+#pragma unroll
+      for (int t = 0; t < KC / 4; t++)
+#pragma unroll
+        for (int a = 0; a < BJ; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[t][a], bv[t][b], acc[a][b]);
+      loads_done();
+      __syncthreads();
+    };
+    issue(X0, 0);
+    loads_done();
+    __syncthreads();
+    for (int c = 0; c < nch; c += 2) {
+      chunk(X0, X1, c + 1);
+      chunk(X1, X0, c + 2);
+    }
+  } else {
+  __shared__ T X[2][KC][LD];
   auto pipeline = [&](auto checked) {
     constexpr bool CHK = decltype(checked)::value;
     auto fetch = [&](T* r, int c) {
@@ -555,17 +609,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 128))) 
     __syncthreads();
     for (int c = 0; c < nch; c += 2) {
       fetch(ra, c + 2);
-      compute(0);
+      compute(X[0]);
       stage(rc, 1);
       __syncthreads();
       fetch(rc, c + 3);
-      compute(1);
+      compute(X[1]);
       stage(ra, 0);
       __syncthreads();
     }
   };
   if (i0 + 128 <= m && j0 + 128 <= m && kw % KC == 0) pipeline(std::false_type{});
   else pipeline(std::true_type{});
+  }
   T* Ls = L + sd.Loff[s];
   T* Us = U + sd.Uoff[s];
   const long long mu = m - k;
