@@ -800,13 +800,15 @@ __device__ __forceinline__ long long x_step_off(const SymDev& sd, int s, int c0)
 }
 
 // Inverse of one SB-column diagonal block, one 64-column strip j per workgroup: X_jj = Linv_j and, going down,
-// X_ij = -Linv_i * sum_{p=j}^{i-1} L_ip X_pj. Thread (r, q): row r of the 64-row tile, columns [16 q, 16 q + 16).
-// task: (s, c0, j, 0). Strips above the diagonal are stored as zeros.
+// X_ij = -Linv_i * sum_{p=j}^{i-1} L_ip X_pj. The 64x64x64 products run on FP64 MFMA: wave w owns rows
+// [16 w, 16 w + 16) of the product and its four 16-column blocks; the operand tiles are staged in LDS, the next
+// pair's loads issued before the current pair's MFMAs. Results go through LDS so that both copies (X column-major,
+// XT row-major) are stored row-coalesced. task: (s, c0, j, 0). Strips above the diagonal are stored as zeros.
 template <typename T>
 __global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks, SymDev sd, const T* __restrict__ L,
                                                 const double* __restrict__ Linv, double* __restrict__ X,
                                                 double* __restrict__ XT) {
-  __shared__ double Ts[NB][NB + 1], Ls[NB][NB + 1];
+  __shared__ double As[NB][NB + 1], Bs[NB][NB + 1];
   const int4 tk = tasks[blockIdx.x];
   const int s = tk.x, c0 = tk.y, j = tk.z;
   const int k = sn_k(sd, s);
@@ -818,6 +820,8 @@ __global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks
   double* Xb = X + x_step_off(sd, s, c0);
   double* XTb = XT + x_step_off(sd, s, c0);
   const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int lane = r, w = q, l16 = lane & 15, kq = lane >> 4;
+  typedef Mfma<double>::v4 v4;
   const double* Ltj = Linv + sd.Ioff[s] + (long long)((c0 + NB * j) / NB) * NB * NB;
   // only entries inside the wd x wd block are stored (the last strip of a step can be narrower than 64)
   for (int cc = 0; cc < 16; cc++) {
@@ -835,53 +839,71 @@ __global__ __launch_bounds__(256) void k_inv_diag(const int4* __restrict__ tasks
     }
   }
   __syncthreads();
+  // acc += As * Bs over the 64 k of the staged tiles (wave w: rows 16 w .., all four column blocks)
+  auto product = [&](v4 (&acc)[4]) {
+#pragma unroll 4
+    for (int kk = 0; kk < NB; kk += 4) {
+      const double a = As[16 * w + l16][kk + kq];
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++) acc[nb] = Mfma<double>::op(a, Bs[kk + kq][16 * nb + l16], acc[nb]);
+    }
+  };
+  // item e of lane l of block nb: entry (16 w + (l >> 4) + 4 e, 16 nb + (l & 15))
+  auto acc_to_lds = [&](const v4 (&acc)[4], double sign, double (*D)[NB + 1]) {
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) D[16 * w + Mfma<double>::row(lane, e)][16 * nb + l16] = sign * acc[nb][e];
+  };
   for (int i = j + 1; i < nsub; i++) {
-    double acc[16];
+    v4 acc[4];
 #pragma unroll
-    for (int cc = 0; cc < 16; cc++) acc[cc] = 0.0;
-    const bool rok = NB * i + r < wd;
+    for (int nb = 0; nb < 4; nb++) acc[nb] = v4{0, 0, 0, 0};
+    // operand tiles of pair p0: As = L_i,p0 (rows of strip i, columns of strip p0), Bs = X_p0,j; thread t holds
+    // entries (t & 63, (t >> 6) + 4 u) of both
+    double xv[16], lv[16];
+    auto fetch = [&](int p0) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int rr = r, cq = q + 4 * u;
+        xv[u] = Xb[(long long)(NB * j + cq) * ld + NB * p0 + rr];  // strips p0 < i < nsub and j < i are full
+        lv[u] = NB * i + rr < wd ? (double)P[(long long)(c0 + NB * p0 + cq) * m + c0 + NB * i + rr] : 0.0;
+      }
+    };
+    fetch(j);
     for (int p0 = j; p0 < i; p0++) {
-      // stage X_pj and the panel tile L_ip (rows of strip i, columns of strip p0) with all loads issued at once
-      double xv[16], lv[16];
 #pragma unroll
-      for (int t = 0; t < 16; t++) {
-        const int idx = threadIdx.x + 256 * t;
-        const int rr = idx & 63, cq = idx >> 6;
-        xv[t] = Xb[(long long)(NB * j + cq) * ld + NB * p0 + rr];  // strips p0 < i < nsub and j < i are full
-        lv[t] = NB * i + rr < wd ? (double)P[(long long)(c0 + NB * p0 + cq) * m + c0 + NB * i + rr] : 0.0;
-      }
-#pragma unroll
-      for (int t = 0; t < 16; t++) {
-        const int idx = threadIdx.x + 256 * t;
-        const int rr = idx & 63, cq = idx >> 6;
-        Ts[rr][cq] = xv[t];
-        Ls[rr][cq] = lv[t];
+      for (int u = 0; u < 16; u++) {
+        As[r][q + 4 * u] = lv[u];
+        Bs[r][q + 4 * u] = xv[u];
       }
       __syncthreads();
-      for (int pp = 0; pp < NB; pp++) {
-        const double a = Ls[r][pp];
-#pragma unroll
-        for (int cc = 0; cc < 16; cc++) acc[cc] += a * Ts[pp][16 * q + cc];
-      }
+      if (p0 + 1 < i) fetch(p0 + 1);
+      product(acc);
       __syncthreads();
     }
-#pragma unroll
-    for (int cc = 0; cc < 16; cc++) Ts[r][16 * q + cc] = acc[cc];
-    __syncthreads();
+    // Bs = sum_p L_ip X_pj, As = Linv_i (Lti[c * NB + r] = Linv_i[r][c]); X_ij = -As * Bs
     const double* Lti = Linv + sd.Ioff[s] + (long long)((c0 + NB * i) / NB) * NB * NB;
+    acc_to_lds(acc, 1.0, Bs);
 #pragma unroll
-    for (int cc = 0; cc < 16; cc++) acc[cc] = 0.0;
-    for (int pp = 0; pp < NB; pp++) {
-      const double a = Lti[pp * NB + r];  // Linv_i[r][pp]
+    for (int u = 0; u < 16; u++) As[r][q + 4 * u] = Lti[(q + 4 * u) * NB + r];
+    __syncthreads();
 #pragma unroll
-      for (int cc = 0; cc < 16; cc++) acc[cc] += a * Ts[pp][16 * q + cc];
+    for (int nb = 0; nb < 4; nb++) acc[nb] = v4{0, 0, 0, 0};
+    product(acc);
+    __syncthreads();
+    acc_to_lds(acc, -1.0, As);
+    __syncthreads();
+    // X (column-major): lanes over rows; XT (row-major): lanes over columns
+    if (NB * i + r < wd) {
+#pragma unroll
+      for (int cc = 0; cc < 16; cc++)
+        Xb[(long long)(NB * j + 16 * q + cc) * ld + NB * i + r] = As[r][16 * q + cc];
     }
-    if (rok) {
-#pragma unroll
-      for (int cc = 0; cc < 16; cc++) {
-        Xb[(long long)(NB * j + 16 * q + cc) * ld + NB * i + r] = -acc[cc];
-        XTb[(long long)(NB * i + r) * ld + NB * j + 16 * q + cc] = -acc[cc];
-      }
+#pragma unroll 4
+    for (int rr = 0; rr < 16; rr++) {
+      const int row = 16 * q + rr;
+      if (NB * i + row < wd) XTb[(long long)(NB * i + row) * ld + NB * j + r] = As[row][r];
     }
     __syncthreads();
   }
