@@ -31,7 +31,8 @@ typedef struct ifea_step_params {
   double u0;                 /* friction regularisation velocity (Forging3DSolver.cs:150) */
   double newton_blend;       /* Forging3DSolver.NewtonBlend */
   double tolerance;          /* beta*|dv|/|v| convergence tolerance */
-  double pcg_tol;            /* PCG relative residual target (1e-7 in the reference, :989) */
+  double pcg_tol;            /* PCG relative residual target (1e-7 in the reference, :984; a looser value makes the
+                                Newton iteration inexact, its own convergence test unchanged) */
   int max_iterations;        /* SimulationConfig3D.MaxIterations */
   int pcg_budget;            /* Forging3DSolver.PcgBudget */
   int quadratic_ls;          /* Forging3DSolver.QuadraticLineSearch */
