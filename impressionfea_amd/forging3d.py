@@ -351,17 +351,20 @@ class Forging3DSolver:
         field and its trend 2 v_n - v_{n-1}, ExtrapolateStart) by the convex functional Pi, the solver's own energy:
         the lowest-energy start is kept, so a poor prediction costs residual-only assemblies and never a worse start
         than the best of the reference's."""
-        from .mlp import trend_start
-        guess = self.warm_start(self)
+        from .mlp import LearnedWarmStart, trend_start
+        trend = trend_start(self)
+        # the network's guess is the trend start plus its correction: the trend computed once for both
+        guess = self.warm_start(self, trend) if isinstance(self.warm_start, LearnedWarmStart) else self.warm_start(self)
         self.prepare(guess)
-        cands = {"learned": guess, "previous": self.vg, "trend": trend_start(self)}
+        cands = {"learned": guess, "previous": self.vg, "trend": trend}
         best, best_e = None, np.inf
         for name, vg in cands.items():
             self.load_guess(vg)
-            e = self.assemble(None, matrix=False, newton=False)[0]
+            e = self.energy()
             if e < best_e:
                 best, best_e = name, e
-        self.load_guess(cands[best])
+        if best != name:              # the device holds the last candidate evaluated
+            self.load_guess(cands[best])
         return best
 
     def first_collapsed(self, p, qmin: float, quality=None) -> int:
@@ -437,6 +440,7 @@ class Forging3DSolver:
         free = ~(((self.contact != FREE) & ~self.release)) & self.mesh.is_surface_node
         cb = cfg.bottom.clearance(p, 0.0)
         ctp = cfg.top.clearance(p, gapT)
+        p_clr = p.copy()
         hit = free & ((cb < 0) | (ctp < 0))
         max_pen = 0.0
         if hit.any():
@@ -468,6 +472,13 @@ class Forging3DSolver:
         self.x = p
         self.contact = cT
         self.gap = gapT
+        # both die clearances of the new positions (the learned start's features read them): those computed above,
+        # recomputed where a node moved since
+        moved = np.any(p != p_clr, axis=1)
+        if moved.any():
+            cb[moved] = cfg.bottom.clearance(p[moved], 0.0)
+            ctp[moved] = cfg.top.clearance(p[moved], gapT)
+        self.clearance = (self.x, cb, ctp)
         self.time += dt
         return ""
 
@@ -584,6 +595,13 @@ class Forging3DSolver:
         # own (host copies) before it computes the friction coefficients from them
         self._state_dirty = True
         return e.value, pw.value, r, val
+
+    def energy(self) -> float:
+        """The functional Pi at the current device velocity (a residual-only Assemble, nothing copied back)."""
+        e, pw = C.c_double(0), C.c_double(0)
+        self._check(N.lib().ifea_step_assemble(self._h, 0, 0, None, C.byref(e), C.byref(pw), None, None), "Assemble")
+        self._state_dirty = True
+        return e.value
 
     def linear_system(self, newton: bool = True):
         """The Newton/Picard system of the current velocity as SolveVelocity forms it (Forging3DSolver.cs:965-1007):

@@ -138,8 +138,12 @@ def node_features(solver):
     vg = solver.vg
     vp = solver.vg_prev if solver.n_converged >= 2 else solver.vg
     contact = (solver.contact != 0).astype(np.float64)
-    cb = cfg.bottom.clearance(x, 0.0)
-    ct = cfg.top.clearance(x, solver.gap)
+    kept = getattr(solver, "clearance", None)
+    if kept is not None and kept[0] is x:
+        cb, ct = kept[1], kept[2]          # computed by the geometry update that produced x
+    else:
+        cb = cfg.bottom.clearance(x, 0.0)
+        ct = cfg.top.clearance(x, solver.gap)
     dist = np.minimum(np.abs(cb), np.abs(ct))
     f = np.concatenate([x / L, vg / cfg.ram, vp / cfg.ram, contact[:, None], (dist / L)[:, None]], axis=1)
     return f.astype(np.float32)
@@ -177,6 +181,6 @@ class LearnedWarmStart:
         os.makedirs(os.path.dirname(path), exist_ok=True)
         np.savez(path, params=self.dev.get_params(), scale=np.float64(self.scale))
 
-    def __call__(self, solver):
+    def __call__(self, solver, trend=None):
         dv = self.dev.forward(node_features(solver)).astype(np.float64)
-        return trend_start(solver) + self.scale * solver.cfg.ram * dv
+        return (trend_start(solver) if trend is None else trend) + self.scale * solver.cfg.ram * dv
