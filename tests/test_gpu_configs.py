@@ -24,23 +24,27 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def test_hexdie_two_increments_against_oracle_loads():
-    """The first two increments of the hex-die workload (120,185 nodes, 677,376 tets, 360,555 DOF), plain FP64 and
-    mixed precision, against the ram loads of the FP64 CPU oracle with MKL PARDISO committed in
-    tests/golden/hexdie_loads.json (written by tests/golden/make_hexdie_loads.py on the same seeded mesh): FP64 within
-    1e-6, mixed within 1e-3 (the 0.1 % bar)."""
+    """The first two increments of the hex-die workload (120,185 nodes, 677,376 tets, 360,555 DOF), plain FP64, mixed
+    precision, and the benchmark's mixed method (mixed precision, learned start judged against the reference's
+    starts, adaptive first solve, PCG stopped at 1e-4), against the ram loads of the FP64 CPU oracle with MKL PARDISO
+    committed in tests/golden/hexdie_loads.json (written by tests/golden/make_hexdie_loads.py on the same seeded
+    mesh): FP64 within 1e-6, the mixed ones within 1e-3 (the 0.1 % bar)."""
+    from impressionfea_amd import warmstart
     gold = json.load(open(os.path.join(HERE, "golden", "hexdie_loads.json")))
     wl = hexdie_production()
     x, t, _, _ = wl.mesh()
     assert x.shape[0] == gold["nodes"] and t.shape[0] == gold["tets"]
     assert float(np.abs(x).sum()) == pytest.approx(gold["x_checksum"], rel=1e-15)
     ref = [r["load"] for r in gold["increments"]]
-    for precision, bar in ((0, 1e-6), (1, 1e-3)):
+    bench_mixed = dict(warm_start=warmstart.load_default(), first_solve_policy=True, pcg_tol=1e-4)
+    for precision, bar, kw in ((0, 1e-6, {}), (1, 1e-3, {}), (1, 1e-3, bench_mixed)):
         t0 = time.time()
-        s = Forging3DSolver.from_workload(wl, precision=precision)
+        s = Forging3DSolver.from_workload(wl, precision=precision, **kw)
         for k in range(2):
             r = s.step()
             err = abs(r.load / ref[k] - 1)
-            print(f"precision {precision} increment {k}: load {r.load:.9e} N, oracle {ref[k]:.9e} N, rel err "
+            print(f"precision {precision}{' (benchmark mixed method)' if kw else ''} increment {k}: load "
+                  f"{r.load:.9e} N, oracle {ref[k]:.9e} N, rel err "
                   f"{err:.2e}, Newton {r.iterations}, factorizations {r.stats['factorizations']}, "
                   f"fallbacks {r.stats['fp64_fallbacks']} ({time.time() - t0:.1f} s)", flush=True)
             assert r.converged
