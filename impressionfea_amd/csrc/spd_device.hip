@@ -533,35 +533,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 128))) 
   // the per-column checks; the choice is uniform over the workgroup. Chunks past the last one load zeros (the
   // pipeline's body has no branches; an odd chunk count adds one chunk of zeros)
   if constexpr (sizeof(T) == 8 && NT == 512 && KC == 16) {
-    // FP64: 16-byte buffer loads straight into LDS (no staging registers, no LDS stores). Lane l of a load brings
-    // tile rows 2l and 2l + 1 of one segment of a k-row (the tile's 128 rows, or its 128 columns), so one load fills
-    // a 1 KB segment; wave wv loads k-rows 2 wv and 2 wv + 1 of a chunk, both segments. The hardware writes zeros
-    // for offsets past the descriptor's range: k-rows past the block take such an offset. The next chunk's loads
-    // are in flight while the current chunk's MFMAs run; each chunk ends with the wave's own loads complete and
-    // one barrier.
-    __shared__ T X0[KC][LD], X1[KC][LD];
+    // FP64: 16-byte buffer loads straight into LDS (no staging registers, no LDS stores), through four LDS stages of
+    // 8 k-rows. Lane l of a load brings tile rows 2l and 2l + 1 of one segment of a k-row (the tile's 128 rows, or its
+    // 128 columns), so one load fills a 1 KB segment; wave wv loads k-row wv of a stage, both segments. The hardware
+    // writes zeros for offsets past the descriptor's range: k-rows past the block take such an offset. A stage's
+    // operands are read from LDS first, then the loads of the stage three ahead are issued, then its MFMAs run; each
+    // stage ends with the wave's loads of the next stage complete (the two stages after it may still be in flight)
+    // and one barrier.
+    constexpr int KD = 8;
+    const int ncd = (kw + KD - 1) / KD;
+    __shared__ T X0[KD][LD], X1[KD][LD], X2[KD][LD], X3[KD][LD];
     const int vo0 = (i0 + 2 * lane) * 8, vo1 = (j0 + 2 * lane) * 8;
     const int wvu = __builtin_amdgcn_readfirstlane(wv);
     auto issue = [&](T (*Xb)[LD], int c) {
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int q = 2 * wvu + h, col = c * KC + q;
-        const bool live = col < kw;
-        const int so = __builtin_amdgcn_readfirstlane(live ? col * m * 8 : 0);
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&Xb[q][0], 16,
-                                                 live ? vo0 : OOB, so, 0, 0);
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&Xb[q][128], 16,
-                                                 live ? vo1 : OOB, so, 0, 0);
-      }
+      const int col = c * KD + wvu;
+      const bool live = col < kw;
+      const int so = __builtin_amdgcn_readfirstlane(live ? col * m * 8 : 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&Xb[wvu][0], 16,
+                                               live ? vo0 : OOB, so, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&Xb[wvu][128], 16,
+                                               live ? vo1 : OOB, so, 0, 0);
     };
-    // s_waitcnt vmcnt(0), other counters unconstrained
-    auto loads_done = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };
-    // a chunk: all of its MFMA operands are read from LDS first, then the next chunk's loads are issued (so no LDS
-    // read waits on them), then the MFMAs run while those loads are in flight; past the block the loads bring zeros
+    // the wave's loads done but the last four (two stages of two), its LDS reads done, then the barrier; written out
+    // because a fenced barrier would wait for every load in flight
+    auto stage_ready = [&]() { asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
     auto chunk = [&](const T (*Xb)[LD], T (*Xn)[LD], int cn) {
-      T av[KC / 4][BJ], bv[KC / 4][4];
+      T av[KD / 4][BJ], bv[KD / 4][4];
 #pragma unroll
-      for (int t = 0; t < KC / 4; t++) {
+      for (int t = 0; t < KD / 4; t++) {
         const int q = 4 * t + (lane >> 4);
 #pragma unroll
         for (int a = 0; a < BJ; a++) av[t][a] = Xb[q][jb + 16 * a + l16];
@@ -570,20 +569,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 128))) 
       }
       issue(Xn, cn);
 #pragma unroll
-      for (int t = 0; t < KC / 4; t++)
+      for (int t = 0; t < KD / 4; t++)
 #pragma unroll
         for (int a = 0; a < BJ; a++)
 #pragma unroll
           for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::op(av[t][a], bv[t][b], acc[a][b]);
-      loads_done();
-      __syncthreads();
+      stage_ready();
     };
     issue(X0, 0);
-    loads_done();
-    __syncthreads();
-    for (int c = 0; c < nch; c += 2) {
-      chunk(X0, X1, c + 1);
-      chunk(X1, X0, c + 2);
+    issue(X1, 1);
+    issue(X2, 2);
+    stage_ready();
+    for (int c = 0; c < ncd; c += 4) {
+      chunk(X0, X3, c + 3);
+      chunk(X1, X0, c + 4);
+      chunk(X2, X1, c + 5);
+      chunk(X3, X2, c + 6);
     }
   } else {
   __shared__ T X[2][KC][LD];
