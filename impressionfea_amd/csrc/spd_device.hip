@@ -452,10 +452,11 @@ __global__ __launch_bounds__(256) void k_syrk(const int4* __restrict__ tasks, in
 
 
 // Outer and mid trailing updates with 128x128 output tiles: NT/64 waves, two along the rows (64 each) and
-// NT/128 along the columns, each wave a block of 16x16 MFMA tiles (64x32 with 8 waves). X is streamed in 16-column
-// chunks through two LDS slots, each chunk's 256 rows (the tile's 128 rows, then its 128 columns) in k-major order.
-// The loads run two chunks ahead in registers: those of chunk c + 2 are issued before the MFMAs of chunk c and
-// written to LDS after the MFMAs of chunk c + 1, one barrier per chunk. They are buffer loads: the front's block
+// NT/128 along the columns, each wave a block of 16x16 MFMA tiles (64x32 with 8 waves). X is streamed through LDS in
+// k-major chunks of 256 rows (the tile's 128 rows, then its 128 columns). FP64: 8-column stages through four LDS
+// slots, filled by loads straight into LDS three stages ahead (below). FP32: 16-column chunks through two slots, the
+// loads two chunks ahead in registers: those of chunk c + 2 are issued before the MFMAs of chunk c and written to LDS
+// after the MFMAs of chunk c + 1, one barrier per chunk. Both are buffer loads: the front's block
 // column kb is the descriptor base, a thread's row is its one 32-bit VGPR offset and the chunk's column a wave-uniform
 // SGPR offset, so no load needs a 64-bit address; a load that must read zero (a row beyond the front, a column
 // beyond the block) takes an offset past the descriptor's range, for which the hardware returns 0. Register use stays
