@@ -43,12 +43,12 @@ def log(*a):
 # The arms: (factor precision, start, adaptive first-solve policy, PCG tolerance). "fp64" is the reference's algorithm
 # on the GPU (PCG to 1e-7, Forging3DSolver.cs:984); the two middle arms separate the effect of the start from that of
 # the precision; "mixed" is the full method, whose Newton iteration is inexact: PCG stops at a relative residual of
-# 1e-4 (Newton's convergence test on the velocity step is the reference's).
+# 1e-2 (Newton's convergence test on the velocity step is the reference's).
 ARMS = {
     "fp64": dict(precision=0, start="previous", first_solve=False, pcg_tol=1e-7),
     "fp64_trend": dict(precision=0, start="trend", first_solve=False, pcg_tol=1e-7),
     "mixed_previous": dict(precision=1, start="previous", first_solve=False, pcg_tol=1e-7),
-    "mixed": dict(precision=1, start="learned", first_solve=True, pcg_tol=1e-4),
+    "mixed": dict(precision=1, start="learned", first_solve=True, pcg_tol=1e-2),
 }
 
 

@@ -412,14 +412,14 @@ def test_mixed_precision_load_within_tenth_percent():
 
 
 def test_inexact_newton_load_within_tenth_percent():
-    """The benchmark's mixed method (FP32 sweeps, trend start, adaptive first solve, PCG stopped at 1e-4 relative
+    """The benchmark's mixed method (FP32 sweeps, trend start, adaptive first solve, PCG stopped at 1e-2 relative
     residual instead of the reference's 1e-7): every increment converges by the reference's test on the velocity step,
     and the ram load stays within 0.1 % of plain FP64 with PCG at 1e-7."""
     from dataclasses import replace
     wl = upset_small(nxy=8, nz=8)
     a = Forging3DSolver.from_workload(wl, precision=0)
     wt = replace(wl, cfg=replace(wl.cfg, extrapolate_start=True))
-    b = Forging3DSolver.from_workload(wt, precision=1, first_solve_policy=True, pcg_tol=1e-4)
+    b = Forging3DSolver.from_workload(wt, precision=1, first_solve_policy=True, pcg_tol=1e-2)
     for k in range(4):
         ra, rb = a.step(), b.step()
         print(f"increment {k}: load {rb.load:.9e} vs {ra.load:.9e} (rel {abs(rb.load / ra.load - 1):.1e}), Newton "
