@@ -346,19 +346,26 @@ def test_increments_parity_fp64(oracle_lib):
 
 
 def test_increments_parity_config1(oracle_lib):
-    """The small upset exactly as configured (10x10x10 cells, penalty 1e6, nonlinear tolerance 1e-6): three increments
-    of the GPU FP64 path against the oracle with MKL PARDISO (or its own Cholesky), ram load within 1e-6."""
+    """The small upset exactly as configured (10x10x10 cells, penalty 1e6, nonlinear tolerance 1e-6), all 20
+    increments: the GPU FP64 path and the mixed method of the bench (FP32 sweeps, judged learned start, adaptive first
+    solve, PCG to 1e-2) against the oracle with MKL PARDISO (or its own Cholesky). Ram load within 1e-6 (FP64) and
+    1e-5 (mixed; the accuracy bar of the mixed path is 1e-3) at every increment."""
     import oracle as O
+    from impressionfea_amd import warmstart
     wl = upset_small()
     x, t, ox, oy = kuhn_cylinder(wl.diameter, wl.height, wl.nxy, wl.nz, seed=0)
     o = O.Oracle(wl.cfg, x, t, ox, oy, wl.gap0, linear=1 if O.pardiso_available() else 0)
     o.initial(0.0)
     s = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0)
     s.initial(0.0)
-    for k in range(3):
-        ro, rs = o.step(), s.step()
-        assert rs.converged
-        assert rs.load == pytest.approx(ro["load"], rel=1e-6)
+    mx = Forging3DSolver(wl.cfg, x, t, ox, oy, wl.gap0, precision=1, warm_start=warmstart.load_default(),
+                         first_solve_policy=True, pcg_tol=1e-2)
+    mx.initial(0.0)
+    for k in range(wl.increments):
+        ro, rs, rm = o.step(), s.step(), mx.step()
+        assert rs.converged and rm.converged, k
+        assert rs.load == pytest.approx(ro["load"], rel=1e-6), k
+        assert rm.load == pytest.approx(ro["load"], rel=1e-5), k
 
 
 def test_quality_check_matches_oracle(oracle_lib):
